@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the filament BD step on one MI355X.
+
+Metric: segment-steps/s = bonds x steps / wall-seconds, one step being one
+reference i_step (two per Delta t).  Default workload: the dense single
+system, 20,000 filaments x 32 sites (620,000 bonds), persistence length
+50 sigma, periodic cube at volume fraction 0.05, WCA eps=1 cutoff
+2^(1/6) sigma, dt=1e-4; FP32 pair forces, FP64 tension solve and positions,
+the step replayed from a hipGraph.  Synthetic initial state (reference-style
+random insertion), state resident in HBM before the timed region.
+
+Printed on one JSON line: the metric, the pair kernel's roofline (HIP-event
+timing of the same kernel), the plain FP64 GPU path on the same workload and
+the CPU oracle (OpenMP, the reference's decomposition) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def box_for_fraction(n_sites, bond_length, diameter, phi):
+    n_sites = np.asarray(n_sites, dtype=float)
+    L = (n_sites - 1) * bond_length
+    v = np.sum(0.25 * np.pi * diameter ** 2 * L + np.pi / 6.0 * diameter ** 3)
+    return 0.5 * (v / phi) ** (1.0 / 3.0)
+
+
+def workload(name, sc):
+    """(params, filaments, description dict)."""
+    if name == "c1":
+        nf, ns = 20000, 32
+        R = box_for_fraction([ns] * nf, 1.0, 1.0, 0.05)
+        p = sc.params_init(R, delta=1e-4, min_cell_length=1.0, seed=20240001)
+        fil = sc.generate_filaments(p, [ns] * nf, float(ns - 1), 50.0, 1.0, 2.0, seed=11)
+        desc = {"workload": "dense single system: 20000 filaments x 32 sites, lp 50 sigma, "
+                            "volume fraction 0.05, periodic cube, WCA, dt 1e-4",
+                "n_filaments": nf, "sites_per_filament": ns, "persistence_length": 50.0,
+                "volume_fraction": 0.05, "box_side": 2 * R}
+    elif name == "c2":
+        rng = np.random.default_rng(20240002)
+        nf = 50000
+        n = np.rint(rng.lognormal(np.log(24.0), 0.8, nf)).astype(np.int64)
+        n = np.clip(n, 8, 256)
+        lp = rng.uniform(10.0, 1000.0, nf)
+        R = box_for_fraction(n, 1.0, 1.0, 0.08)
+        p = sc.params_init(R, delta=1e-4, min_cell_length=1.0, seed=20240003)
+        fil = sc.generate_filaments(p, n, (n - 1).astype(float), lp, 1.0, 2.0, seed=12)
+        desc = {"workload": "uneven lengths: 50000 filaments, sites lognormal(median 24, "
+                            "sigma_ln 0.8) clipped to [8,256], lp U[10,1000] sigma, "
+                            "volume fraction 0.08, periodic cube, WCA, dt 1e-4",
+                "n_filaments": nf, "volume_fraction": 0.08, "box_side": 2 * R}
+    elif name == "c0":
+        p, spec = sc.params_from_yaml(os.path.join(ROOT, "simcore_amd", "configs",
+                                                   "c0_plain_fp64.yaml"))
+        nb, _ = sc.filament_layout(spec["length"], spec["min_bond_length"],
+                                   spec["min_length"], spec["max_length"])
+        fil = sc.generate_filaments(p, [nb + 1] * int(spec["num"]), spec["length"],
+                                    spec["persistence_length"], spec["diameter"],
+                                    spec["friction_ratio"], seed=7)
+        desc = {"workload": "parity system: 64 filaments x 20 sites, lp 100 sigma, box 40 "
+                            "sigma, WCA, dt 1e-4", "n_filaments": 64, "sites_per_filament": 20}
+    else:
+        raise ValueError(name)
+    desc["bonds"] = int(fil.nb)
+    return p, fil, desc
+
+
+def pair_bytes(precision, nb, nc):
+    """Algorithmic HBM bytes of one pair-kernel launch: each bond's state
+    read once, each candidate's pair indices and sticky force slot read and
+    its contact parameters written (the conditional slot write of in-range
+    pairs is not counted)."""
+    if precision == "mixed":
+        return nb * (24 + 16) + nc * (8 + 16 + 8)
+    return nb * (24 + 24 + 8) + nc * (8 + 32 + 16)
+
+
+def timed_steps(sysobj, steps):
+    t0 = time.perf_counter()
+    sysobj.step(steps)  # returns after the stream has drained
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--fp64-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(systems shard by ensemble member; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import simcore_amd as sc
+
+    log(f"[bench] building workload {args.workload}")
+    p, fil, desc = workload(args.workload, sc)
+    log(f"[bench] {fil.nf} filaments, {fil.nb} bonds")
+    dev = sc.FilamentSystem(p, fil, precision=args.precision, use_graph=True)
+    dev.step(args.warmup)
+    log(f"[bench] warmup {args.warmup} steps done, counters {dev.counters()}")
+    c0 = dev.counters()
+    wall = timed_steps(dev, args.steps)
+    c1 = dev.counters()
+    value = fil.nb * args.steps / wall
+    log(f"[bench] timed {args.steps} steps in {wall:.4f} s -> {value:.4e} segment-steps/s, "
+        f"rebuilds {c1['n_rebuilds'] - c0['n_rebuilds']}")
+
+    # per-kernel HIP-event timing of the same kernels (eager launches)
+    dev.profile(True)
+    dev.step(args.profile_steps)
+    prof = dev.profile_read()
+    dev.profile(False)
+    nc = dev.counters()["n_candidates"]
+    pair_ms = prof["pair"][0]
+    bytes_pair = pair_bytes(args.precision, fil.nb, nc)
+    achieved = bytes_pair / (pair_ms * 1e-3) / 1e9 if pair_ms > 0 else 0.0
+    log(f"[bench] profile: {json.dumps(prof)} nc={nc}")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_pair32" if args.precision == "mixed" else "k_pair64",
+                "kernel_avg_ms": pair_ms, "algorithmic_bytes": bytes_pair,
+                "candidates": nc, "per_step_ms": {k: v[0] for k, v in prof.items()}}
+    dev.close()
+
+    extra = {}
+    if not args.no_fp64_baseline and args.precision != "fp64":
+        d64 = sc.FilamentSystem(p, fil, precision="fp64", use_graph=False)
+        d64.step(10)
+        w64 = timed_steps(d64, args.fp64_steps)
+        extra["plain_fp64_gpu"] = {"value": fil.nb * args.fp64_steps / w64,
+                                   "unit": "segment-steps/s", "steps": args.fp64_steps,
+                                   "note": "one system, FP64 pair path, eager launches"}
+        log(f"[bench] plain fp64: {extra['plain_fp64_gpu']['value']:.4e}")
+        d64.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle.oracle_py import OracleSystem
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        if threads <= 0:
+            threads = len(os.sched_getaffinity(0))
+        ora = OracleSystem(p, fil, n_threads=threads)
+        ora.step(10)
+        t0 = time.perf_counter()
+        rc = ora.step(args.cpu_steps)
+        tcpu = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed: {rc}")
+        cpu = {"value": fil.nb * args.cpu_steps / tcpu, "unit": "segment-steps/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same workload after 10 warm-up "
+                         f"steps (incl. the initial cell-list build), {tcpu:.2f} s wall, "
+                         f"OpenMP static chunks as in the reference"}
+        log(f"[bench] cpu oracle: {cpu['value']:.4e} on {threads} threads")
+        ora.close()
+
+    out = {
+        "metric": "segment-steps/s", "value": value, "unit": "segment-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": dict(desc, precision=args.precision, graph=True),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

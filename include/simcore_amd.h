@@ -1,0 +1,189 @@
+/*
+ * simcore_amd.h -- C ABI of the MI355X (gfx950) filament Brownian-dynamics
+ * hot path.  Plain pointers and sizes only; every call returns an int status
+ * (SC_OK = 0, negative = error class) and never throws.  sc_last_error()
+ * holds the message of the last failure on a handle.
+ *
+ * A handle owns one system's device state on one HIP stream.  Caller arrays
+ * are borrowed for the duration of a call (copied in or out).  One host
+ * thread per handle at a time.
+ *
+ * Site and bond arrays are SoA: x block, y block, z block (3*N doubles).
+ * Sites of filament f are [site_off[f], site_off[f+1]); its bonds are
+ * [site_off[f]-f, site_off[f+1]-f-1).
+ */
+#ifndef SIMCORE_AMD_H
+#define SIMCORE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SC_ABI_VERSION 1
+
+/* status codes */
+#define SC_OK 0
+#define SC_ERR_ARG (-1)        /* bad argument / unsupported configuration   */
+#define SC_ERR_HIP (-2)        /* HIP runtime failure                        */
+#define SC_ERR_STATE (-3)      /* call out of order (e.g. step before load)  */
+#define SC_ERR_UNPHYSICAL (-4) /* dr > 10000: the reference throws
+                                  runtime_error("Unphysical result")
+                                  (include/simcore/species.hpp:114-118)     */
+#define SC_ERR_NOMEM (-5)      /* device memory / admission failure          */
+#define SC_ERR_PARSE (-6)      /* parameter file could not be parsed         */
+
+/* arithmetic of the pair path */
+#define SC_PREC_FP64 0  /* everything FP64, bit-compatible with the CPU path */
+#define SC_PREC_MIXED 1 /* FP32 distance/WCA math, FP64 minimum image,
+                           near-parallel branch, tensions and positions      */
+
+typedef struct sc_system *sc_handle;
+
+/* System parameters.  Replaces the system_parameters fields the hot path
+ * reads (include/simcore/parameters.hpp:157-235) plus the derived cell-list
+ * and unit-cell quantities (src/cell_list.cpp:9-64, src/space.cpp:85-138). */
+typedef struct sc_params {
+  int n_dim;               /* must be 3                                    */
+  int n_periodic;          /* must be 3 (fully periodic box)               */
+  double unit_cell[9];     /* space_struct.unit_cell (orthorhombic)        */
+  double unit_cell_inv[9]; /* space_struct.unit_cell_inv                   */
+  double system_radius;    /* params.system_radius                         */
+  double min_cell_length;  /* CellList::_min_cell_length_ (max over the
+                              SetMinCellLength calls)                       */
+  double delta;            /* params.delta (one Delta t = two steps)       */
+  double wca_eps;          /* params.wca_eps                               */
+  double wca_sig;          /* params.wca_sig                               */
+  double f_cutoff;         /* params.f_cutoff                              */
+  int stoch_flag;          /* params.stoch_flag                            */
+  int precision;           /* SC_PREC_FP64 or SC_PREC_MIXED                */
+  int use_graph;           /* capture the step in a hipGraph               */
+  int reserved;
+  uint64_t seed;           /* thermal-noise stream key                     */
+  double capacity_factor;  /* initial candidate capacity / bonds (0: 4)    */
+} sc_params;
+
+/* Per-filament parameters (Filament::SetParameters / SetDiffusion,
+ * src/filament.cpp:6-84, 320-348). */
+typedef struct sc_filament_params {
+  double bond_length;      /* Filament::bond_length_                       */
+  double persistence_length; /* bending_stiffness_ (kT = 1)               */
+  double gamma_par;        /* friction_par_                                */
+  double gamma_perp;       /* friction_perp_                               */
+  double diameter;         /* diameter_                                    */
+} sc_filament_params;
+
+int sc_abi_version(void);
+
+/* Replaces Simulation::InitSimulation's space / interaction-manager /
+ * cell-list set-up: Space::Init (src/space.cpp:15-83),
+ * InteractionManager::Init + InitInteractions
+ * (src/interaction_manager.cpp:4-56), CellList::Init (src/cell_list.cpp:
+ * 17-64), WCAPotential::Init (include/simcore/wca_potential.hpp:34-48). */
+int sc_create(const sc_params *params, sc_handle *out);
+
+/* Replaces the state FilamentSpecies holds after Simulation::InsertSpecies
+ * (src/simulation.cpp:261-405): Filament::InsertFilament's
+ * UpdateBondPositions + UpdatePrevPositions (src/filament.cpp:225-249) and
+ * the final ResetCellList + CheckUpdateObjects -> UpdateInteractions +
+ * ZeroDrTot (src/simulation.cpp:400-403).  r is SoA (3*Ns doubles). */
+int sc_load_filaments(sc_handle h, int32_t n_filaments,
+                      const int32_t *site_off, const double *r,
+                      const sc_filament_params *fil);
+
+/* Replaces n iterations of Simulation::RunSimulation's step body
+ * (src/simulation.cpp:37-62): ZeroForces, InteractionManager::Interact
+ * (src/interaction_manager.cpp:63-95) and FilamentSpecies::UpdatePositions
+ * (src/filament_species.cpp:59-111), honouring the midstep parity. */
+int sc_step(sc_handle h, int32_t n_steps);
+
+/* Object::GetPosition / GetPrevPosition of sites (include/simcore/
+ * object.hpp:82-88); f_rand is Site::random_force_ (src/site.cpp:103-107).
+ * Any pointer may be NULL. */
+int sc_get_sites(sc_handle h, double *r, double *r_prev, double *f_rand);
+/* Bond state after Bond::ReInit (src/bond.cpp:22-37): centre, scaled
+ * position, unit orientation, length.  Any pointer may be NULL. */
+int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u,
+                 double *len);
+/* Per-bond force/torque after ApplyPairInteractions of the last step
+ * (src/interaction_manager.cpp:500-517). */
+int sc_get_bond_forces(sc_handle h, double *f, double *t);
+/* Filament::tensions_ of the last CalculateTensions, one per bond
+ * (src/filament.cpp:778-838). */
+int sc_get_tensions(sc_handle h, double *tension);
+/* CellList::FindCellCoords of every bond at the last rebuild, packed
+ * (x*n + y)*n + z (src/cell_list.cpp:139-157). */
+int sc_get_cell_keys(sc_handle h, int32_t *keys);
+/* InteractionManager::pair_interactions_ (live list, reference order):
+ * obj1/obj2 bond indices.  *n receives the count; arrays may be NULL. */
+int sc_get_candidates(sc_handle h, int32_t *b1, int32_t *b2, size_t cap,
+                      size_t *n);
+/* Interaction::force of each live candidate (sticky slot), 3*n doubles. */
+int sc_get_slot_forces(sc_handle h, double *f, size_t cap);
+/* Displacement bookkeeping for the rebuild rule (src/mesh.cpp:533-556,
+ * src/object.cpp:274-287): per-site accumulator. */
+int sc_get_dr_state(sc_handle h, double *acc);
+/* Counters: steps done, rebuilds done, live candidates, cells per side. */
+int sc_get_counters(sc_handle h, int64_t *i_step, int64_t *n_rebuilds,
+                    int64_t *n_candidates, int32_t *n_cells_1d);
+/* Overwrite site positions (host-side perturbation, e.g. to exercise the
+ * unphysical-dr error path).  Bond state is re-initialised from them. */
+int sc_set_sites(sc_handle h, const double *r);
+
+/* Device bytes this handle holds. */
+int sc_device_bytes(sc_handle h, size_t *bytes);
+/* Average per-launch duration (ms) of each kernel family recorded with HIP
+ * events on the handle's stream while profiling was enabled; families:
+ * 0 pair, 1 reduce, 2 integrate, 3 rebuild (whole stage). */
+int sc_profile_enable(sc_handle h, int on);
+int sc_profile_read(sc_handle h, double *avg_ms, int64_t *launches,
+                    int n_families);
+
+const char *sc_last_error(sc_handle h);
+int sc_destroy(sc_handle h);
+
+/* ---- host layer: parameter files and system generation ------------------- */
+
+/* Reads a reference-style YAML parameter file (a subset of YAML: maps,
+ * sequences of maps, flow lists, scalars, comments) merged over the
+ * reference defaults (config/default_config.yaml) and fills sc_params for
+ * the first filament species (src/params_parser.cpp:4-86,
+ * src/simulation.cpp:248-251, src/interaction_manager.cpp:46-54).
+ * Species values come back through `spec` (9 doubles, may be NULL): num,
+ * length, persistence_length, diameter, min_bond_length, friction_ratio,
+ * n_steps, min_length, max_length.  Only 3D, fully periodic, boundary 0,
+ * WCA systems are accepted (SC_ERR_ARG otherwise). */
+int sc_params_from_yaml(const char *path, sc_params *out, double *spec);
+
+/* Fills sc_params with the reference defaults (config/default_config.yaml)
+ * for a fully periodic cubic box of side 2*system_radius; the unit cell and
+ * its inverse follow Space::InitUnitCell + invert_sym_3d_matrix
+ * (src/space.cpp:85-138, src/linear_algebra.cpp:52-64). */
+int sc_params_init(sc_params *out, double system_radius);
+
+/* Filament::InitFilamentLength, static-length branch
+ * (src/filament.cpp:104-147): bonds and bond length of a filament. */
+int sc_filament_layout(double length, double min_bond_length,
+                       double min_length, double max_length,
+                       int32_t *n_bonds, double *bond_length);
+
+/* Generates filament site positions in the reference's "random" insertion
+ * style (Filament::InsertFilament / GenerateProbableOrientation,
+ * src/filament.cpp:225-249, 350-383) with overlap rejection
+ * (InteractionManager::CheckOverlap, src/interaction_manager.cpp:561-571),
+ * using the build's own RNG.  Per filament: site count, persistence length,
+ * diameter, total length.  Two-call pattern: with r == NULL only
+ * *n_sites_total is written. */
+int sc_generate_filaments(const sc_params *params, int32_t n_filaments,
+                          const int32_t *n_sites, const double *length,
+                          const double *persistence_length,
+                          const double *diameter, double friction_ratio,
+                          uint64_t seed, int32_t *site_off, double *r,
+                          sc_filament_params *fil, int64_t *n_sites_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+"""ctypes binding of libsimcore_amd.so (declared in include/simcore_amd.h).
+
+The library is loaded from the package directory only.  There is no
+fallback: if the shared object is missing, importing this module raises.
+"""
+import ctypes
+import os
+from ctypes import (POINTER, c_char_p, c_double, c_int, c_int32, c_int64,
+                    c_size_t, c_uint64, c_void_p)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsimcore_amd.so")
+
+SC_OK = 0
+SC_ERR_ARG = -1
+SC_ERR_HIP = -2
+SC_ERR_STATE = -3
+SC_ERR_UNPHYSICAL = -4
+SC_ERR_NOMEM = -5
+SC_ERR_PARSE = -6
+SC_PREC_FP64 = 0
+SC_PREC_MIXED = 1
+
+# every symbol include/simcore_amd.h declares
+EXPORTS = [
+    "sc_abi_version", "sc_create", "sc_load_filaments", "sc_step",
+    "sc_get_sites", "sc_get_bonds", "sc_get_bond_forces", "sc_get_tensions",
+    "sc_get_cell_keys", "sc_get_candidates", "sc_get_slot_forces",
+    "sc_get_dr_state", "sc_get_counters", "sc_set_sites", "sc_device_bytes",
+    "sc_profile_enable", "sc_profile_read", "sc_last_error", "sc_destroy",
+    "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
+    "sc_generate_filaments",
+]
+
+
+class ScParams(ctypes.Structure):
+    _fields_ = [
+        ("n_dim", c_int), ("n_periodic", c_int),
+        ("unit_cell", c_double * 9), ("unit_cell_inv", c_double * 9),
+        ("system_radius", c_double), ("min_cell_length", c_double),
+        ("delta", c_double), ("wca_eps", c_double), ("wca_sig", c_double),
+        ("f_cutoff", c_double), ("stoch_flag", c_int), ("precision", c_int),
+        ("use_graph", c_int), ("reserved", c_int), ("seed", c_uint64),
+        ("capacity_factor", c_double),
+    ]
+
+
+class ScFilamentParams(ctypes.Structure):
+    _fields_ = [
+        ("bond_length", c_double), ("persistence_length", c_double),
+        ("gamma_par", c_double), ("gamma_perp", c_double),
+        ("diameter", c_double),
+    ]
+
+
+class SimcoreError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"simcore_amd error {code}: {msg}")
+        self.code = code
+
+
+class UnphysicalResult(SimcoreError):
+    """The reference throws std::runtime_error("Unphysical result") when a
+    filament's tracked displacement exceeds 10000 (species.hpp:114-118)."""
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` "
+            "(make -C simcore_amd/csrc)")
+    lib = ctypes.CDLL(LIB_PATH)
+    H = c_void_p
+    dp, ip, lp = POINTER(c_double), POINTER(c_int32), POINTER(c_int64)
+    sig = {
+        "sc_abi_version": ([], c_int),
+        "sc_create": ([POINTER(ScParams), POINTER(H)], c_int),
+        "sc_load_filaments": ([H, c_int32, ip, dp, POINTER(ScFilamentParams)], c_int),
+        "sc_step": ([H, c_int32], c_int),
+        "sc_get_sites": ([H, dp, dp, dp], c_int),
+        "sc_get_bonds": ([H, dp, dp, dp, dp], c_int),
+        "sc_get_bond_forces": ([H, dp, dp], c_int),
+        "sc_get_tensions": ([H, dp], c_int),
+        "sc_get_cell_keys": ([H, ip], c_int),
+        "sc_get_candidates": ([H, ip, ip, c_size_t, POINTER(c_size_t)], c_int),
+        "sc_get_slot_forces": ([H, dp, c_size_t], c_int),
+        "sc_get_dr_state": ([H, dp], c_int),
+        "sc_get_counters": ([H, lp, lp, lp, ip], c_int),
+        "sc_set_sites": ([H, dp], c_int),
+        "sc_device_bytes": ([H, POINTER(c_size_t)], c_int),
+        "sc_profile_enable": ([H, c_int], c_int),
+        "sc_profile_read": ([H, dp, lp, c_int], c_int),
+        "sc_last_error": ([H], c_char_p),
+        "sc_destroy": ([H], c_int),
+        "sc_params_from_yaml": ([c_char_p, POINTER(ScParams), dp], c_int),
+        "sc_params_init": ([POINTER(ScParams), c_double], c_int),
+        "sc_filament_layout": ([c_double, c_double, c_double, c_double, ip, dp], c_int),
+        "sc_generate_filaments": ([POINTER(ScParams), c_int32, ip, dp, dp, dp, c_double,
+                                   c_uint64, ip, dp, POINTER(ScFilamentParams), lp], c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+lib = _load()

@@ -1,0 +1,627 @@
+// sc_api.cpp -- C ABI of the device path: handle lifetime, HBM allocation,
+// step orchestration (eager or hipGraph), candidate-capacity growth, taps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sc_internal.h"
+
+using sc::Ctl;
+using sc::Dev;
+using sc::Geo;
+
+namespace {
+
+struct Bin {
+  int maxn = 0;
+  int count = 0;
+  int32_t *list = nullptr;
+};
+
+constexpr int kFamilies = 4;  // pair, reduce, integrate, rebuild
+
+}  // namespace
+
+struct sc_system {
+  sc_params p{};
+  Geo g{};
+  Dev d{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool loaded = false;
+  std::vector<int32_t> site_off;
+  std::vector<Bin> bins;
+  int pair_grid = 1024;
+  // graph
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  // profiling
+  bool prof = false;
+  hipEvent_t ev[2 * kFamilies] = {};
+  double prof_ms[kFamilies] = {};
+  int64_t prof_n[kFamilies] = {};
+  // allocations
+  std::vector<void *> fixed_allocs;
+  std::vector<void *> cand_allocs;
+  size_t fixed_bytes = 0, cand_bytes = 0;
+};
+
+namespace {
+
+int fail(sc_system *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, expr)                                                       \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return fail((h), SC_ERR_HIP,                                            \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+template <typename T>
+int dalloc(sc_system *h, T **p, size_t n, bool cand) {
+  size_t bytes = sizeof(T) * (n ? n : 1);
+  void *v = nullptr;
+  hipError_t e = hipMalloc(&v, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(h, SC_ERR_NOMEM,
+                "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  }
+  *p = static_cast<T *>(v);
+  if (cand) {
+    h->cand_allocs.push_back(v);
+    h->cand_bytes += bytes;
+  } else {
+    h->fixed_allocs.push_back(v);
+    h->fixed_bytes += bytes;
+  }
+  return SC_OK;
+}
+
+void drop_graph(sc_system *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+}
+
+void free_cand(sc_system *h) {
+  for (void *v : h->cand_allocs) (void)hipFree(v);
+  h->cand_allocs.clear();
+  h->cand_bytes = 0;
+}
+
+int alloc_cand(sc_system *h, int64_t cap) {
+  free_cand(h);
+  drop_graph(h);
+  Dev &d = h->d;
+  d.pairs = nullptr;
+  d.slot64 = nullptr;
+  d.lm64 = nullptr;
+  d.slot32 = nullptr;
+  d.lm32 = nullptr;
+  int rc;
+  if ((rc = dalloc(h, &d.pairs, cap, true))) return rc;
+  if (h->p.precision == SC_PREC_FP64) {
+    if ((rc = dalloc(h, &d.slot64, cap, true))) return rc;
+    if ((rc = dalloc(h, &d.lm64, cap, true))) return rc;
+  } else {
+    if ((rc = dalloc(h, &d.slot32, cap, true))) return rc;
+    if ((rc = dalloc(h, &d.lm32, cap, true))) return rc;
+  }
+  if ((rc = dalloc(h, &d.inc, 2 * cap, true))) return rc;
+  h->g.cap = cap;
+  return SC_OK;
+}
+
+Ctl read_ctl(sc_system *h) {
+  Ctl c{};
+  (void)hipMemcpy(&c, h->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
+  return c;
+}
+
+int write_ctl(sc_system *h, const Ctl &c) {
+  HIPCHK(h, hipMemcpy(h->d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
+  return SC_OK;
+}
+
+void enqueue_step(sc_system *h) {
+  const Dev &d = h->d;
+  const Geo &g = h->g;
+  hipStream_t st = h->stream;
+  const bool pf = h->prof;
+  sc::launch_decide(d, g, st);
+  if (pf) (void)hipEventRecord(h->ev[6], st);
+  sc::launch_rebuild(d, g, st);
+  if (pf) {
+    (void)hipEventRecord(h->ev[7], st);
+    (void)hipEventRecord(h->ev[0], st);
+  }
+  sc::launch_pair(d, g, h->p.precision, h->pair_grid, st);
+  if (pf) {
+    (void)hipEventRecord(h->ev[1], st);
+    (void)hipEventRecord(h->ev[2], st);
+  }
+  sc::launch_reduce(d, g, h->p.precision, st);
+  if (pf) {
+    (void)hipEventRecord(h->ev[3], st);
+    (void)hipEventRecord(h->ev[4], st);
+  }
+  for (const Bin &b : h->bins)
+    sc::launch_integrate(d, g, b.list, b.count, b.maxn, h->p.precision, st);
+  if (pf) (void)hipEventRecord(h->ev[5], st);
+  sc::launch_commit(d, st);
+}
+
+int capture_graph(sc_system *h) {
+  drop_graph(h);
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(h);
+  hipGraph_t gr = nullptr;
+  HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
+  h->graph = gr;
+  HIPCHK(h, hipGraphInstantiate(&h->gexec, gr, nullptr, nullptr, 0));
+  return SC_OK;
+}
+
+// Runs the rebuild stage outside a step (load time), growing the candidate
+// capacity until the list fits.
+int initial_rebuild(sc_system *h) {
+  for (int attempt = 0; attempt < 16; ++attempt) {
+    Ctl c = read_ctl(h);
+    c.rebuild = 1;
+    c.abort = 0;
+    c.overflow = 0;
+    int rc = write_ctl(h, c);
+    if (rc) return rc;
+    sc::launch_rebuild(h->d, h->g, h->stream);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipGetLastError());
+    c = read_ctl(h);
+    if (c.overflow) {
+      int64_t ncap = std::max<int64_t>(2 * h->g.cap, c.nc_need + c.nc_need / 4 + 1024);
+      if ((rc = alloc_cand(h, ncap))) return rc;
+      continue;
+    }
+    c.rebuild = 0;
+    return write_ctl(h, c);
+  }
+  return fail(h, SC_ERR_NOMEM, "candidate capacity growth did not converge");
+}
+
+int check_handle(sc_system *h, bool need_loaded) {
+  if (!h) return SC_ERR_ARG;
+  if (need_loaded && !h->loaded)
+    return fail(h, SC_ERR_STATE, "no filaments loaded");
+  return SC_OK;
+}
+
+void get3(double *dst, const double *src_dev, int64_t n) {
+  (void)hipMemcpy(dst, src_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_abi_version(void) { return SC_ABI_VERSION; }
+
+int sc_create(const sc_params *params, sc_handle *out) {
+  if (!params || !out) return SC_ERR_ARG;
+  *out = nullptr;
+  const sc_params &p = *params;
+  if (p.n_dim != 3 || p.n_periodic != 3) return SC_ERR_ARG;
+  if (!(p.delta > 0) || !(p.system_radius > 0) || !(p.wca_sig > 0))
+    return SC_ERR_ARG;
+  if (p.precision != SC_PREC_FP64 && p.precision != SC_PREC_MIXED)
+    return SC_ERR_ARG;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (i != j && (p.unit_cell[3 * i + j] != 0.0 || p.unit_cell_inv[3 * i + j] != 0.0))
+        return SC_ERR_ARG;  // orthorhombic boxes only
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SC_ERR_HIP;
+  sc_system *h = new sc_system();
+  h->p = p;
+  Geo &g = h->g;
+  std::memcpy(g.h, p.unit_cell, sizeof(g.h));
+  std::memcpy(g.hinv, p.unit_cell_inv, sizeof(g.hinv));
+  // WCAPotential::Init (include/simcore/wca_potential.hpp:34-48)
+  double rcut = std::pow(2.0, 1.0 / 6.0) * p.wca_sig;
+  g.rcut2 = rcut * rcut;
+  g.c12 = 4.0 * p.wca_eps * std::pow(p.wca_sig, 12.0);
+  g.c6 = 4.0 * p.wca_eps * std::pow(p.wca_sig, 6.0);
+  g.fcut = p.f_cutoff;
+  g.delta = p.delta;
+  // InteractionManager::InitInteractions -> CellList::SetMinCellLength /
+  // Init (src/interaction_manager.cpp:46-54, src/cell_list.cpp:9-64)
+  double min_len = p.min_cell_length;
+  double rc_len = std::sqrt(g.rcut2);
+  if (rc_len > min_len) min_len = rc_len;
+  int n1d = (int)std::floor(2 * p.system_radius / min_len);
+  if (n1d <= 3) n1d = 3;
+  double cell_len = (double)2 * p.system_radius / n1d;
+  g.n1d = n1d;
+  g.thr = 0.25 * cell_len * cell_len;
+  g.n_cells = (int64_t)n1d * n1d * n1d;
+  if (g.n_cells >= (int64_t)1 << 31) {
+    delete h;
+    return SC_ERR_ARG;
+  }
+  g.stoch = p.stoch_flag ? 1 : 0;
+  g.seed = p.seed;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return SC_ERR_HIP;
+  }
+  *out = h;
+  return SC_OK;
+}
+
+int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
+                      const double *r, const sc_filament_params *fil) {
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  if (h->loaded) return fail(h, SC_ERR_STATE, "filaments already loaded");
+  if (nf <= 0 || !site_off || !r || !fil) return fail(h, SC_ERR_ARG, "null input");
+  if (site_off[0] != 0) return fail(h, SC_ERR_ARG, "site_off[0] must be 0");
+  for (int f = 0; f < nf; ++f) {
+    int n = site_off[f + 1] - site_off[f];
+    if (n < 3) return fail(h, SC_ERR_ARG, "a filament needs at least 2 bonds");
+    if (n > 256) return fail(h, SC_ERR_ARG, "at most 256 sites per filament");
+    if (!(fil[f].bond_length > 0) || !(fil[f].gamma_par > 0) || !(fil[f].gamma_perp > 0))
+      return fail(h, SC_ERR_ARG, "bad filament parameters");
+  }
+  Geo &g = h->g;
+  Dev &d = h->d;
+  g.nf = nf;
+  g.ns = site_off[nf];
+  g.nb = g.ns - nf;
+  h->site_off.assign(site_off, site_off + nf + 1);
+  const int64_t ns = g.ns, nb = g.nb;
+  if ((rc = dalloc(h, &d.ctl, 1, false))) return rc;
+  if ((rc = dalloc(h, &d.site_off, nf + 1, false))) return rc;
+  if ((rc = dalloc(h, &d.fpar, 8 * (size_t)nf, false))) return rc;
+  if ((rc = dalloc(h, &d.norm, 2 * (size_t)nf, false))) return rc;
+  if ((rc = dalloc(h, &d.bond_fil, nb, false))) return rc;
+  double **site_arrays[] = {&d.r, &d.rp, &d.fr, &d.dr0};
+  for (double **a : site_arrays)
+    if ((rc = dalloc(h, a, 3 * (size_t)ns, false))) return rc;
+  if ((rc = dalloc(h, &d.acc, ns, false))) return rc;
+  if ((rc = dalloc(h, &d.acc_prev, ns, false))) return rc;
+  double **bond3[] = {&d.spos, &d.u, &d.pos, &d.bf, &d.bt};
+  for (double **a : bond3)
+    if ((rc = dalloc(h, a, 3 * (size_t)nb, false))) return rc;
+  if ((rc = dalloc(h, &d.len, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.tension, nb, false))) return rc;
+  d.ub32 = nullptr;
+  if (h->p.precision == SC_PREC_MIXED)
+    if ((rc = dalloc(h, &d.ub32, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.keys, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.cell_count, g.n_cells, false))) return rc;
+  if ((rc = dalloc(h, &d.cell_start, g.n_cells + 1, false))) return rc;
+  if ((rc = dalloc(h, &d.sorted, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.skey, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.ccount, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.coff, nb + 1, false))) return rc;
+  if ((rc = dalloc(h, &d.deg, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.inc_start, nb + 1, false))) return rc;
+  int64_t tmp = std::max(sc::scan_tmp_elems(g.n_cells), sc::scan_tmp_elems(nb));
+  if ((rc = dalloc(h, &d.scan_tmp, tmp, false))) return rc;
+
+  // host-side per-filament tables
+  std::vector<double> fpar(8 * (size_t)nf);
+  std::vector<int32_t> norm(2 * (size_t)nf), bond_fil(nb);
+  int max_n = 0;
+  for (int f = 0; f < nf; ++f) {
+    const sc_filament_params &q = fil[f];
+    double *fp = &fpar[8 * (size_t)f];
+    fp[0] = q.bond_length;
+    fp[1] = q.persistence_length;
+    fp[2] = q.gamma_par;
+    fp[3] = q.gamma_perp;
+    // Filament::SetDiffusion (src/filament.cpp:346-347)
+    fp[4] = std::sqrt(24.0 * q.gamma_par / h->p.delta);
+    fp[5] = std::sqrt(24.0 * q.gamma_perp / h->p.delta);
+    fp[6] = q.diameter;
+    fp[7] = 0.0;
+    norm[2 * f] = 1;  // normalize_switch_ starts true
+    norm[2 * f + 1] = 0;
+    int n = site_off[f + 1] - site_off[f];
+    max_n = std::max(max_n, n);
+    for (int i = 0; i < n - 1; ++i) bond_fil[site_off[f] - f + i] = f;
+  }
+  HIPCHK(h, hipMemcpy(d.site_off, site_off, sizeof(int32_t) * (nf + 1), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(d.fpar, fpar.data(), sizeof(double) * fpar.size(), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(d.norm, norm.data(), sizeof(int32_t) * norm.size(), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(d.bond_fil, bond_fil.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(d.r, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(d.rp, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(d.dr0, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(d.fr, 0, sizeof(double) * 3 * ns));
+  HIPCHK(h, hipMemset(d.acc, 0, sizeof(double) * ns));
+  HIPCHK(h, hipMemset(d.acc_prev, 0, sizeof(double) * ns));
+  HIPCHK(h, hipMemset(d.bf, 0, sizeof(double) * 3 * nb));
+  HIPCHK(h, hipMemset(d.bt, 0, sizeof(double) * 3 * nb));
+  HIPCHK(h, hipMemset(d.tension, 0, sizeof(double) * nb));
+  HIPCHK(h, hipMemset(d.cell_count, 0, sizeof(int32_t) * g.n_cells));
+  HIPCHK(h, hipMemset(d.deg, 0, sizeof(int32_t) * nb));
+  Ctl c{};
+  if ((rc = write_ctl(h, c))) return rc;
+
+  // length bins for the integrator (one launch each)
+  const int bin_max[] = {32, 64, 128, 256};
+  std::vector<std::vector<int32_t>> lists(4);
+  for (int f = 0; f < nf; ++f) {
+    int n = site_off[f + 1] - site_off[f];
+    for (int k = 0; k < 4; ++k)
+      if (n <= bin_max[k]) {
+        lists[k].push_back(f);
+        break;
+      }
+  }
+  for (int k = 0; k < 4; ++k) {
+    if (lists[k].empty()) continue;
+    Bin b;
+    b.maxn = bin_max[k];
+    b.count = (int)lists[k].size();
+    if ((rc = dalloc(h, &b.list, lists[k].size(), false))) return rc;
+    HIPCHK(h, hipMemcpy(b.list, lists[k].data(), sizeof(int32_t) * lists[k].size(),
+                        hipMemcpyHostToDevice));
+    h->bins.push_back(b);
+  }
+  double cf = h->p.capacity_factor > 0 ? h->p.capacity_factor : 4.0;
+  int64_t cap = (int64_t)(cf * nb) + 4096;
+  if ((rc = alloc_cand(h, cap))) return rc;
+  {
+    int64_t blocks = (cap + 255) / 256;
+    h->pair_grid = (int)std::min<int64_t>(std::max<int64_t>(blocks, 1), 8192);
+  }
+  sc::launch_bond_init(d, g, h->p.precision, h->stream);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipGetLastError());
+  if ((rc = initial_rebuild(h))) return rc;
+  h->loaded = true;
+  return SC_OK;
+}
+
+int sc_step(sc_handle h, int32_t n_steps) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (n_steps < 0) return fail(h, SC_ERR_ARG, "negative step count");
+  Ctl c = read_ctl(h);
+  if (c.abort) return fail(h, c.err ? c.err : SC_ERR_STATE, "handle is in an error state");
+  int64_t target = c.i_step + n_steps;
+  for (int attempt = 0; attempt < 32; ++attempt) {
+    int64_t todo = target - c.i_step;
+    if (todo <= 0) return SC_OK;
+    const bool use_graph = h->p.use_graph && !h->prof;
+    if (use_graph && !h->gexec)
+      if ((rc = capture_graph(h))) return rc;
+    for (int64_t s = 0; s < todo; ++s) {
+      if (use_graph) {
+        HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+      } else {
+        enqueue_step(h);
+        if (h->prof) {
+          HIPCHK(h, hipEventSynchronize(h->ev[5]));
+          float ms = 0.f;
+          const int pairs[kFamilies][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}};
+          for (int k = 0; k < kFamilies; ++k) {
+            (void)hipEventElapsedTime(&ms, h->ev[pairs[k][0]], h->ev[pairs[k][1]]);
+            h->prof_ms[k] += ms;
+            h->prof_n[k] += 1;
+          }
+        }
+      }
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipGetLastError());
+    c = read_ctl(h);
+    if (!c.abort) continue;
+    if (c.overflow) {
+      int64_t ncap = std::max<int64_t>(2 * h->g.cap, c.nc_need + c.nc_need / 4 + 1024);
+      if ((rc = alloc_cand(h, ncap))) return rc;
+      h->pair_grid = (int)std::min<int64_t>((ncap + 255) / 256, 8192);
+      c.abort = 0;
+      c.overflow = 0;
+      if ((rc = write_ctl(h, c))) return rc;
+      continue;
+    }
+    if (c.err == SC_ERR_UNPHYSICAL)
+      return fail(h, SC_ERR_UNPHYSICAL, "Unphysical result: dr > 10000");
+    return fail(h, c.err ? c.err : SC_ERR_STATE, "step aborted");
+  }
+  return fail(h, SC_ERR_NOMEM, "candidate capacity growth did not converge");
+}
+
+int sc_get_sites(sc_handle h, double *r, double *r_prev, double *f_rand) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (r) get3(r, h->d.r, h->g.ns);
+  if (r_prev) get3(r_prev, h->d.rp, h->g.ns);
+  if (f_rand) get3(f_rand, h->d.fr, h->g.ns);
+  return SC_OK;
+}
+
+int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u, double *len) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (pos) get3(pos, h->d.pos, h->g.nb);
+  if (spos) get3(spos, h->d.spos, h->g.nb);
+  if (u) get3(u, h->d.u, h->g.nb);
+  if (len) HIPCHK(h, hipMemcpy(len, h->d.len, sizeof(double) * h->g.nb, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+int sc_get_bond_forces(sc_handle h, double *f, double *t) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (f) get3(f, h->d.bf, h->g.nb);
+  if (t) get3(t, h->d.bt, h->g.nb);
+  return SC_OK;
+}
+
+int sc_get_tensions(sc_handle h, double *tension) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!tension) return fail(h, SC_ERR_ARG, "null output");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(tension, h->d.tension, sizeof(double) * h->g.nb, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+int sc_get_cell_keys(sc_handle h, int32_t *keys) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!keys) return fail(h, SC_ERR_ARG, "null output");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(keys, h->d.keys, sizeof(int32_t) * h->g.nb, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+int sc_get_candidates(sc_handle h, int32_t *b1, int32_t *b2, size_t cap, size_t *n) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  Ctl c = read_ctl(h);
+  if (n) *n = (size_t)c.nc;
+  if (!b1 && !b2) return SC_OK;
+  if (cap < (size_t)c.nc) return fail(h, SC_ERR_ARG, "output capacity too small");
+  std::vector<int2> pr((size_t)c.nc);
+  if (c.nc)
+    HIPCHK(h, hipMemcpy(pr.data(), h->d.pairs, sizeof(int2) * c.nc, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < c.nc; ++i) {
+    if (b1) b1[i] = pr[i].x;
+    if (b2) b2[i] = pr[i].y;
+  }
+  return SC_OK;
+}
+
+int sc_get_slot_forces(sc_handle h, double *f, size_t cap) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  Ctl c = read_ctl(h);
+  if (!f || cap < (size_t)c.nc) return fail(h, SC_ERR_ARG, "output capacity too small");
+  const int64_t nc = c.nc;
+  if (h->p.precision == SC_PREC_FP64) {
+    std::vector<double4> v(nc);
+    if (nc) HIPCHK(h, hipMemcpy(v.data(), h->d.slot64, sizeof(double4) * nc, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nc; ++i) {
+      f[i] = v[i].x;
+      f[nc + i] = v[i].y;
+      f[2 * nc + i] = v[i].z;
+    }
+  } else {
+    std::vector<float4> v(nc);
+    if (nc) HIPCHK(h, hipMemcpy(v.data(), h->d.slot32, sizeof(float4) * nc, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nc; ++i) {
+      f[i] = v[i].x;
+      f[nc + i] = v[i].y;
+      f[2 * nc + i] = v[i].z;
+    }
+  }
+  return SC_OK;
+}
+
+int sc_get_dr_state(sc_handle h, double *acc) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!acc) return fail(h, SC_ERR_ARG, "null output");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(acc, h->d.acc, sizeof(double) * h->g.ns, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+int sc_get_counters(sc_handle h, int64_t *i_step, int64_t *n_rebuilds,
+                    int64_t *n_candidates, int32_t *n_cells_1d) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  Ctl c = read_ctl(h);
+  if (i_step) *i_step = c.i_step;
+  if (n_rebuilds) *n_rebuilds = c.n_rebuilds;
+  if (n_candidates) *n_candidates = c.nc;
+  if (n_cells_1d) *n_cells_1d = h->g.n1d;
+  return SC_OK;
+}
+
+int sc_set_sites(sc_handle h, const double *r) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!r) return fail(h, SC_ERR_ARG, "null input");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(h->d.r, r, sizeof(double) * 3 * h->g.ns, hipMemcpyHostToDevice));
+  sc::launch_bond_init(h->d, h->g, h->p.precision, h->stream);
+  Ctl c = read_ctl(h);
+  if (c.i_step >= 2 && c.i_step % 2 == 0) {
+    // the last full step already folded its displacement into the rebuild
+    // test of the next step; redo that with the new positions
+    c.dr_bits = 0ull;
+    if ((rc = write_ctl(h, c))) return rc;
+    sc::launch_redo_dr(h->d, h->g, h->stream);
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipGetLastError());
+  return SC_OK;
+}
+
+int sc_device_bytes(sc_handle h, size_t *bytes) {
+  if (!h || !bytes) return SC_ERR_ARG;
+  *bytes = h->fixed_bytes + h->cand_bytes;
+  return SC_OK;
+}
+
+int sc_profile_enable(sc_handle h, int on) {
+  if (!h) return SC_ERR_ARG;
+  if (on && !h->ev[0])
+    for (auto &e : h->ev) HIPCHK(h, hipEventCreate(&e));
+  h->prof = on != 0;
+  for (int k = 0; k < kFamilies; ++k) {
+    h->prof_ms[k] = 0;
+    h->prof_n[k] = 0;
+  }
+  return SC_OK;
+}
+
+int sc_profile_read(sc_handle h, double *avg_ms, int64_t *launches, int n) {
+  if (!h) return SC_ERR_ARG;
+  for (int k = 0; k < n && k < kFamilies; ++k) {
+    if (avg_ms) avg_ms[k] = h->prof_n[k] ? h->prof_ms[k] / h->prof_n[k] : 0.0;
+    if (launches) launches[k] = h->prof_n[k];
+  }
+  return SC_OK;
+}
+
+const char *sc_last_error(sc_handle h) {
+  if (!h) return "null handle";
+  return h->err.c_str();
+}
+
+int sc_destroy(sc_handle h) {
+  if (!h) return SC_ERR_ARG;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graph(h);
+  free_cand(h);
+  for (void *v : h->fixed_allocs) (void)hipFree(v);
+  for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return SC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// sc_internal.h -- device-side data layout and kernel launchers of the
+// MI355X filament BD path.  Internal to libsimcore_amd.so.
+//
+// HBM layout of one system (all SoA, filament-major):
+//   sites  (Ns):  r[3], r_prev[3], f_rand[3], dr0[3], acc        FP64
+//   bonds  (Nb):  spos[3], u[3], len, pos[3], F[3], T[3], tension  FP64
+//                 ub32 float4 {u, l/2}  (mixed path only)           FP32
+//   cells  (n^3): count, start (n^3+1)                              int32
+//   candidates (cap): pairs int2, slot F (4 doubles or float4),
+//                 lambda/mu (double2 or float2), incidence CSR      int32
+// Bonds of filament f occupy [site_off[f]-f, site_off[f+1]-f-1), so
+// "shares a site" (Bond::HasNeighbor) is |b1-b2| == 1 within a filament.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/simcore_amd.h"
+
+namespace sc {
+
+// Device-resident control block.  Every kernel of a step reads it, so a
+// whole step (rebuild branch included) can be replayed from a hipGraph
+// without host round-trips.
+struct Ctl {
+  int64_t i_step;          // completed steps
+  int64_t n_rebuilds;      // completed rebuilds
+  int64_t nc;              // live candidates
+  int64_t nc_need;         // candidates the last emission scan asked for
+  int64_t cur_step;        // step being executed
+  unsigned long long dr_bits;  // max per-site accumulator of the last full step
+  int32_t abort;           // nonzero: every kernel returns at once
+  int32_t err;             // SC_ERR_* when abort was raised by an error
+  int32_t rebuild;         // the current step rebuilds the candidate list
+  int32_t midstep;         // the current step is a midstep
+  int32_t force_rebuild;   // host request (load time)
+  int32_t overflow;        // candidate capacity exceeded
+};
+
+// Per-system constants passed by value to kernels.
+struct Geo {
+  double h[9];
+  double hinv[9];
+  double c12, c6, rcut2, fcut;
+  double delta;
+  double thr;              // 0.25 * cell_len^2
+  int n1d;
+  int stoch;
+  uint64_t seed;
+  int32_t nf;
+  int64_t ns, nb, n_cells, cap;
+};
+
+// Device pointers (a plain struct so it can be passed by value).
+struct Dev {
+  Ctl *ctl;
+  // filaments
+  int32_t *site_off;       // nf+1
+  double *fpar;            // nf*8: bl, kappa, gpar, gperp, sig_par, sig_perp, diam, 0
+  int32_t *norm;           // nf*2: normalize_switch_, n_normalize_
+  int32_t *bond_fil;       // nb
+  // sites
+  double *r, *rp, *fr, *dr0, *acc, *acc_prev;
+  // bonds
+  double *spos, *u, *len, *pos, *bf, *bt, *tension;
+  float4 *ub32;            // mixed path: {u, l/2}
+  // cell list
+  int32_t *keys, *cell_count, *cell_start, *sorted, *skey;
+  int32_t *ccount, *coff;
+  // candidates
+  int2 *pairs;
+  double4 *slot64;         // {Fx,Fy,Fz,-}
+  double2 *lm64;           // {lambda, mu}
+  float4 *slot32;          // {Fx,Fy,Fz,-}
+  float2 *lm32;
+  int32_t *deg, *inc_start, *inc;
+  // scan scratch
+  int32_t *scan_tmp;
+};
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+enum Guard { kGuardStep = 0, kGuardRebuild = 1 };
+
+// Launchers (sc_kernels.hip).  All are asynchronous on `st`.
+void launch_decide(const Dev &d, const Geo &g, hipStream_t st);
+void launch_commit(const Dev &d, hipStream_t st);
+void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st);
+void launch_pair(const Dev &d, const Geo &g, int precision, int grid,
+                 hipStream_t st);
+void launch_reduce(const Dev &d, const Geo &g, int precision, hipStream_t st);
+void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
+                      int bin_count, int bin_maxn, int precision,
+                      hipStream_t st);
+void launch_bond_init(const Dev &d, const Geo &g, int precision,
+                      hipStream_t st);
+void launch_pair32(const Dev &d, const Geo &g, int grid, hipStream_t st);
+void launch_reduce32(const Dev &d, const Geo &g, hipStream_t st);
+void launch_zero_i32(int32_t *p, int64_t n, hipStream_t st);
+void launch_redo_dr(const Dev &d, const Geo &g, hipStream_t st);
+void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
+                 const Ctl *ctl, int guard, hipStream_t st);
+int64_t scan_tmp_elems(int64_t n);
+
+}  // namespace sc

@@ -1,0 +1,288 @@
+// sc_math.h -- device math of the filament BD path, in the reference's
+// operation order (the FP64 path is compared value-for-value with a CPU
+// restatement; these functions must stay free of reassociation).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sc {
+
+// NINT (include/simcore/macros.hpp:6): round half away from zero to int.
+__device__ __forceinline__ int nint(double x) {
+  return x < 0.0 ? (int)(x - 0.5) : (int)(x + 0.5);
+}
+
+// CellList::FindCellCoords (src/cell_list.cpp:139-157) packed as
+// (x*n + y)*n + z, the order of MakePairs' cell_[x][y][z] walk.  Scaled
+// coordinates lie in [-0.5, 0.5]; the clamp to [0, n-1] only guards the
+// half-ulp case s = -0.5 - 2^-54, where the reference would index cell -1.
+__device__ __forceinline__ int32_t cell_key(int n1d, double sx, double sy,
+                                            double sz) {
+  double s[3] = {sx, sy, sz};
+  int c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double x = s[k] + 0.5;
+    int xc = (int)floor(n1d * x);
+    if (xc == n1d) xc -= 1;
+    xc = xc < 0 ? 0 : (xc >= n1d ? n1d - 1 : xc);
+    c[k] = xc;
+  }
+  return ((int32_t)c[0] * n1d + c[1]) * n1d + c[2];
+}
+
+// Non-redundant neighbour stencil of AssignCellNeighbors(false)
+// (src/cell_list.cpp:206-262) for a fully periodic 3D grid: (x+1,y,z);
+// (x-1..x+1, y+1, z); (x-1..x+1, y-1..y+1, z+1), wrapped.
+__device__ __forceinline__ void half_shell(int n, int32_t key, int32_t *out) {
+  const int x = key / (n * n);
+  const int y = (key / n) % n;
+  const int z = key % n;
+  int cnt = 0;
+  for (int zp = z; zp < z + 2; ++zp) {
+    int nz = (zp == n) ? 0 : zp;
+    int y_begin = (zp > z ? y - 1 : y);
+    for (int yp = y_begin; yp < y + 2; ++yp) {
+      int ny = yp < 0 ? n - 1 : (yp == n ? 0 : yp);
+      int x_begin = (yp > y || zp > z ? x - 1 : x + 1);
+      for (int xp = x_begin; xp < x + 2; ++xp) {
+        int nx = xp < 0 ? n - 1 : (xp == n ? 0 : xp);
+        out[cnt++] = ((int32_t)nx * n + ny) * n + nz;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+  double m = 0.0;
+  m += a[0] * b[0];
+  m += a[1] * b[1];
+  m += a[2] * b[2];
+  return m;
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b,
+                                       double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __forceinline__ void normalize3(double *a) {
+  double m = 0.0;
+  m += a[0] * a[0];
+  m += a[1] * a[1];
+  m += a[2] * a[2];
+  m = sqrt(m);
+  a[0] /= m;
+  a[1] /= m;
+  a[2] /= m;
+}
+
+// tridiagonal_solver (src/linear_algebra.cpp:164-197): Thomas, in place on
+// c and d, n unknowns.
+__device__ __forceinline__ void thomas(int n, const double *a, const double *b,
+                                       double *c, double *d) {
+  n--;
+  c[0] /= b[0];
+  d[0] /= b[0];
+  for (int i = 1; i < n; ++i) {
+    double den = b[i] - a[i - 1] * c[i - 1];
+    c[i] /= den;
+    d[i] = (d[i] - a[i - 1] * d[i - 1]) / den;
+  }
+  d[n] = (d[n] - a[n - 1] * d[n - 1]) / (b[n] - a[n - 1] * c[n - 1]);
+  for (int i = n; i-- > 0;) d[i] -= c[i] * d[i + 1];
+}
+
+__device__ __forceinline__ double dabs(double x) { return x < 0 ? -x : x; }
+__device__ __forceinline__ double dsign(double a, double b) {
+  return b >= 0.0 ? fabs(a) : -fabs(a);
+}
+
+// MinimumDistance::Sphero for a fully periodic box
+// (src/minimum_distance.cpp:202-344).  Returns r_min, |r_min|^2 and the
+// contact parameters lambda (along u1) and mu (along u2).
+__device__ __forceinline__ void sphero64(const double *h, const double *s_1,
+                                         const double *u_1, double length_1,
+                                         const double *s_2, const double *u_2,
+                                         double length_2, double *r_min,
+                                         double *r_min_mag2, double *lam_out,
+                                         double *mu_out) {
+  double dr[3], ds[3];
+  const double hl1 = 0.5 * length_1, hl2 = 0.5 * length_2;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ds[i] = s_2[i] - s_1[i];
+    ds[i] -= (double)nint(ds[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    dr[i] = 0.0;
+    for (int j = 0; j < 3; ++j) dr[i] += h[3 * i + j] * ds[j];
+  }
+  double a = 0.0, b = 0.0, c = 0.0;  // dr.u1, dr.u2, u1.u2
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    a += dr[i] * u_1[i];
+    b += dr[i] * u_2[i];
+    c += u_1[i] * u_2[i];
+  }
+  double lambda, mu;
+  const double denom = 1.0 - c * c;
+  if (denom < 1.0e-12) {
+    lambda = a / 2.0;
+    mu = -b / 2.0;
+  } else {
+    lambda = (a - c * b) / denom;
+    mu = (-b + c * a) / denom;
+  }
+  double lambda_mag = dabs(lambda), mu_mag = dabs(mu);
+  if (lambda_mag > hl1 && mu_mag > hl2) {
+    double la = dsign(hl1, lambda);
+    double ma = -b + la * c;
+    if (dabs(ma) > hl2) ma = dsign(hl2, ma);
+    double ra[3], rb[3], m2a = 0.0, m2b = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ra[i] = dr[i] - la * u_1[i] + ma * u_2[i];
+      m2a += ra[i] * ra[i];
+    }
+    double mb = dsign(hl2, mu);
+    double lb = a + mb * c;
+    if (dabs(lb) > hl1) lb = dsign(hl1, lb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      rb[i] = dr[i] - lb * u_1[i] + mb * u_2[i];
+      m2b += rb[i] * rb[i];
+    }
+    if (m2a < m2b) {
+      lambda = la;
+      mu = ma;
+      *r_min_mag2 = m2a;
+      for (int i = 0; i < 3; ++i) r_min[i] = ra[i];
+    } else {
+      lambda = lb;
+      mu = mb;
+      *r_min_mag2 = m2b;
+      for (int i = 0; i < 3; ++i) r_min[i] = rb[i];
+    }
+  } else {
+    if (lambda_mag > hl1) {
+      lambda = dsign(hl1, lambda);
+      mu = -b + lambda * c;
+      if (dabs(mu) > hl2) mu = dsign(hl2, mu);
+    } else if (mu_mag > hl2) {
+      mu = dsign(hl2, mu);
+      lambda = a + mu * c;
+      if (dabs(lambda) > hl1) lambda = dsign(hl1, lambda);
+    }
+    double m2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      r_min[i] = dr[i] - lambda * u_1[i] + mu * u_2[i];
+      m2 += r_min[i] * r_min[i];
+    }
+    *r_min_mag2 = m2;
+  }
+  *lam_out = lambda;
+  *mu_out = mu;
+}
+
+// PotentialManager::CalcPotential (include/simcore/potential_manager.hpp:
+// 92-103) -> MaxForcePotential when dr^2 < 1e-12
+// (include/simcore/max_force_potential.hpp:11-40), else WCAPotential
+// (include/simcore/wca_potential.hpp:14-32).  Returns 1 if force written.
+__device__ __forceinline__ int pair_force64(double c12, double c6,
+                                            double fcut, double *dr,
+                                            double dr2, double lam, double mu,
+                                            const double *u1, const double *u2,
+                                            double *force) {
+  if (dr2 < 1e-12) {
+    double c1[3], c2[3];
+    for (int i = 0; i < 3; ++i) {
+      c1[i] = lam * u1[i];
+      c2[i] = mu * u2[i];
+    }
+    if (c1[0] != 0.0 || c1[1] != 0.0 || c1[2] != 0.0)
+      for (int i = 0; i < 3; ++i) dr[i] = c1[i] - c2[i];
+    double rmag = 0.0;
+    for (int i = 0; i < 3; ++i) rmag += dr[i] * dr[i];
+    rmag = sqrt(rmag);
+    if (rmag < 1e-12) return 0;
+    double rinv = 1.0 / (rmag);
+    for (int i = 0; i < 3; ++i) force[i] = fcut * dr[i] * rinv;
+    return 1;
+  }
+  double rmag = sqrt(dr2);
+  double rinv = 1.0 / (rmag);
+  double rinv2 = rinv * rinv;
+  double r6 = rinv2 * rinv2 * rinv2;
+  double ffac = -(12.0 * c12 * r6 - 6.0 * c6) * r6 * rinv;
+  if (dabs(ffac) > fcut) ffac = (ffac >= 0.0 ? 1 : -1) * fcut;
+  for (int i = 0; i < 3; ++i) force[i] = ffac * dr[i] * rinv;
+  return 1;
+}
+
+// Metric-corrected bending terms of Filament::CalculateBendingForces, 3D
+// branch (src/filament.cpp:725-775).  For component d, p and q are the
+// other two components in increasing order.
+__device__ __forceinline__ double bterm(const double *a, const double *b,
+                                        int d) {
+  const int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
+  return (1 - b[d] * b[d]) * a[d] - b[d] * b[p] * a[p] - b[d] * b[q] * a[q];
+}
+__device__ __forceinline__ void bend_add(double *f, double ke, const double *u1,
+                                         const double *u2) {
+  for (int d = 0; d < 3; ++d) f[d] += ke * bterm(u1, u2, d);
+}
+__device__ __forceinline__ void bend_mid(double *f, double ke, const double *u1,
+                                         const double *u2) {
+  for (int d = 0; d < 3; ++d) f[d] += ke * (bterm(u2, u1, d) - bterm(u1, u2, d));
+}
+__device__ __forceinline__ void bend_sub(double *f, double ke, const double *u1,
+                                         const double *u2) {
+  for (int d = 0; d < 3; ++d) f[d] -= ke * bterm(u2, u1, d);
+}
+
+// Element (a,b) of the site mobility 1/gpar t t^T + 1/gperp (I - t t^T)
+// (src/filament.cpp:782-794).
+__device__ __forceinline__ double gam(double igpar, double igperp,
+                                      const double *t, int a, int b) {
+  return igpar * (t[a] * t[b]) + igperp * ((a == b ? 1 : 0) - t[a] * t[b]);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11): the injected thermal-noise stream.
+__device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0,
+                                           uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+    const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+  }
+}
+
+// xi_d = (x_d + 1/2) 2^-32 - 1/2 (exact), standing in for
+// RandomUniform() - 0.5 of src/filament.cpp:529-530; counter =
+// (global site, midstep index).
+__device__ __forceinline__ void noise_xi(uint64_t seed, int64_t site,
+                                         int64_t m, double *xi) {
+  uint32_t c[4] = {(uint32_t)site, (uint32_t)((uint64_t)site >> 32),
+                   (uint32_t)m, (uint32_t)((uint64_t)m >> 32)};
+  philox4x32(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  for (int k = 0; k < 3; ++k)
+    xi[k] = ((double)c[k] + 0.5) * 2.3283064365386962890625e-10 - 0.5;
+}
+
+}  // namespace sc

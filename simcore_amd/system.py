@@ -1,0 +1,236 @@
+"""Host-side mirror of the reference's filament species / simulation
+interface for the BD hot path, driving the HIP path through the C ABI.
+
+`FilamentSystem` corresponds to one reference `Simulation` holding one
+`FilamentSpecies` (src/simulation.cpp, src/filament_species.cpp): `step(n)`
+is n iterations of RunSimulation's body, and the getters are the object
+accessors the reference exposes (positions, bond forces, tensions, the
+interaction manager's pair list and cell assignments).
+"""
+import ctypes
+from ctypes import byref, c_double, c_int32, c_int64, c_size_t, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import (SC_ERR_UNPHYSICAL, SC_OK, SC_PREC_FP64, SC_PREC_MIXED,
+                   ScFilamentParams, ScParams, SimcoreError, UnphysicalResult,
+                   lib)
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(c_int32))
+
+
+def _check(rc, h=None, what=""):
+    if rc == SC_OK:
+        return
+    msg = what
+    if h is not None and h.value:
+        msg = f"{what}: {lib.sc_last_error(h).decode()}"
+    if rc == SC_ERR_UNPHYSICAL:
+        raise UnphysicalResult(rc, msg)
+    raise SimcoreError(rc, msg)
+
+
+def params_init(system_radius, **kw):
+    """sc_params with the reference defaults for a periodic cube of side
+    2*system_radius; keyword arguments override fields."""
+    p = ScParams()
+    _check(lib.sc_params_init(byref(p), float(system_radius)), what="sc_params_init")
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def params_from_yaml(path):
+    """(sc_params, species dict) from a reference-style YAML file."""
+    p = ScParams()
+    spec = (c_double * 9)()
+    _check(lib.sc_params_from_yaml(str(path).encode(), byref(p), spec),
+           what=f"sc_params_from_yaml({path})")
+    keys = ["num", "length", "persistence_length", "diameter", "min_bond_length",
+            "friction_ratio", "n_steps", "min_length", "max_length"]
+    return p, dict(zip(keys, list(spec)))
+
+
+def filament_layout(length, min_bond_length, min_length=5.0, max_length=500.0):
+    """(n_bonds, bond_length) as Filament::InitFilamentLength computes them."""
+    nb = c_int32()
+    bl = c_double()
+    _check(lib.sc_filament_layout(length, min_bond_length, min_length, max_length,
+                                  byref(nb), byref(bl)), what="sc_filament_layout")
+    return nb.value, bl.value
+
+
+@dataclass
+class Filaments:
+    """Filament set in the C ABI's layout."""
+    site_off: np.ndarray      # int32, nf+1
+    r: np.ndarray             # float64, 3*Ns (SoA)
+    fil: object               # ctypes array of ScFilamentParams
+
+    @property
+    def nf(self):
+        return len(self.site_off) - 1
+
+    @property
+    def ns(self):
+        return int(self.site_off[-1])
+
+    @property
+    def nb(self):
+        return self.ns - self.nf
+
+    def fil_array(self, field):
+        return np.array([getattr(f, field) for f in self.fil])
+
+
+def generate_filaments(params, n_sites, length, persistence_length, diameter=None,
+                       friction_ratio=2.0, seed=1):
+    """Random reference-style insertion with overlap rejection."""
+    n_sites = np.ascontiguousarray(n_sites, dtype=np.int32)
+    nf = len(n_sites)
+    length = np.ascontiguousarray(np.broadcast_to(length, (nf,)), dtype=np.float64)
+    lp = np.ascontiguousarray(np.broadcast_to(persistence_length, (nf,)), dtype=np.float64)
+    d = np.ascontiguousarray(np.broadcast_to(1.0 if diameter is None else diameter, (nf,)),
+                             dtype=np.float64)
+    tot = c_int64()
+    _check(lib.sc_generate_filaments(byref(params), nf, _ip(n_sites), _dp(length), _dp(lp),
+                                     _dp(d), friction_ratio, seed, None, None, None,
+                                     byref(tot)), what="sc_generate_filaments")
+    site_off = np.zeros(nf + 1, dtype=np.int32)
+    r = np.zeros(3 * tot.value, dtype=np.float64)
+    fil = (ScFilamentParams * nf)()
+    _check(lib.sc_generate_filaments(byref(params), nf, _ip(n_sites), _dp(length), _dp(lp),
+                                     _dp(d), friction_ratio, seed, _ip(site_off), _dp(r), fil,
+                                     byref(tot)), what="sc_generate_filaments")
+    return Filaments(site_off, r, fil)
+
+
+class FilamentSystem:
+    """One system's device state (sc_handle)."""
+
+    def __init__(self, params, filaments, precision="fp64", use_graph=False,
+                 capacity_factor=0.0):
+        p = ScParams()
+        ctypes.pointer(p)[0] = params
+        p.precision = {"fp64": SC_PREC_FP64, "mixed": SC_PREC_MIXED}[precision]
+        p.use_graph = int(bool(use_graph))
+        p.capacity_factor = capacity_factor
+        self.params = p
+        self.filaments = filaments
+        self.h = c_void_p()
+        _check(lib.sc_create(byref(p), byref(self.h)), what="sc_create")
+        so = np.ascontiguousarray(filaments.site_off, dtype=np.int32)
+        r = np.ascontiguousarray(filaments.r, dtype=np.float64)
+        _check(lib.sc_load_filaments(self.h, filaments.nf, _ip(so), _dp(r), filaments.fil),
+               self.h, "sc_load_filaments")
+        self.ns, self.nb, self.nf = filaments.ns, filaments.nb, filaments.nf
+
+    # -- lifecycle -----------------------------------------------------------
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib.sc_destroy(self.h)
+        self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stepping ------------------------------------------------------------
+    def step(self, n=1):
+        _check(lib.sc_step(self.h, int(n)), self.h, "sc_step")
+
+    # -- taps ----------------------------------------------------------------
+    def sites(self):
+        r = np.empty(3 * self.ns)
+        rp = np.empty(3 * self.ns)
+        fr = np.empty(3 * self.ns)
+        _check(lib.sc_get_sites(self.h, _dp(r), _dp(rp), _dp(fr)), self.h, "sc_get_sites")
+        return r, rp, fr
+
+    def bonds(self):
+        pos, sp, u = (np.empty(3 * self.nb) for _ in range(3))
+        ln = np.empty(self.nb)
+        _check(lib.sc_get_bonds(self.h, _dp(pos), _dp(sp), _dp(u), _dp(ln)), self.h,
+               "sc_get_bonds")
+        return pos, sp, u, ln
+
+    def bond_forces(self):
+        f = np.empty(3 * self.nb)
+        t = np.empty(3 * self.nb)
+        _check(lib.sc_get_bond_forces(self.h, _dp(f), _dp(t)), self.h, "sc_get_bond_forces")
+        return f, t
+
+    def tensions(self):
+        t = np.empty(self.nb)
+        _check(lib.sc_get_tensions(self.h, _dp(t)), self.h, "sc_get_tensions")
+        return t
+
+    def cell_keys(self):
+        k = np.empty(self.nb, dtype=np.int32)
+        _check(lib.sc_get_cell_keys(self.h, _ip(k)), self.h, "sc_get_cell_keys")
+        return k
+
+    def candidates(self):
+        n = c_size_t()
+        _check(lib.sc_get_candidates(self.h, None, None, 0, byref(n)), self.h,
+               "sc_get_candidates")
+        b1 = np.empty(n.value, dtype=np.int32)
+        b2 = np.empty(n.value, dtype=np.int32)
+        _check(lib.sc_get_candidates(self.h, _ip(b1), _ip(b2), n.value, byref(n)), self.h,
+               "sc_get_candidates")
+        return b1, b2
+
+    def slot_forces(self):
+        n = self.counters()["n_candidates"]
+        f = np.empty(3 * n)
+        _check(lib.sc_get_slot_forces(self.h, _dp(f), n), self.h, "sc_get_slot_forces")
+        return f
+
+    def dr_state(self):
+        a = np.empty(self.ns)
+        _check(lib.sc_get_dr_state(self.h, _dp(a)), self.h, "sc_get_dr_state")
+        return a
+
+    def counters(self):
+        i_step, nreb, nc = c_int64(), c_int64(), c_int64()
+        n1d = c_int32()
+        _check(lib.sc_get_counters(self.h, byref(i_step), byref(nreb), byref(nc), byref(n1d)),
+               self.h, "sc_get_counters")
+        return {"i_step": i_step.value, "n_rebuilds": nreb.value,
+                "n_candidates": nc.value, "n_cells_1d": n1d.value}
+
+    def set_sites(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        _check(lib.sc_set_sites(self.h, _dp(r)), self.h, "sc_set_sites")
+
+    def device_bytes(self):
+        b = c_size_t()
+        _check(lib.sc_device_bytes(self.h, byref(b)), self.h, "sc_device_bytes")
+        return b.value
+
+    def profile(self, on=True):
+        _check(lib.sc_profile_enable(self.h, int(on)), self.h, "sc_profile_enable")
+
+    def profile_read(self):
+        ms = (c_double * 4)()
+        n = (c_int64 * 4)()
+        _check(lib.sc_profile_read(self.h, ms, n, 4), self.h, "sc_profile_read")
+        names = ["pair", "reduce", "integrate", "rebuild"]
+        return {k: (ms[i], n[i]) for i, k in enumerate(names)}

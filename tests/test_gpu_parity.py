@@ -1,0 +1,141 @@
+"""Parity of the HIP path against the CPU oracle on the same inputs and the
+same injected noise stream (C ABI calls through simcore_amd)."""
+import numpy as np
+import pytest
+
+import simcore_amd as sc
+from oracle.oracle_py import OracleSystem
+from tests.helpers import c0_system, culled
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(num=None, seed=7, precision="fp64", use_graph=False, capacity_factor=0.0):
+    p, fil = c0_system(num=num, seed=seed)
+    dev = sc.FilamentSystem(p, fil, precision=precision, use_graph=use_graph,
+                            capacity_factor=capacity_factor)
+    ora = OracleSystem(p, fil)
+    return p, fil, dev, ora
+
+
+def test_initial_cell_keys_and_candidates_bit_exact():
+    p, fil, dev, ora = _pair()
+    print("case: C0 64x20 seed 7, load-time rebuild", flush=True)
+    np.testing.assert_array_equal(dev.cell_keys(), ora.cell_keys())
+    ob1, ob2 = culled(*ora.candidates(), fil.site_off)
+    db1, db2 = dev.candidates()
+    np.testing.assert_array_equal(db1, ob1)
+    np.testing.assert_array_equal(db2, ob2)
+    # bond state from the same site positions: bit-identical FP64
+    for a, b in zip(dev.bonds(), ora.bonds()):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_fp64_one_step_bit_exact():
+    p, fil, dev, ora = _pair()
+    print("case: C0 fp64 1 step", flush=True)
+    dev.step(1)
+    assert ora.step(1) == 0
+    fd, td = dev.bond_forces()
+    fo, to = ora.bond_forces()
+    np.testing.assert_array_equal(fd, fo)
+    np.testing.assert_array_equal(td, to)
+    np.testing.assert_array_equal(dev.tensions(), ora.tensions())
+    for a, b in zip(dev.sites(), ora.sites()):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(dev.slot_forces(), ora.slot_forces())
+
+
+def test_fp64_100_steps_bit_exact():
+    p, fil, dev, ora = _pair()
+    print("case: C0 fp64 100 steps", flush=True)
+    # cumulative 1, 3, 7, 21, 51, 99, 100.  After odd steps both sides hold
+    # the same displacement accumulators (the device adds a full step's
+    # displacement right after it, the CPU path at the next step's start).
+    for chunk in (1, 2, 4, 14, 30, 48, 1):
+        dev.step(chunk)
+        assert ora.step(chunk) == 0
+        cd, co = dev.counters(), ora.counters()
+        assert cd["i_step"] == co["i_step"]
+        assert cd["n_rebuilds"] == co["n_rebuilds"]
+        np.testing.assert_array_equal(dev.cell_keys(), ora.cell_keys())
+        db1, db2 = dev.candidates()
+        ob1, ob2 = ora.candidates()
+        np.testing.assert_array_equal(db1, ob1)
+        np.testing.assert_array_equal(db2, ob2)
+        for a, b in zip(dev.sites(), ora.sites()):
+            np.testing.assert_array_equal(a, b)
+        if cd["i_step"] % 2 == 1:
+            np.testing.assert_array_equal(dev.dr_state(), ora.dr_state()[0])
+    assert dev.counters()["n_rebuilds"] > 1
+
+
+def test_graph_replay_matches_eager():
+    print("case: C0 fp64 graph vs eager 60 steps", flush=True)
+    _, _, eager, _ = _pair()
+    _, _, graph, _ = _pair(use_graph=True)
+    for _ in range(3):
+        eager.step(20)
+        graph.step(20)
+        for a, b in zip(eager.sites(), graph.sites()):
+            np.testing.assert_array_equal(a, b)
+    assert eager.counters() == graph.counters()
+
+
+def test_capacity_growth_is_exact():
+    print("case: C0 fp64 tiny candidate capacity", flush=True)
+    _, _, ref, _ = _pair()
+    _, _, small, _ = _pair(capacity_factor=0.01)
+    ref.step(40)
+    small.step(40)
+    for a, b in zip(ref.sites(), small.sites()):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_mixed_forces_within_tolerance():
+    """FP32 pair math: per-bond force error relative to the FP64 force
+    scale <= 1e-5 after one step from identical state."""
+    print("case: C0 mixed 1 step", flush=True)
+    p, fil, dev, ora = _pair(precision="mixed")
+    dev.step(1)
+    assert ora.step(1) == 0
+    fd, td = dev.bond_forces()
+    fo, to = ora.bond_forces()
+    scale = np.linalg.norm(fo)
+    assert scale > 0
+    err = np.linalg.norm(fd - fo) / scale
+    print("mixed relative force error", err)
+    assert err <= 1e-5
+    terr = np.linalg.norm(td - to) / np.linalg.norm(to)
+    assert terr <= 1e-5
+
+
+def test_unphysical_displacement_raises_at_the_same_step():
+    print("case: C0 8 filaments, filament 0 moved by 2e4 after step 2", flush=True)
+    p, fil, dev, ora = _pair(num=8, seed=9)
+    dev.step(2)
+    assert ora.step(2) == 0
+    r = dev.sites()[0]
+    r[: fil.site_off[1]] += 2.0e4
+    dev.set_sites(r)
+    ora.set_sites(r)
+    assert ora.step(1) == -1
+    with pytest.raises(sc.UnphysicalResult):
+        dev.step(1)
+    assert dev.counters()["i_step"] == 2
+
+
+def test_set_sites_keeps_trajectories_identical():
+    print("case: C0 small perturbation after a full step", flush=True)
+    p, fil, dev, ora = _pair(num=16, seed=13)
+    dev.step(6)
+    assert ora.step(6) == 0
+    r = dev.sites()[0]
+    r[5] += 0.3
+    dev.set_sites(r)
+    ora.set_sites(r)
+    dev.step(40)
+    assert ora.step(40) == 0
+    for a, b in zip(dev.sites(), ora.sites()):
+        np.testing.assert_array_equal(a, b)
+    assert dev.counters()["n_rebuilds"] == ora.counters()["n_rebuilds"]
