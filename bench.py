@@ -57,11 +57,14 @@ def workload(name, sc):
         n = np.clip(n, 8, 256)
         lp = rng.uniform(10.0, 1000.0, nf)
         R = box_for_fraction(n, 1.0, 1.0, 0.08)
-        p = sc.params_init(R, delta=1e-4, min_cell_length=1.0, seed=20240003)
+        # dt 1e-5: the midstep scheme is explicit, and lp = 1000 sigma puts
+        # the stiffest bending mode at ~16 kappa/(l^3 gamma_perp) ~ 7e4 per
+        # unit time, beyond the 2/dt stability limit at dt = 1e-4
+        p = sc.params_init(R, delta=1e-5, min_cell_length=1.0, seed=20240003)
         fil = sc.generate_filaments(p, n, (n - 1).astype(float), lp, 1.0, 2.0, seed=12)
         desc = {"workload": "uneven lengths: 50000 filaments, sites lognormal(median 24, "
                             "sigma_ln 0.8) clipped to [8,256], lp U[10,1000] sigma, "
-                            "volume fraction 0.08, periodic cube, WCA, dt 1e-4",
+                            "volume fraction 0.08, periodic cube, WCA, dt 1e-5",
                 "n_filaments": nf, "volume_fraction": 0.08, "box_side": 2 * R}
     elif name == "c0":
         p, spec = sc.params_from_yaml(os.path.join(ROOT, "simcore_amd", "configs",
@@ -80,13 +83,14 @@ def workload(name, sc):
 
 
 def pair_bytes(precision, nb, nc):
-    """Algorithmic HBM bytes of one pair-kernel launch: each bond's state
-    read once, each candidate's pair indices and sticky force slot read and
-    its contact parameters written (the conditional slot write of in-range
-    pairs is not counted)."""
-    if precision == "mixed":
-        return nb * (24 + 16) + nc * (8 + 16 + 8)
-    return nb * (24 + 24 + 8) + nc * (8 + 32 + 16)
+    """Algorithmic HBM bytes of one launch of the fused pair/scatter kernel:
+    per bond its 64-byte state record read, its force and torque (6 FP64)
+    written and its incidence offset read; per candidate the sticky force
+    slot and its two incidence entries {candidate, partner} read.  Re-reads of
+    a partner's record and the conditional slot write of in-range pairs are
+    not counted."""
+    slot = 16 if precision == "mixed" else 32
+    return nb * (64 + 48 + 4) + nc * (slot + 2 * 8)
 
 
 def timed_steps(sysobj, steps):
@@ -140,7 +144,8 @@ def main():
     log(f"[bench] profile: {json.dumps(prof)} nc={nc}")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_pair32" if args.precision == "mixed" else "k_pair64",
+                "kernel": "k_pair_fused<true>" if args.precision == "mixed"
+                          else "k_pair_fused<false>",
                 "kernel_avg_ms": pair_ms, "algorithmic_bytes": bytes_pair,
                 "candidates": nc, "per_step_ms": {k: v[0] for k, v in prof.items()}}
     dev.close()
@@ -150,9 +155,15 @@ def main():
         d64 = sc.FilamentSystem(p, fil, precision="fp64", use_graph=False)
         d64.step(10)
         w64 = timed_steps(d64, args.fp64_steps)
+        d64.profile(True)
+        d64.step(args.profile_steps)
+        p64 = d64.profile_read()
+        d64.profile(False)
         extra["plain_fp64_gpu"] = {"value": fil.nb * args.fp64_steps / w64,
                                    "unit": "segment-steps/s", "steps": args.fp64_steps,
-                                   "note": "one system, FP64 pair path, eager launches"}
+                                   "note": "one system, FP64 everywhere (bit-compatible with "
+                                           "the CPU path), eager launches",
+                                   "per_step_ms": {k: v[0] for k, v in p64.items()}}
         log(f"[bench] plain fp64: {extra['plain_fp64_gpu']['value']:.4e}")
         d64.close()
 

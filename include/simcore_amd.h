@@ -136,10 +136,17 @@ int sc_set_sites(sc_handle h, const double *r);
 int sc_device_bytes(sc_handle h, size_t *bytes);
 /* Average per-launch duration (ms) of each kernel family recorded with HIP
  * events on the handle's stream while profiling was enabled; families:
- * 0 pair, 1 reduce, 2 integrate, 3 rebuild (whole stage). */
+ * 0 pair pass with the fused per-bond force scatter, 1 unused (0),
+ * 2 integrate (all length bins), 3 rebuild stage (odd steps only). */
 int sc_profile_enable(sc_handle h, int on);
 int sc_profile_read(sc_handle h, double *avg_ms, int64_t *launches,
                     int n_families);
+
+/* Measurement probe with no counterpart in the reference: launches a plain
+ * streaming copy of `bytes` (read once, written once) `repeats` times and
+ * reports the average duration; used to calibrate hardware byte counters
+ * and as the achievable-bandwidth reference next to the roofline. */
+int sc_copy_probe(size_t bytes, int repeats, double *avg_ms);
 
 const char *sc_last_error(sc_handle h);
 int sc_destroy(sc_handle h);

@@ -1032,7 +1032,12 @@ void orc_get_dr(const orc_sys *s, double *acc, double *dr_max_per_fil) {
   if (dr_max_per_fil) memcpy(dr_max_per_fil, s->dr_tot, sizeof(double) * (size_t)s->nf);
 }
 void orc_set_sites(orc_sys *s, const double *r) {
+  /* positions replaced from outside: bond state follows (Mesh::
+   * UpdateBondPositions, src/mesh.cpp:324-334) */
   for (int d = 0; d < 3; ++d) memcpy(s->r[d], r + d * s->ns, sizeof(double) * (size_t)s->ns);
+  for (int f = 0; f < s->nf; ++f)
+    for (int i = 0; i < s->site_off[f + 1] - s->site_off[f] - 1; ++i)
+      bond_reinit(s, s->site_off[f] - f + i, s->site_off[f] + i);
 }
 
 void orc_bending_forces(int n, const double *u, double kappa, double bl,

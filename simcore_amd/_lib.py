@@ -29,7 +29,7 @@ EXPORTS = [
     "sc_get_dr_state", "sc_get_counters", "sc_set_sites", "sc_device_bytes",
     "sc_profile_enable", "sc_profile_read", "sc_last_error", "sc_destroy",
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
-    "sc_generate_filaments",
+    "sc_generate_filaments", "sc_copy_probe",
 ]
 
 
@@ -92,6 +92,7 @@ def _load():
         "sc_profile_enable": ([H, c_int], c_int),
         "sc_profile_read": ([H, dp, lp, c_int], c_int),
         "sc_last_error": ([H], c_char_p),
+        "sc_copy_probe": ([c_size_t, c_int, dp], c_int),
         "sc_destroy": ([H], c_int),
         "sc_params_from_yaml": ([c_char_p, POINTER(ScParams), dp], c_int),
         "sc_params_init": ([POINTER(ScParams), c_double], c_int),

@@ -23,7 +23,7 @@ struct Bin {
   int32_t *list = nullptr;
 };
 
-constexpr int kFamilies = 4;  // pair, reduce, integrate, rebuild
+constexpr int kFamilies = 4;  // pair+scatter, (unused), integrate, rebuild
 
 }  // namespace
 
@@ -37,9 +37,11 @@ struct sc_system {
   std::vector<int32_t> site_off;
   std::vector<Bin> bins;
   int pair_grid = 1024;
-  // graph
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  // graphs: [0] odd step (rebuild stage included), [1] even step (a rebuild
+  // is only ever decided after a full step, i.e. at odd steps), [2] the
+  // odd+even pair
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   // profiling
   bool prof = false;
   hipEvent_t ev[2 * kFamilies] = {};
@@ -88,10 +90,12 @@ int dalloc(sc_system *h, T **p, size_t n, bool cand) {
 }
 
 void drop_graph(sc_system *h) {
-  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  h->gexec = nullptr;
-  h->graph = nullptr;
+  for (int k = 0; k < 3; ++k) {
+    if (h->gexec[k]) (void)hipGraphExecDestroy(h->gexec[k]);
+    if (h->graph[k]) (void)hipGraphDestroy(h->graph[k]);
+    h->gexec[k] = nullptr;
+    h->graph[k] = nullptr;
+  }
 }
 
 void free_cand(sc_system *h) {
@@ -106,17 +110,13 @@ int alloc_cand(sc_system *h, int64_t cap) {
   Dev &d = h->d;
   d.pairs = nullptr;
   d.slot64 = nullptr;
-  d.lm64 = nullptr;
   d.slot32 = nullptr;
-  d.lm32 = nullptr;
   int rc;
   if ((rc = dalloc(h, &d.pairs, cap, true))) return rc;
   if (h->p.precision == SC_PREC_FP64) {
     if ((rc = dalloc(h, &d.slot64, cap, true))) return rc;
-    if ((rc = dalloc(h, &d.lm64, cap, true))) return rc;
   } else {
     if ((rc = dalloc(h, &d.slot32, cap, true))) return rc;
-    if ((rc = dalloc(h, &d.lm32, cap, true))) return rc;
   }
   if ((rc = dalloc(h, &d.inc, 2 * cap, true))) return rc;
   h->g.cap = cap;
@@ -134,14 +134,16 @@ int write_ctl(sc_system *h, const Ctl &c) {
   return SC_OK;
 }
 
-void enqueue_step(sc_system *h) {
+// One step.  `odd` says whether the step index is odd: only odd steps can
+// rebuild, so even steps skip the rebuild stage's launches.
+void enqueue_step(sc_system *h, bool odd) {
   const Dev &d = h->d;
   const Geo &g = h->g;
   hipStream_t st = h->stream;
   const bool pf = h->prof;
   sc::launch_decide(d, g, st);
   if (pf) (void)hipEventRecord(h->ev[6], st);
-  sc::launch_rebuild(d, g, st);
+  if (odd) sc::launch_rebuild(d, g, st);
   if (pf) {
     (void)hipEventRecord(h->ev[7], st);
     (void)hipEventRecord(h->ev[0], st);
@@ -151,7 +153,6 @@ void enqueue_step(sc_system *h) {
     (void)hipEventRecord(h->ev[1], st);
     (void)hipEventRecord(h->ev[2], st);
   }
-  sc::launch_reduce(d, g, h->p.precision, st);
   if (pf) {
     (void)hipEventRecord(h->ev[3], st);
     (void)hipEventRecord(h->ev[4], st);
@@ -162,14 +163,17 @@ void enqueue_step(sc_system *h) {
   sc::launch_commit(d, st);
 }
 
-int capture_graph(sc_system *h) {
+int capture_graphs(sc_system *h) {
   drop_graph(h);
-  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_step(h);
-  hipGraph_t gr = nullptr;
-  HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
-  h->graph = gr;
-  HIPCHK(h, hipGraphInstantiate(&h->gexec, gr, nullptr, nullptr, 0));
+  for (int k = 0; k < 3; ++k) {
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    if (k == 0 || k == 2) enqueue_step(h, true);
+    if (k == 1 || k == 2) enqueue_step(h, false);
+    hipGraph_t gr = nullptr;
+    HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
+    h->graph[k] = gr;
+    HIPCHK(h, hipGraphInstantiate(&h->gexec[k], gr, nullptr, nullptr, 0));
+  }
   return SC_OK;
 }
 
@@ -298,14 +302,10 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     if ((rc = dalloc(h, a, 3 * (size_t)ns, false))) return rc;
   if ((rc = dalloc(h, &d.acc, ns, false))) return rc;
   if ((rc = dalloc(h, &d.acc_prev, ns, false))) return rc;
-  double **bond3[] = {&d.spos, &d.u, &d.pos, &d.bf, &d.bt};
-  for (double **a : bond3)
-    if ((rc = dalloc(h, a, 3 * (size_t)nb, false))) return rc;
-  if ((rc = dalloc(h, &d.len, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.br, 8 * (size_t)nb, false))) return rc;
+  if ((rc = dalloc(h, &d.bf, 3 * (size_t)nb, false))) return rc;
+  if ((rc = dalloc(h, &d.bt, 3 * (size_t)nb, false))) return rc;
   if ((rc = dalloc(h, &d.tension, nb, false))) return rc;
-  d.ub32 = nullptr;
-  if (h->p.precision == SC_PREC_MIXED)
-    if ((rc = dalloc(h, &d.ub32, nb, false))) return rc;
   if ((rc = dalloc(h, &d.keys, nb, false))) return rc;
   if ((rc = dalloc(h, &d.cell_count, g.n_cells, false))) return rc;
   if ((rc = dalloc(h, &d.cell_start, g.n_cells + 1, false))) return rc;
@@ -359,17 +359,18 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = write_ctl(h, c))) return rc;
 
   // length bins for the integrator (one launch each)
-  const int bin_max[] = {32, 64, 128, 256};
-  std::vector<std::vector<int32_t>> lists(4);
+  const int bin_max[] = {8, 16, 32, 64, 128, 256};
+  constexpr int kBins = 6;
+  std::vector<std::vector<int32_t>> lists(kBins);
   for (int f = 0; f < nf; ++f) {
     int n = site_off[f + 1] - site_off[f];
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < kBins; ++k)
       if (n <= bin_max[k]) {
         lists[k].push_back(f);
         break;
       }
   }
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < kBins; ++k) {
     if (lists[k].empty()) continue;
     Bin b;
     b.maxn = bin_max[k];
@@ -405,24 +406,36 @@ int sc_step(sc_handle h, int32_t n_steps) {
     int64_t todo = target - c.i_step;
     if (todo <= 0) return SC_OK;
     const bool use_graph = h->p.use_graph && !h->prof;
-    if (use_graph && !h->gexec)
-      if ((rc = capture_graph(h))) return rc;
-    for (int64_t s = 0; s < todo; ++s) {
+    if (use_graph && !h->gexec[0])
+      if ((rc = capture_graphs(h))) return rc;
+    int64_t next = c.i_step + 1;  // index of the next step
+    int64_t s = 0;
+    while (s < todo) {
+      const bool odd = (next & 1) != 0;
       if (use_graph) {
-        HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+        if (odd && todo - s >= 2) {
+          HIPCHK(h, hipGraphLaunch(h->gexec[2], h->stream));
+          s += 2;
+          next += 2;
+          continue;
+        }
+        HIPCHK(h, hipGraphLaunch(h->gexec[odd ? 0 : 1], h->stream));
       } else {
-        enqueue_step(h);
+        enqueue_step(h, odd);
         if (h->prof) {
           HIPCHK(h, hipEventSynchronize(h->ev[5]));
           float ms = 0.f;
           const int pairs[kFamilies][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}};
           for (int k = 0; k < kFamilies; ++k) {
+            if (k == 3 && !odd) continue;
             (void)hipEventElapsedTime(&ms, h->ev[pairs[k][0]], h->ev[pairs[k][1]]);
             h->prof_ms[k] += ms;
             h->prof_n[k] += 1;
           }
         }
       }
+      s += 1;
+      next += 1;
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipGetLastError());
@@ -458,10 +471,32 @@ int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u, double *len)
   int rc = check_handle(h, true);
   if (rc) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (pos) get3(pos, h->d.pos, h->g.nb);
-  if (spos) get3(spos, h->d.spos, h->g.nb);
-  if (u) get3(u, h->d.u, h->g.nb);
-  if (len) HIPCHK(h, hipMemcpy(len, h->d.len, sizeof(double) * h->g.nb, hipMemcpyDeviceToHost));
+  const int64_t nb = h->g.nb, ns = h->g.ns;
+  std::vector<double> rec(8 * (size_t)nb);
+  HIPCHK(h, hipMemcpy(rec.data(), h->d.br, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+  for (int64_t b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) {
+      if (spos) spos[k * nb + b] = rec[8 * b + k];
+      if (u) u[k * nb + b] = rec[8 * b + 3 + k];
+    }
+  if (len)
+    for (int64_t b = 0; b < nb; ++b) len[b] = rec[8 * b + 6];
+  if (pos) {
+    // bond centre r1 + 0.5 (r2 - r1) (src/bond.cpp:27-34), from the sites
+    std::vector<double> r(3 * (size_t)ns);
+    HIPCHK(h, hipMemcpy(r.data(), h->d.r, sizeof(double) * r.size(), hipMemcpyDeviceToHost));
+    const int nf = h->g.nf;
+    for (int f = 0; f < nf; ++f) {
+      const int64_t s0 = h->site_off[f], s1 = h->site_off[f + 1];
+      for (int64_t i = s0; i < s1 - 1; ++i) {
+        const int64_t b = i - f;
+        for (int k = 0; k < 3; ++k) {
+          const double o = r[k * ns + i + 1] - r[k * ns + i];
+          pos[k * nb + b] = r[k * ns + i] + 0.5 * o;
+        }
+      }
+    }
+  }
   return SC_OK;
 }
 
@@ -604,6 +639,40 @@ int sc_profile_read(sc_handle h, double *avg_ms, int64_t *launches, int n) {
     if (launches) launches[k] = h->prof_n[k];
   }
   return SC_OK;
+}
+
+int sc_copy_probe(size_t bytes, int repeats, double *avg_ms) {
+  if (bytes < 32 || repeats <= 0) return SC_ERR_ARG;
+  const int64_t n32 = (int64_t)(bytes / 32);
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n32 * 32) != hipSuccess) return SC_ERR_NOMEM;
+  if (hipMalloc(&b, n32 * 32) != hipSuccess) {
+    (void)hipFree(a);
+    return SC_ERR_NOMEM;
+  }
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SC_OK;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      hipMemsetAsync(a, 1, n32 * 32, st) != hipSuccess) {
+    rc = SC_ERR_HIP;
+  } else {
+    sc::launch_copy_probe(a, b, n32, st);  // warm
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < repeats; ++r) sc::launch_copy_probe(a, b, n32, st);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = SC_ERR_HIP;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (avg_ms) *avg_ms = ms / repeats;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
 }
 
 const char *sc_last_error(sc_handle h) {

@@ -1,8 +1,10 @@
 // sc_host.cpp -- host layer: reference-style YAML parameter files, derived
 // system constants, and filament generation in the reference's "random"
 // insertion style.  Pure host C++ (no device code).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -455,10 +457,18 @@ double seg_dist2(const double *c1, const double *u1, double h1, const double *c2
   return m2;
 }
 
+// Dense uniform grid of accepted bond centres (head/next linked lists).
 struct Grid {
   double L, cl;
   int n;
-  std::unordered_map<int64_t, std::vector<int>> cells;
+  std::vector<int32_t> head, next;
+  void init(double L_, double min_cell) {
+    L = L_;
+    n = std::max(1, (int)std::floor(L / min_cell));
+    if ((int64_t)n * n * n > (int64_t)64 << 20) n = 400;  // bounded memory
+    cl = L / n;
+    head.assign((size_t)n * n * n, -1);
+  }
   int idx(double x) const {
     double s = x / L - std::floor(x / L);
     int c = (int)(s * n);
@@ -469,6 +479,12 @@ struct Grid {
     y = ((y % n) + n) % n;
     z = ((z % n) + n) % n;
     return ((int64_t)x * n + y) * n + z;
+  }
+  void add(int j, const double *c) {
+    int64_t k = key(idx(c[0]), idx(c[1]), idx(c[2]));
+    if ((int)next.size() <= j) next.resize(j + 1, -1);
+    next[j] = head[k];
+    head[k] = j;
   }
 };
 
@@ -489,60 +505,60 @@ int generate(double R, int nf, const int *n_sites, const double *length,
     maxbl = std::max(maxbl, length[f] / (n_sites[f] - 1));
   }
   Grid G;
-  G.L = L;
-  G.cl = maxd + maxbl;
-  G.n = std::max(1, (int)std::floor(L / G.cl));
-  G.cl = L / G.n;
-  site_off->assign(1, 0);
-  std::vector<double> xyz;  // AoS during build
+  G.init(L, maxd + maxbl);
+  // Long filaments are placed first (into the emptier box); the output
+  // keeps the caller's filament order.
+  std::vector<int> order(nf);
+  for (int f = 0; f < nf; ++f) order[f] = f;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return n_sites[a] > n_sites[b]; });
+  std::vector<std::vector<double>> placed_pts(nf);
   std::vector<double> bc, bu, bh, bd;  // accepted bonds
-  fil->clear();
+  fil->assign(nf, sc_filament_params{});
   Rng rng(seed);
-  for (int f = 0; f < nf; ++f) {
+  int64_t total_attempts = 0;
+  for (int f : order) {
     const int ns = n_sites[f];
     const double bl = length[f] / (ns - 1);
     const double d = diameter[f];
     bool placed = false;
     std::vector<double> pts(3 * (size_t)ns);
-    for (int attempt = 0; attempt < 100000 && !placed; ++attempt) {
+    int64_t attempts_f = 0;
+    for (int attempt = 0; attempt < 1000000 && !placed; ++attempt) {
+      ++attempts_f;
       // Mesh::InitRandomBond + Filament::InsertFilament
-      // (src/mesh.cpp:244-258, src/filament.cpp:225-249)
+      // (src/mesh.cpp:244-258, src/filament.cpp:225-249), each new bond
+      // tested against the accepted bonds as it is grown
+      // (InteractionManager::CheckOverlap: dr^2 < (d1+d2)^2/4); an attempt
+      // stops at its first overlap.
       double u[3], pos[3];
       random_unit_vector(rng, u);
       for (int k = 0; k < 3; ++k) pos[k] = (2.0 * rng.uniform_pos() - 1.0) * (R - d);
       for (int k = 0; k < 3; ++k) pos[k] -= 0.5 * u[k] * length[f];
       for (int k = 0; k < 3; ++k) pts[k] = pos[k];
-      for (int k = 0; k < 3; ++k) pts[3 + k] = pts[k] + bl * u[k];
-      for (int i = 2; i < ns; ++i) {
-        probable_orientation(rng, kappa[f], bl, u);
-        for (int k = 0; k < 3; ++k) pts[3 * i + k] = pts[3 * (i - 1) + k] + bl * u[k];
-      }
-      // overlap test of the new bonds against accepted bonds
-      // (InteractionManager::CheckOverlap: dr^2 < (d1+d2)^2/4)
       bool ok = true;
-      for (int i = 0; i < ns - 1 && ok; ++i) {
+      for (int i = 1; i < ns && ok; ++i) {
+        if (i >= 2) probable_orientation(rng, kappa[f], bl, u);
+        for (int k = 0; k < 3; ++k) pts[3 * i + k] = pts[3 * (i - 1) + k] + bl * u[k];
         double c[3], uu[3], l2 = 0;
         for (int k = 0; k < 3; ++k) {
-          uu[k] = pts[3 * (i + 1) + k] - pts[3 * i + k];
+          uu[k] = pts[3 * i + k] - pts[3 * (i - 1) + k];
           l2 += uu[k] * uu[k];
         }
         double l = std::sqrt(l2);
         for (int k = 0; k < 3; ++k) {
-          c[k] = pts[3 * i + k] + 0.5 * uu[k];
+          c[k] = pts[3 * (i - 1) + k] + 0.5 * uu[k];
           uu[k] /= l;
         }
         int cx = G.idx(c[0]), cy = G.idx(c[1]), cz = G.idx(c[2]);
         for (int dx = -1; dx <= 1 && ok; ++dx)
           for (int dy = -1; dy <= 1 && ok; ++dy)
             for (int dz = -1; dz <= 1 && ok; ++dz) {
-              auto it = G.cells.find(G.key(cx + dx, cy + dy, cz + dz));
-              if (it == G.cells.end()) continue;
-              for (int j : it->second) {
+              for (int j = G.head[G.key(cx + dx, cy + dy, cz + dz)]; j >= 0 && ok;
+                   j = G.next[j]) {
                 double lim = 0.5 * (d + bd[j]);
-                if (seg_dist2(c, uu, 0.5 * l, &bc[3 * j], &bu[3 * j], bh[j], L) < lim * lim) {
+                if (seg_dist2(c, uu, 0.5 * l, &bc[3 * j], &bu[3 * j], bh[j], L) < lim * lim)
                   ok = false;
-                  break;
-                }
               }
             }
       }
@@ -566,22 +582,32 @@ int generate(double R, int nf, const int *n_sites, const double *length,
         }
         bh.push_back(0.5 * l);
         bd.push_back(d);
-        G.cells[G.key(G.idx(c[0]), G.idx(c[1]), G.idx(c[2]))].push_back(j);
+        G.add(j, c);
       }
     }
     if (!placed) {
-      *err = "insertion failed for filament " + std::to_string(f);
+      *err = "insertion failed for filament " + std::to_string(f) + " (" +
+             std::to_string(ns) + " sites) after " + std::to_string(attempts_f) + " attempts";
       return -1;
     }
-    xyz.insert(xyz.end(), pts.begin(), pts.end());
-    site_off->push_back(site_off->back() + ns);
+    total_attempts += attempts_f;
+    placed_pts[f] = pts;
     sc_filament_params q{};
     q.bond_length = bl;
     q.persistence_length = kappa[f];
     q.diameter = d;
     friction(length[f], d, ns, friction_ratio, &q.gamma_par, &q.gamma_perp);
-    fil->push_back(q);
+    (*fil)[f] = q;
   }
+  site_off->assign(1, 0);
+  std::vector<double> xyz;
+  for (int f = 0; f < nf; ++f) {
+    xyz.insert(xyz.end(), placed_pts[f].begin(), placed_pts[f].end());
+    site_off->push_back(site_off->back() + n_sites[f]);
+  }
+  if (std::getenv("SIMCORE_AMD_VERBOSE"))
+    std::fprintf(stderr, "simcore_amd: inserted %d filaments in %lld attempts\n", nf,
+                 (long long)total_attempts);
   const int64_t NS = site_off->back();
   r->assign(3 * (size_t)NS, 0.0);
   for (int64_t i = 0; i < NS; ++i)

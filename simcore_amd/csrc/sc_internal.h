@@ -3,11 +3,11 @@
 //
 // HBM layout of one system (all SoA, filament-major):
 //   sites  (Ns):  r[3], r_prev[3], f_rand[3], dr0[3], acc        FP64
-//   bonds  (Nb):  spos[3], u[3], len, pos[3], F[3], T[3], tension  FP64
-//                 ub32 float4 {u, l/2}  (mixed path only)           FP32
+//   bonds  (Nb):  record {spos[3], u[3], len, -} (64 B), F[3], T[3],
+//                 tension                                          FP64
 //   cells  (n^3): count, start (n^3+1)                              int32
-//   candidates (cap): pairs int2, slot F (4 doubles or float4),
-//                 lambda/mu (double2 or float2), incidence CSR      int32
+//   candidates (cap): pairs int2, sticky slot F (double4 or float4),
+//                 incidence CSR (2 per candidate)                   int32
 // Bonds of filament f occupy [site_off[f]-f, site_off[f+1]-f-1), so
 // "shares a site" (Bond::HasNeighbor) is |b1-b2| == 1 within a filament.
 #pragma once
@@ -63,18 +63,17 @@ struct Dev {
   // sites
   double *r, *rp, *fr, *dr0, *acc, *acc_prev;
   // bonds
-  double *spos, *u, *len, *pos, *bf, *bt, *tension;
-  float4 *ub32;            // mixed path: {u, l/2}
+  double *br;              // nb*8: {s.x, s.y, s.z, u.x, u.y, u.z, len, 0}
+  double *bf, *bt, *tension;
   // cell list
   int32_t *keys, *cell_count, *cell_start, *sorted, *skey;
   int32_t *ccount, *coff;
   // candidates
   int2 *pairs;
   double4 *slot64;         // {Fx,Fy,Fz,-}
-  double2 *lm64;           // {lambda, mu}
   float4 *slot32;          // {Fx,Fy,Fz,-}
-  float2 *lm32;
-  int32_t *deg, *inc_start, *inc;
+  int32_t *deg, *inc_start;
+  int2 *inc;               // per incidence {c << 1 | role, partner bond}
   // scan scratch
   int32_t *scan_tmp;
 };
@@ -91,16 +90,14 @@ void launch_commit(const Dev &d, hipStream_t st);
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st);
 void launch_pair(const Dev &d, const Geo &g, int precision, int grid,
                  hipStream_t st);
-void launch_reduce(const Dev &d, const Geo &g, int precision, hipStream_t st);
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
                       int bin_count, int bin_maxn, int precision,
                       hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
-void launch_pair32(const Dev &d, const Geo &g, int grid, hipStream_t st);
-void launch_reduce32(const Dev &d, const Geo &g, hipStream_t st);
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t st);
 void launch_redo_dr(const Dev &d, const Geo &g, hipStream_t st);
+void launch_copy_probe(const void *src, void *dst, int64_t n32, hipStream_t st);
 void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
                  const Ctl *ctl, int guard, hipStream_t st);
 int64_t scan_tmp_elems(int64_t n);

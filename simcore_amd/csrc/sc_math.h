@@ -285,4 +285,36 @@ __device__ __forceinline__ void noise_xi(uint64_t seed, int64_t site,
     xi[k] = ((double)c[k] + 0.5) * 2.3283064365386962890625e-10 - 0.5;
 }
 
+// Bond::ReInit + Object::UpdatePeriodic (src/bond.cpp:22-37,
+// src/object.cpp:202-210, src/linear_algebra.cpp:100-117) from the two site
+// positions; writes the 64-byte bond record {scaled position, unit
+// orientation, length}.  (Templated on the pointer/constant structs.)
+template <class DevT, class GeoT>
+__device__ __forceinline__ void bond_reinit(const DevT &d, const GeoT &g,
+                                            int64_t b, const double *r1,
+                                            const double *r2, int prec) {
+  double o[3], l = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    o[k] = r2[k] - r1[k];
+    l += o[k] * o[k];
+  }
+  l = sqrt(l);
+  double pos[3];
+  for (int k = 0; k < 3; ++k) {
+    pos[k] = r1[k] + 0.5 * o[k];
+    o[k] /= l;
+  }
+  double sp[3];
+  for (int k = 0; k < 3; ++k) {
+    double si = 0.0;
+    for (int j = 0; j < 3; ++j) si += g.hinv[3 * k + j] * pos[j];
+    si -= (double)nint(si);
+    sp[k] = si;
+  }
+  double4 *rec = reinterpret_cast<double4 *>(d.br + 8 * b);
+  rec[0] = make_double4(sp[0], sp[1], sp[2], o[0]);
+  rec[1] = make_double4(o[1], o[2], l, 0.0);
+  (void)prec;
+}
+
 }  // namespace sc

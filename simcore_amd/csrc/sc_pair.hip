@@ -1,0 +1,236 @@
+// sc_pair.hip -- pair pass and force scatter, fused and bond-centric.
+//
+// Replaces InteractionManager::CalculatePairInteractions +
+// ApplyPairInteractions (src/interaction_manager.cpp:448-517).  One thread
+// owns one bond and walks the bond's incidence list (its candidates in
+// list order, CSR built at rebuild time).  For every candidate it computes
+// the pair with the candidate's own (obj1, obj2) ordering -- the thread of
+// obj1 and the thread of obj2 run the same instruction sequence on the same
+// inputs and obtain the same bits -- updates the sticky force slot when the
+// pair is inside the cutoff, and adds +F, contact1 x F (obj1) or -F,
+// -contact2 x F (obj2) in candidate order: the serial scatter's summation
+// order, with no atomics and no second pass.
+//
+// Bond state is one 64-byte record per bond (scaled position, orientation,
+// length), so each gather touches one half cache line.
+//
+// FP64 path: bit-for-bit the reference's arithmetic.  Mixed path: minimum
+// image in FP64 from the FP64 scaled positions (box up to ~250 sigma), then
+// FP32 segment distance and WCA; near-parallel orientations
+// (1-(u1.u2)^2 < 2e-2, where FP32 line parameters lose digits) and
+// near-contact pairs (dr^2 < 1e-6) redo the pair with the FP64 algorithm.
+#include <hip/hip_runtime.h>
+
+#include "sc_internal.h"
+#include "sc_math.h"
+
+namespace sc {
+
+namespace {
+
+constexpr float kParallelSwitch = 2.0e-2f;
+
+__device__ __forceinline__ float fabs_(float x) { return x < 0.f ? -x : x; }
+__device__ __forceinline__ float fsign_(float a, float b) {
+  return b >= 0.f ? fabsf(a) : -fabsf(a);
+}
+
+struct BondRec {
+  double s[3], u[3], len;
+};
+
+__device__ __forceinline__ BondRec load_rec(const double *br, int64_t b) {
+  const double4 *p = reinterpret_cast<const double4 *>(br + 8 * b);
+  const double4 a = p[0], c = p[1];
+  BondRec r;
+  r.s[0] = a.x;
+  r.s[1] = a.y;
+  r.s[2] = a.z;
+  r.u[0] = a.w;
+  r.u[1] = c.x;
+  r.u[2] = c.y;
+  r.len = c.z;
+  return r;
+}
+
+// FP32 MinimumDistance::Sphero (src/minimum_distance.cpp:231-343) on the
+// FP32 minimum-image separation; false when the FP64 path must be used.
+__device__ __forceinline__ bool sphero32(const float *dr, const float *u1, float hl1,
+                                         const float *u2, float hl2, float *rmin,
+                                         float *rm2, float *lam_o, float *mu_o) {
+  const float a = dr[0] * u1[0] + dr[1] * u1[1] + dr[2] * u1[2];
+  const float b = dr[0] * u2[0] + dr[1] * u2[1] + dr[2] * u2[2];
+  const float c = u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2];
+  const float denom = 1.0f - c * c;
+  if (denom < kParallelSwitch) return false;
+  const float idn = 1.0f / denom;
+  float lambda = (a - c * b) * idn;
+  float mu = (-b + c * a) * idn;
+  const float lm = fabs_(lambda), mm = fabs_(mu);
+  if (lm > hl1 && mm > hl2) {
+    float la = fsign_(hl1, lambda);
+    float ma = -b + la * c;
+    if (fabs_(ma) > hl2) ma = fsign_(hl2, ma);
+    float mb = fsign_(hl2, mu);
+    float lb = a + mb * c;
+    if (fabs_(lb) > hl1) lb = fsign_(hl1, lb);
+    float m2a = 0.f, m2b = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float ra = dr[i] - la * u1[i] + ma * u2[i];
+      const float rb = dr[i] - lb * u1[i] + mb * u2[i];
+      m2a += ra * ra;
+      m2b += rb * rb;
+    }
+    if (m2a < m2b) {
+      lambda = la;
+      mu = ma;
+    } else {
+      lambda = lb;
+      mu = mb;
+    }
+  } else if (lm > hl1) {
+    lambda = fsign_(hl1, lambda);
+    mu = -b + lambda * c;
+    if (fabs_(mu) > hl2) mu = fsign_(hl2, mu);
+  } else if (mm > hl2) {
+    mu = fsign_(hl2, mu);
+    lambda = a + mu * c;
+    if (fabs_(lambda) > hl1) lambda = fsign_(hl1, lambda);
+  }
+  float m2 = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    rmin[i] = dr[i] - lambda * u1[i] + mu * u2[i];
+    m2 += rmin[i] * rmin[i];
+  }
+  if (m2 < 1.0e-6f) return false;
+  *rm2 = m2;
+  *lam_o = lambda;
+  *mu_o = mu;
+  return true;
+}
+
+}  // namespace
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_pair_fused(Dev d, Geo g) {
+  if (d.ctl->abort) return;
+  const int64_t nb = g.nb;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const BondRec own = load_rec(d.br, b);
+  double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+  const int32_t q0 = d.inc_start[b], q1 = d.inc_start[b + 1];
+  for (int32_t q = q0; q < q1; ++q) {
+    const int2 ent = d.inc[q];
+    const int32_t c = ent.x >> 1;
+    const bool second = (ent.x & 1) != 0;  // this bond is obj2 of candidate c
+    const BondRec oth = load_rec(d.br, ent.y);
+    const BondRec &r1 = second ? oth : own;
+    const BondRec &r2 = second ? own : oth;
+    double fc[3], lam, mu;
+    if (!MIXED) {
+      double dr[3], dr2;
+      sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
+      bool wrote = false;
+      if (!(dr2 > g.rcut2)) {
+        double f[3];
+        if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
+          fc[0] = f[0];
+          fc[1] = f[1];
+          fc[2] = f[2];
+          d.slot64[c] = make_double4(f[0], f[1], f[2], 0.0);
+          wrote = true;
+        }
+      }
+      if (!wrote) {
+        const double4 o = d.slot64[c];
+        fc[0] = o.x;
+        fc[1] = o.y;
+        fc[2] = o.z;
+      }
+    } else {
+      float dr32[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double ds = r2.s[k] - r1.s[k];
+        ds -= (double)nint(ds);
+        dr32[k] = (float)(g.h[4 * k] * ds);
+      }
+      const float u1[3] = {(float)r1.u[0], (float)r1.u[1], (float)r1.u[2]};
+      const float u2[3] = {(float)r2.u[0], (float)r2.u[1], (float)r2.u[2]};
+      float rmin[3], rm2, lf, mf;
+      bool wrote = false;
+      if (sphero32(dr32, u1, (float)(0.5 * r1.len), u2, (float)(0.5 * r2.len), rmin, &rm2, &lf,
+                   &mf)) {
+        lam = lf;
+        mu = mf;
+        if (!(rm2 > (float)g.rcut2)) {
+          const float rinv = 1.0f / sqrtf(rm2);
+          const float rinv2 = rinv * rinv;
+          const float r6 = rinv2 * rinv2 * rinv2;
+          float ffac = -(12.0f * (float)g.c12 * r6 - 6.0f * (float)g.c6) * r6 * rinv;
+          const float fcut = (float)g.fcut;
+          if (fabs_(ffac) > fcut) ffac = (ffac >= 0.f ? 1.f : -1.f) * fcut;
+          const float s = ffac * rinv;
+          const float4 f = make_float4(s * rmin[0], s * rmin[1], s * rmin[2], 0.f);
+          d.slot32[c] = f;
+          fc[0] = f.x;
+          fc[1] = f.y;
+          fc[2] = f.z;
+          wrote = true;
+        }
+      } else {
+        double dr[3], dr2;
+        sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
+        if (!(dr2 > g.rcut2)) {
+          double f[3];
+          if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
+            const float4 f4 = make_float4((float)f[0], (float)f[1], (float)f[2], 0.f);
+            d.slot32[c] = f4;
+            fc[0] = f4.x;
+            fc[1] = f4.y;
+            fc[2] = f4.z;
+            wrote = true;
+          }
+        }
+      }
+      if (!wrote) {
+        const float4 o = d.slot32[c];
+        fc[0] = o.x;
+        fc[1] = o.y;
+        fc[2] = o.z;
+      }
+    }
+    // contact of this bond and the torque cross product
+    // (src/interaction_manager.cpp:469-470, src/minimum_distance.cpp:339-342)
+    double con[3], t[3];
+    const double sc = second ? mu : lam;
+    for (int k = 0; k < 3; ++k) con[k] = sc * own.u[k];
+    cross3(con, fc, t);
+    if (!second) {
+      for (int k = 0; k < 3; ++k) F[k] += fc[k];
+      for (int k = 0; k < 3; ++k) T[k] += t[k];
+    } else {
+      for (int k = 0; k < 3; ++k) F[k] -= fc[k];
+      for (int k = 0; k < 3; ++k) T[k] -= t[k];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    d.bf[k * nb + b] = F[k];
+    d.bt[k * nb + b] = T[k];
+  }
+}
+
+void launch_pair(const Dev &d, const Geo &g, int precision, int /*grid*/,
+                 hipStream_t st) {
+  const int64_t blocks = (g.nb + 255) / 256;
+  const dim3 grid((unsigned)(blocks > 0 ? blocks : 1)), block(256);
+  if (precision == SC_PREC_FP64)
+    hipLaunchKernelGGL(k_pair_fused<false>, grid, block, 0, st, d, g);
+  else
+    hipLaunchKernelGGL(k_pair_fused<true>, grid, block, 0, st, d, g);
+}
+
+}  // namespace sc

@@ -232,5 +232,5 @@ class FilamentSystem:
         ms = (c_double * 4)()
         n = (c_int64 * 4)()
         _check(lib.sc_profile_read(self.h, ms, n, 4), self.h, "sc_profile_read")
-        names = ["pair", "reduce", "integrate", "rebuild"]
+        names = ["pair", "unused", "integrate", "rebuild"]
         return {k: (ms[i], n[i]) for i, k in enumerate(names)}

@@ -139,3 +139,29 @@ def test_set_sites_keeps_trajectories_identical():
     for a, b in zip(dev.sites(), ora.sites()):
         np.testing.assert_array_equal(a, b)
     assert dev.counters()["n_rebuilds"] == ora.counters()["n_rebuilds"]
+
+
+def test_mixed_trajectory_within_tolerance():
+    """Mixed path (FP32 pair math, FP64 integrator with the algebraically
+    reduced forms and parallel cyclic reduction) against the FP64 CPU path
+    from the same state and noise: positions within 1e-8 sigma after one
+    step and 1e-5 sigma after 100 steps; candidate sets identical."""
+    print("case: C0 mixed trajectory 1 and 100 steps", flush=True)
+    p, fil, dev, ora = _pair(precision="mixed")
+    dev.step(1)
+    assert ora.step(1) == 0
+    d1 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 1 step", d1)
+    assert d1 <= 1e-8
+    np.testing.assert_allclose(dev.tensions(), ora.tensions(), rtol=1e-5,
+                               atol=1e-5 * np.abs(ora.tensions()).max())
+    dev.step(99)
+    assert ora.step(99) == 0
+    d100 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 100 steps", d100)
+    assert d100 <= 1e-5
+    assert dev.counters()["n_rebuilds"] == ora.counters()["n_rebuilds"]
+    db1, db2 = dev.candidates()
+    ob1, ob2 = ora.candidates()
+    np.testing.assert_array_equal(db1, ob1)
+    np.testing.assert_array_equal(db2, ob2)
