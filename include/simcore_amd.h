@@ -98,6 +98,16 @@ int sc_load_filaments(sc_handle h, int32_t n_filaments,
  * (src/interaction_manager.cpp:63-95) and FilamentSpecies::UpdatePositions
  * (src/filament_species.cpp:59-111), honouring the midstep parity. */
 int sc_step(sc_handle h, int32_t n_steps);
+/* sc_step split in two, for callers that overlap host work or other systems
+ * with the device (the reference's loop is synchronous; the split itself
+ * has no counterpart there).  sc_step_async enqueues n steps behind any
+ * already enqueued and returns at once; sc_step_poll never blocks: *done is
+ * 1 when every enqueued step has completed.  Candidate-capacity growth and
+ * the "Unphysical result" error are handled and reported by sc_step_poll.
+ * The getters wait for enqueued work; sc_set_sites refuses while steps are
+ * in flight. */
+int sc_step_async(sc_handle h, int32_t n_steps);
+int sc_step_poll(sc_handle h, int *done);
 
 /* Object::GetPosition / GetPrevPosition of sites (include/simcore/
  * object.hpp:82-88); f_rand is Site::random_force_ (src/site.cpp:103-107).
@@ -150,6 +160,66 @@ int sc_copy_probe(size_t bytes, int repeats, double *avg_ms);
 
 const char *sc_last_error(sc_handle h);
 int sc_destroy(sc_handle h);
+
+/* ---- ensemble runner ------------------------------------------------------ */
+/* Serves independent systems of different sizes and seeds (the reference's
+ * n_runs use case, src/simulation_manager.cpp:461-545, which runs them one
+ * after another): requests are queued, admitted while the runner's device
+ * memory (one arena, fixed at creation) and its concurrency limit allow,
+ * kept resident on the device, advanced on their own streams so their work
+ * overlaps, and retired -- final positions copied to the host, memory
+ * returned -- as they finish.  Admission is first-fit in arrival order.
+ * A system's result does not depend on what else runs beside it. */
+typedef struct sc_ensemble_s *sc_ensemble;
+
+typedef struct sc_ensemble_params {
+  size_t hbm_budget;        /* arena bytes; 0: 80% of the free device memory */
+  int32_t max_active;       /* systems resident at once; 0: 16             */
+  int32_t steps_per_launch; /* steps enqueued per system per round; 0: 64  */
+} sc_ensemble_params;
+
+/* request status */
+#define SC_REQ_QUEUED 0
+#define SC_REQ_RUNNING 1
+#define SC_REQ_DONE 2
+/* negative: failed with that SC_ERR_* code */
+
+typedef struct sc_request_info {
+  int32_t status;
+  int32_t n_filaments;
+  int64_t n_sites, n_bonds, n_steps, steps_done;
+  double t_submit, t_admit, t_done; /* seconds on the ensemble clock       */
+  size_t device_bytes;              /* footprint while resident            */
+} sc_request_info;
+
+typedef struct sc_ensemble_stats {
+  int64_t submitted, completed, failed, running, queued;
+  double latency_p50, latency_p99, latency_mean, latency_max; /* s, done  */
+  double segment_steps;  /* bonds x steps of the completed requests        */
+  double span;           /* first submission to last completion, s         */
+  int32_t peak_active;   /* most systems resident at once                  */
+  size_t peak_bytes;     /* most arena bytes in use at once                */
+  size_t arena_bytes;
+} sc_ensemble_stats;
+
+int sc_ensemble_create(const sc_ensemble_params *params, sc_ensemble *out);
+/* Queues one system (inputs copied).  t_submit < 0: now. */
+int sc_ensemble_submit(sc_ensemble e, const sc_params *params,
+                       int32_t n_filaments, const int32_t *site_off,
+                       const double *r, const sc_filament_params *fil,
+                       int64_t n_steps, double t_submit, int64_t *request_id);
+/* One scheduling round; never waits for the device. */
+int sc_ensemble_progress(sc_ensemble e, int32_t *n_running, int32_t *n_queued);
+/* Rounds until nothing is queued or running, or timeout_s (<= 0: none). */
+int sc_ensemble_drain(sc_ensemble e, double timeout_s);
+int sc_ensemble_request(sc_ensemble e, int64_t request_id, sc_request_info *info);
+/* Final site positions (3*n_sites, SoA) of a finished request. */
+int sc_ensemble_result(sc_ensemble e, int64_t request_id, double *r);
+int sc_ensemble_stats_get(sc_ensemble e, sc_ensemble_stats *out);
+/* Seconds since sc_ensemble_create (the clock of t_submit/t_admit/t_done). */
+double sc_ensemble_clock(sc_ensemble e);
+const char *sc_ensemble_last_error(sc_ensemble e);
+int sc_ensemble_destroy(sc_ensemble e);
 
 /* ---- host layer: parameter files and system generation ------------------- */
 

@@ -8,9 +8,10 @@ from ._lib import (SC_PREC_FP64, SC_PREC_MIXED, EXPORTS, LIB_PATH, ScFilamentPar
                    ScParams, SimcoreError, UnphysicalResult, lib)
 from .system import (FilamentSystem, Filaments, filament_layout, generate_filaments,
                      params_from_yaml, params_init)
+from .ensemble import Ensemble
 
 __all__ = [
-    "FilamentSystem", "Filaments", "generate_filaments", "params_init",
+    "FilamentSystem", "Filaments", "Ensemble", "generate_filaments", "params_init",
     "params_from_yaml", "filament_layout", "ScParams", "ScFilamentParams",
     "SimcoreError", "UnphysicalResult", "SC_PREC_FP64", "SC_PREC_MIXED",
     "EXPORTS", "LIB_PATH", "lib",

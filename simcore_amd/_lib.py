@@ -29,8 +29,16 @@ EXPORTS = [
     "sc_get_dr_state", "sc_get_counters", "sc_set_sites", "sc_device_bytes",
     "sc_profile_enable", "sc_profile_read", "sc_last_error", "sc_destroy",
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
-    "sc_generate_filaments", "sc_copy_probe",
+    "sc_generate_filaments", "sc_copy_probe", "sc_step_async", "sc_step_poll",
+    "sc_ensemble_create", "sc_ensemble_submit", "sc_ensemble_progress",
+    "sc_ensemble_drain", "sc_ensemble_request", "sc_ensemble_result",
+    "sc_ensemble_stats_get", "sc_ensemble_clock", "sc_ensemble_last_error",
+    "sc_ensemble_destroy",
 ]
+
+SC_REQ_QUEUED = 0
+SC_REQ_RUNNING = 1
+SC_REQ_DONE = 2
 
 
 class ScParams(ctypes.Structure):
@@ -51,6 +59,28 @@ class ScFilamentParams(ctypes.Structure):
         ("gamma_par", c_double), ("gamma_perp", c_double),
         ("diameter", c_double),
     ]
+
+
+class ScEnsembleParams(ctypes.Structure):
+    _fields_ = [("hbm_budget", c_size_t), ("max_active", c_int32),
+                ("steps_per_launch", c_int32)]
+
+
+class ScRequestInfo(ctypes.Structure):
+    _fields_ = [("status", c_int32), ("n_filaments", c_int32),
+                ("n_sites", c_int64), ("n_bonds", c_int64), ("n_steps", c_int64),
+                ("steps_done", c_int64), ("t_submit", c_double), ("t_admit", c_double),
+                ("t_done", c_double), ("device_bytes", c_size_t)]
+
+
+class ScEnsembleStats(ctypes.Structure):
+    _fields_ = [("submitted", c_int64), ("completed", c_int64), ("failed", c_int64),
+                ("running", c_int64), ("queued", c_int64),
+                ("latency_p50", c_double), ("latency_p99", c_double),
+                ("latency_mean", c_double), ("latency_max", c_double),
+                ("segment_steps", c_double), ("span", c_double),
+                ("peak_active", c_int32), ("peak_bytes", c_size_t),
+                ("arena_bytes", c_size_t)]
 
 
 class SimcoreError(RuntimeError):
@@ -78,6 +108,8 @@ def _load():
         "sc_create": ([POINTER(ScParams), POINTER(H)], c_int),
         "sc_load_filaments": ([H, c_int32, ip, dp, POINTER(ScFilamentParams)], c_int),
         "sc_step": ([H, c_int32], c_int),
+        "sc_step_async": ([H, c_int32], c_int),
+        "sc_step_poll": ([H, POINTER(c_int)], c_int),
         "sc_get_sites": ([H, dp, dp, dp], c_int),
         "sc_get_bonds": ([H, dp, dp, dp, dp], c_int),
         "sc_get_bond_forces": ([H, dp, dp], c_int),
@@ -94,6 +126,17 @@ def _load():
         "sc_last_error": ([H], c_char_p),
         "sc_copy_probe": ([c_size_t, c_int, dp], c_int),
         "sc_destroy": ([H], c_int),
+        "sc_ensemble_create": ([POINTER(ScEnsembleParams), POINTER(H)], c_int),
+        "sc_ensemble_submit": ([H, POINTER(ScParams), c_int32, ip, dp,
+                                POINTER(ScFilamentParams), c_int64, c_double, lp], c_int),
+        "sc_ensemble_progress": ([H, ip, ip], c_int),
+        "sc_ensemble_drain": ([H, c_double], c_int),
+        "sc_ensemble_request": ([H, c_int64, POINTER(ScRequestInfo)], c_int),
+        "sc_ensemble_result": ([H, c_int64, dp], c_int),
+        "sc_ensemble_stats_get": ([H, POINTER(ScEnsembleStats)], c_int),
+        "sc_ensemble_clock": ([H], c_double),
+        "sc_ensemble_last_error": ([H], c_char_p),
+        "sc_ensemble_destroy": ([H], c_int),
         "sc_params_from_yaml": ([c_char_p, POINTER(ScParams), dp], c_int),
         "sc_params_init": ([POINTER(ScParams), c_double], c_int),
         "sc_filament_layout": ([c_double, c_double, c_double, c_double, ip, dp], c_int),

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "sc_internal.h"
+#include "sc_system.h"
 
 using sc::Ctl;
 using sc::Dev;
@@ -47,13 +48,36 @@ struct sc_system {
   hipEvent_t ev[2 * kFamilies] = {};
   double prof_ms[kFamilies] = {};
   int64_t prof_n[kFamilies] = {};
-  // allocations
+  // asynchronous stepping: work enqueued up to step index `target`; the
+  // pinned Ctl mirror is refreshed at the end of every enqueued range and
+  // `done_ev` marks the end of the last one
+  int64_t i_step_host = 0;  // i_step when nothing is pending
+  int64_t target = 0;
+  bool pending = false;
+  int grow_rounds = 0;
+  hipEvent_t done_ev = nullptr;
+  Ctl *ctl_host = nullptr;
+  // allocations (from `arena` when set, else hipMalloc)
+  sc::Arena *arena = nullptr;
+  bool own_stream = true;
   std::vector<void *> fixed_allocs;
   std::vector<void *> cand_allocs;
   size_t fixed_bytes = 0, cand_bytes = 0;
 };
 
 namespace {
+
+// InteractionManager::InitInteractions -> CellList::SetMinCellLength /
+// Init (src/interaction_manager.cpp:46-54, src/cell_list.cpp:9-64)
+int cells_per_side(const sc_params &p) {
+  const double rcut = std::pow(2.0, 1.0 / 6.0) * p.wca_sig;
+  double min_len = p.min_cell_length;
+  const double rc_len = std::sqrt(rcut * rcut);
+  if (rc_len > min_len) min_len = rc_len;
+  int n1d = (int)std::floor(2 * p.system_radius / min_len);
+  if (n1d <= 3) n1d = 3;
+  return n1d;
+}
 
 int fail(sc_system *h, int code, const std::string &msg) {
   if (h) h->err = msg;
@@ -68,15 +92,30 @@ int fail(sc_system *h, int code, const std::string &msg) {
                   std::string(#expr) + ": " + hipGetErrorString(e_));         \
   } while (0)
 
+void dfree(sc_system *h, void *v) {
+  if (h->arena)
+    h->arena->put(v);
+  else
+    (void)hipFree(v);
+}
+
 template <typename T>
 int dalloc(sc_system *h, T **p, size_t n, bool cand) {
-  size_t bytes = sizeof(T) * (n ? n : 1);
+  size_t bytes = sc::Arena::round(sizeof(T) * (n ? n : 1));
   void *v = nullptr;
-  hipError_t e = hipMalloc(&v, bytes);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(h, SC_ERR_NOMEM,
-                "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  if (h->arena) {
+    v = h->arena->get(bytes);
+    if (!v) {
+      *p = nullptr;
+      return fail(h, SC_ERR_NOMEM, "arena exhausted (" + std::to_string(bytes) + " bytes)");
+    }
+  } else {
+    hipError_t e = hipMalloc(&v, bytes);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return fail(h, SC_ERR_NOMEM,
+                  "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
   }
   *p = static_cast<T *>(v);
   if (cand) {
@@ -99,7 +138,7 @@ void drop_graph(sc_system *h) {
 }
 
 void free_cand(sc_system *h) {
-  for (void *v : h->cand_allocs) (void)hipFree(v);
+  for (void *v : h->cand_allocs) dfree(h, v);
   h->cand_allocs.clear();
   h->cand_bytes = 0;
 }
@@ -123,14 +162,17 @@ int alloc_cand(sc_system *h, int64_t cap) {
   return SC_OK;
 }
 
+// Control-block transfers, ordered on the system's stream.
 Ctl read_ctl(sc_system *h) {
   Ctl c{};
-  (void)hipMemcpy(&c, h->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost);
+  (void)hipMemcpyAsync(&c, h->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream);
+  (void)hipStreamSynchronize(h->stream);
   return c;
 }
 
 int write_ctl(sc_system *h, const Ctl &c) {
-  HIPCHK(h, hipMemcpy(h->d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpyAsync(h->d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return SC_OK;
 }
 
@@ -213,6 +255,48 @@ void get3(double *dst, const double *src_dev, int64_t n) {
   (void)hipMemcpy(dst, src_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
 }
 
+// Enqueues `count` steps after step index `from` (the next step is
+// from+1), then the Ctl mirror copy and the completion event.
+int enqueue_range(sc_system *h, int64_t from, int64_t count) {
+  const bool use_graph = h->p.use_graph && !h->prof;
+  int rc;
+  if (use_graph && !h->gexec[0])
+    if ((rc = capture_graphs(h))) return rc;
+  int64_t next = from + 1;
+  int64_t s = 0;
+  while (s < count) {
+    const bool odd = (next & 1) != 0;
+    if (use_graph) {
+      if (odd && count - s >= 2) {
+        HIPCHK(h, hipGraphLaunch(h->gexec[2], h->stream));
+        s += 2;
+        next += 2;
+        continue;
+      }
+      HIPCHK(h, hipGraphLaunch(h->gexec[odd ? 0 : 1], h->stream));
+    } else {
+      enqueue_step(h, odd);
+      if (h->prof) {
+        HIPCHK(h, hipEventSynchronize(h->ev[5]));
+        float ms = 0.f;
+        const int pairs[kFamilies][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}};
+        for (int k = 0; k < kFamilies; ++k) {
+          if (k == 3 && !odd) continue;
+          (void)hipEventElapsedTime(&ms, h->ev[pairs[k][0]], h->ev[pairs[k][1]]);
+          h->prof_ms[k] += ms;
+          h->prof_n[k] += 1;
+        }
+      }
+    }
+    s += 1;
+    next += 1;
+  }
+  HIPCHK(h, hipMemcpyAsync(h->ctl_host, h->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipEventRecord(h->done_ev, h->stream));
+  h->pending = true;
+  return SC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -220,6 +304,15 @@ extern "C" {
 int sc_abi_version(void) { return SC_ABI_VERSION; }
 
 int sc_create(const sc_params *params, sc_handle *out) {
+  return sc::system_create(params, nullptr, nullptr, out);
+}
+
+}  // extern "C"
+
+namespace sc {
+
+int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
+                  sc_handle *out) {
   if (!params || !out) return SC_ERR_ARG;
   *out = nullptr;
   const sc_params &p = *params;
@@ -246,13 +339,7 @@ int sc_create(const sc_params *params, sc_handle *out) {
   g.c6 = 4.0 * p.wca_eps * std::pow(p.wca_sig, 6.0);
   g.fcut = p.f_cutoff;
   g.delta = p.delta;
-  // InteractionManager::InitInteractions -> CellList::SetMinCellLength /
-  // Init (src/interaction_manager.cpp:46-54, src/cell_list.cpp:9-64)
-  double min_len = p.min_cell_length;
-  double rc_len = std::sqrt(g.rcut2);
-  if (rc_len > min_len) min_len = rc_len;
-  int n1d = (int)std::floor(2 * p.system_radius / min_len);
-  if (n1d <= 3) n1d = 3;
+  const int n1d = cells_per_side(p);
   double cell_len = (double)2 * p.system_radius / n1d;
   g.n1d = n1d;
   g.thr = 0.25 * cell_len * cell_len;
@@ -263,13 +350,50 @@ int sc_create(const sc_params *params, sc_handle *out) {
   }
   g.stoch = p.stoch_flag ? 1 : 0;
   g.seed = p.seed;
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+  h->arena = arena;
+  if (stream) {
+    h->stream = stream;
+    h->own_stream = false;
+  } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return SC_ERR_HIP;
   }
+  void *pin = nullptr;
+  if (hipEventCreateWithFlags(&h->done_ev, hipEventDisableTiming) != hipSuccess ||
+      hipHostMalloc(&pin, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) {
+    sc_destroy(h);
+    return SC_ERR_HIP;
+  }
+  h->ctl_host = static_cast<Ctl *>(pin);
+  *h->ctl_host = Ctl{};
   *out = h;
   return SC_OK;
 }
+
+size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, int n_bins) {
+  const int64_t n1d = cells_per_side(*params);
+  const int64_t n_cells = n1d * n1d * n1d;
+  const int64_t nb = ns - nf;
+  auto R = [](int64_t n) { return Arena::round((size_t)(n > 0 ? n : 1)); };
+  size_t b = R(sizeof(Ctl)) + R(4 * (nf + 1)) + R(64 * (int64_t)nf) + R(8 * (int64_t)nf) +
+             R(4 * nb);
+  b += 4 * R(24 * ns) + 2 * R(8 * ns);                    // r, rp, fr, dr0, acc, acc_prev
+  b += R(64 * nb) + 2 * R(24 * nb) + R(8 * nb);           // record, F, T, tension
+  b += R(4 * nb) + R(4 * n_cells) + R(4 * (n_cells + 1));  // keys, cell count/start
+  b += 3 * R(4 * nb) + R(4 * (nb + 1)) + R(4 * nb) + R(4 * (nb + 1));
+  const int64_t tmp = std::max(scan_tmp_elems(n_cells), scan_tmp_elems(nb));
+  b += R(4 * tmp);
+  b += R(4 * (int64_t)nf) + (size_t)n_bins * Arena::kAlign;  // bin lists
+  const double cf = params->capacity_factor > 0 ? params->capacity_factor : 4.0;
+  const int64_t cap = (int64_t)(cf * nb) + 4096;
+  const int64_t slot = params->precision == SC_PREC_FP64 ? 32 : 16;
+  b += R(8 * cap) + R(slot * cap) + R(16 * cap);
+  return b;
+}
+
+}  // namespace sc
+
+extern "C" {
 
 int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
                       const double *r, const sc_filament_params *fil) {
@@ -340,21 +464,21 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     max_n = std::max(max_n, n);
     for (int i = 0; i < n - 1; ++i) bond_fil[site_off[f] - f + i] = f;
   }
-  HIPCHK(h, hipMemcpy(d.site_off, site_off, sizeof(int32_t) * (nf + 1), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(d.fpar, fpar.data(), sizeof(double) * fpar.size(), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(d.norm, norm.data(), sizeof(int32_t) * norm.size(), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(d.bond_fil, bond_fil.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(d.r, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(d.rp, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(d.dr0, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemset(d.fr, 0, sizeof(double) * 3 * ns));
-  HIPCHK(h, hipMemset(d.acc, 0, sizeof(double) * ns));
-  HIPCHK(h, hipMemset(d.acc_prev, 0, sizeof(double) * ns));
-  HIPCHK(h, hipMemset(d.bf, 0, sizeof(double) * 3 * nb));
-  HIPCHK(h, hipMemset(d.bt, 0, sizeof(double) * 3 * nb));
-  HIPCHK(h, hipMemset(d.tension, 0, sizeof(double) * nb));
-  HIPCHK(h, hipMemset(d.cell_count, 0, sizeof(int32_t) * g.n_cells));
-  HIPCHK(h, hipMemset(d.deg, 0, sizeof(int32_t) * nb));
+  HIPCHK(h, hipMemcpyAsync(d.site_off, site_off, sizeof(int32_t) * (nf + 1), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.fpar, fpar.data(), sizeof(double) * fpar.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.norm, norm.data(), sizeof(int32_t) * norm.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.bond_fil, bond_fil.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.r, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.rp, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.dr0, r, sizeof(double) * 3 * ns, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.fr, 0, sizeof(double) * 3 * ns, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.acc, 0, sizeof(double) * ns, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.acc_prev, 0, sizeof(double) * ns, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.bf, 0, sizeof(double) * 3 * nb, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.bt, 0, sizeof(double) * 3 * nb, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.tension, 0, sizeof(double) * nb, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.cell_count, 0, sizeof(int32_t) * g.n_cells, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.deg, 0, sizeof(int32_t) * nb, h->stream));
   Ctl c{};
   if ((rc = write_ctl(h, c))) return rc;
 
@@ -376,8 +500,8 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     b.maxn = bin_max[k];
     b.count = (int)lists[k].size();
     if ((rc = dalloc(h, &b.list, lists[k].size(), false))) return rc;
-    HIPCHK(h, hipMemcpy(b.list, lists[k].data(), sizeof(int32_t) * lists[k].size(),
-                        hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpyAsync(b.list, lists[k].data(), sizeof(int32_t) * lists[k].size(),
+                             hipMemcpyHostToDevice, h->stream));
     h->bins.push_back(b);
   }
   double cf = h->p.capacity_factor > 0 ? h->p.capacity_factor : 4.0;
@@ -391,70 +515,84 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipGetLastError());
   if ((rc = initial_rebuild(h))) return rc;
+  *h->ctl_host = read_ctl(h);
+  h->i_step_host = h->ctl_host->i_step;
   h->loaded = true;
   return SC_OK;
 }
 
 int sc_step(sc_handle h, int32_t n_steps) {
+  int rc = sc_step_async(h, n_steps);
+  if (rc) return rc;
+  for (;;) {
+    HIPCHK(h, hipEventSynchronize(h->done_ev));
+    int done = 0;
+    if ((rc = sc_step_poll(h, &done))) return rc;
+    if (done) return SC_OK;
+  }
+}
+
+int sc_step_async(sc_handle h, int32_t n_steps) {
   int rc = check_handle(h, true);
   if (rc) return rc;
   if (n_steps < 0) return fail(h, SC_ERR_ARG, "negative step count");
-  Ctl c = read_ctl(h);
-  if (c.abort) return fail(h, c.err ? c.err : SC_ERR_STATE, "handle is in an error state");
-  int64_t target = c.i_step + n_steps;
-  for (int attempt = 0; attempt < 32; ++attempt) {
-    int64_t todo = target - c.i_step;
-    if (todo <= 0) return SC_OK;
-    const bool use_graph = h->p.use_graph && !h->prof;
-    if (use_graph && !h->gexec[0])
-      if ((rc = capture_graphs(h))) return rc;
-    int64_t next = c.i_step + 1;  // index of the next step
-    int64_t s = 0;
-    while (s < todo) {
-      const bool odd = (next & 1) != 0;
-      if (use_graph) {
-        if (odd && todo - s >= 2) {
-          HIPCHK(h, hipGraphLaunch(h->gexec[2], h->stream));
-          s += 2;
-          next += 2;
-          continue;
-        }
-        HIPCHK(h, hipGraphLaunch(h->gexec[odd ? 0 : 1], h->stream));
-      } else {
-        enqueue_step(h, odd);
-        if (h->prof) {
-          HIPCHK(h, hipEventSynchronize(h->ev[5]));
-          float ms = 0.f;
-          const int pairs[kFamilies][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}};
-          for (int k = 0; k < kFamilies; ++k) {
-            if (k == 3 && !odd) continue;
-            (void)hipEventElapsedTime(&ms, h->ev[pairs[k][0]], h->ev[pairs[k][1]]);
-            h->prof_ms[k] += ms;
-            h->prof_n[k] += 1;
-          }
-        }
-      }
-      s += 1;
-      next += 1;
-    }
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    HIPCHK(h, hipGetLastError());
-    c = read_ctl(h);
-    if (!c.abort) continue;
-    if (c.overflow) {
-      int64_t ncap = std::max<int64_t>(2 * h->g.cap, c.nc_need + c.nc_need / 4 + 1024);
-      if ((rc = alloc_cand(h, ncap))) return rc;
-      h->pair_grid = (int)std::min<int64_t>((ncap + 255) / 256, 8192);
-      c.abort = 0;
-      c.overflow = 0;
-      if ((rc = write_ctl(h, c))) return rc;
-      continue;
-    }
-    if (c.err == SC_ERR_UNPHYSICAL)
-      return fail(h, SC_ERR_UNPHYSICAL, "Unphysical result: dr > 10000");
-    return fail(h, c.err ? c.err : SC_ERR_STATE, "step aborted");
+  if (!h->pending) {
+    if (h->ctl_host->abort)
+      return fail(h, h->ctl_host->err ? h->ctl_host->err : SC_ERR_STATE,
+                  "handle is in an error state");
+    h->target = h->i_step_host;
+    h->grow_rounds = 0;
   }
-  return fail(h, SC_ERR_NOMEM, "candidate capacity growth did not converge");
+  if (n_steps == 0) return SC_OK;
+  const int64_t from = h->target;
+  h->target += n_steps;
+  return enqueue_range(h, from, n_steps);
+}
+
+int sc_step_poll(sc_handle h, int *done) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (done) *done = 0;
+  if (!h->pending) {
+    if (done) *done = 1;
+    return SC_OK;
+  }
+  const hipError_t q = hipEventQuery(h->done_ev);
+  if (q == hipErrorNotReady) return SC_OK;
+  if (q != hipSuccess)
+    return fail(h, SC_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+  HIPCHK(h, hipGetLastError());
+  Ctl c = *h->ctl_host;
+  h->i_step_host = c.i_step;
+  if (!c.abort) {
+    h->pending = false;
+    if (done) *done = 1;
+    return SC_OK;
+  }
+  if (c.overflow && h->grow_rounds < 32) {
+    // the step that overflowed left the state untouched (every kernel after
+    // the capacity check returned): grow the candidate buffers and run the
+    // remaining steps again from there
+    ++h->grow_rounds;
+    int64_t ncap = std::max<int64_t>(2 * h->g.cap, c.nc_need + c.nc_need / 4 + 1024);
+    if ((rc = alloc_cand(h, ncap))) {
+      h->pending = false;
+      return rc;
+    }
+    h->pair_grid = (int)std::min<int64_t>((ncap + 255) / 256, 8192);
+    c.abort = 0;
+    c.overflow = 0;
+    if ((rc = write_ctl(h, c))) {
+      h->pending = false;
+      return rc;
+    }
+    return enqueue_range(h, c.i_step, h->target - c.i_step);
+  }
+  h->pending = false;
+  if (c.overflow) return fail(h, SC_ERR_NOMEM, "candidate capacity growth did not converge");
+  if (c.err == SC_ERR_UNPHYSICAL)
+    return fail(h, SC_ERR_UNPHYSICAL, "Unphysical result: dr > 10000");
+  return fail(h, c.err ? c.err : SC_ERR_STATE, "step aborted");
 }
 
 int sc_get_sites(sc_handle h, double *r, double *r_prev, double *f_rand) {
@@ -598,6 +736,7 @@ int sc_set_sites(sc_handle h, const double *r) {
   int rc = check_handle(h, true);
   if (rc) return rc;
   if (!r) return fail(h, SC_ERR_ARG, "null input");
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(h->d.r, r, sizeof(double) * 3 * h->g.ns, hipMemcpyHostToDevice));
   sc::launch_bond_init(h->d, h->g, h->p.precision, h->stream);
@@ -685,10 +824,12 @@ int sc_destroy(sc_handle h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graph(h);
   free_cand(h);
-  for (void *v : h->fixed_allocs) (void)hipFree(v);
+  for (void *v : h->fixed_allocs) dfree(h, v);
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->done_ev) (void)hipEventDestroy(h->done_ev);
+  if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+  if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return SC_OK;
 }

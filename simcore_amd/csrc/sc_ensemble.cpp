@@ -1,0 +1,443 @@
+// sc_ensemble.cpp -- ensemble runner: continuous batching of independent
+// filament systems on one device.
+//
+// The reference runs the n_runs of a parameter file one after another
+// (SimulationManager::RunSimulations, src/simulation_manager.cpp:461-545).
+// Here requests arrive at any time and share the device: a request is
+// admitted when a stream slot is free and its footprint fits the runner's
+// arena, its state stays resident until its last step, its steps are
+// enqueued in chunks on its own stream (work of different systems overlaps
+// on the device), and it is retired -- final positions copied out, arena
+// blocks returned -- as soon as it finishes, which lets the next request in.
+// Systems never share buffers or streams while resident, so a result is the
+// same whatever else runs beside it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "sc_system.h"
+
+namespace sc {
+
+int Arena::init(size_t bytes) {
+  release();
+  bytes = round(bytes);
+  void *p = nullptr;
+  if (bytes == 0 || hipMalloc(&p, bytes) != hipSuccess) return SC_ERR_NOMEM;
+  base_ = static_cast<char *>(p);
+  cap_ = bytes;
+  free_.clear();
+  free_[0] = bytes;
+  taken_.clear();
+  used_bytes_ = 0;
+  return SC_OK;
+}
+
+void Arena::release() {
+  if (base_) (void)hipFree(base_);
+  base_ = nullptr;
+  cap_ = 0;
+  used_bytes_ = 0;
+  free_.clear();
+  taken_.clear();
+}
+
+void *Arena::get(size_t bytes) {
+  bytes = round(bytes ? bytes : 1);
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < bytes) continue;
+    const size_t off = it->first, sz = it->second;
+    free_.erase(it);
+    if (sz > bytes) free_[off + bytes] = sz - bytes;
+    taken_[off] = bytes;
+    used_bytes_ += bytes;
+    return base_ + off;
+  }
+  return nullptr;
+}
+
+void Arena::put(void *p) {
+  if (!p || !base_) return;
+  const size_t off = static_cast<size_t>(static_cast<char *>(p) - base_);
+  auto t = taken_.find(off);
+  if (t == taken_.end()) return;
+  const size_t sz = t->second;
+  taken_.erase(t);
+  used_bytes_ -= sz;
+  auto it = free_.emplace(off, sz).first;
+  auto nx = std::next(it);
+  if (nx != free_.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    free_.erase(nx);
+  }
+  if (it != free_.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      free_.erase(it);
+    }
+  }
+}
+
+size_t Arena::largest_free() const {
+  size_t m = 0;
+  for (const auto &kv : free_) m = std::max(m, kv.second);
+  return m;
+}
+
+}  // namespace sc
+
+namespace {
+
+struct Request {
+  int64_t id = 0;
+  sc_params p{};
+  int32_t nf = 0;
+  int64_t ns = 0;
+  std::vector<int32_t> site_off;
+  std::vector<double> r;
+  std::vector<sc_filament_params> fil;
+  int n_bins = 0;
+  int64_t n_steps = 0;
+  int64_t enqueued = 0;
+  int32_t status = SC_REQ_QUEUED;
+  double t_submit = 0, t_admit = -1, t_done = -1;
+  size_t est_bytes = 0, bytes = 0;
+  sc_handle h = nullptr;
+  int slot = -1;
+  std::vector<double> r_out;
+  std::string err;
+};
+
+int count_bins(const std::vector<int32_t> &so) {
+  const int bin_max[] = {8, 16, 32, 64, 128, 256};
+  bool used[6] = {};
+  for (size_t f = 0; f + 1 < so.size(); ++f) {
+    const int n = so[f + 1] - so[f];
+    for (int k = 0; k < 6; ++k)
+      if (n <= bin_max[k]) {
+        used[k] = true;
+        break;
+      }
+  }
+  int c = 0;
+  for (bool u : used) c += u;
+  return c;
+}
+
+}  // namespace
+
+struct sc_ensemble_s {
+  sc_ensemble_params prm{};
+  sc::Arena arena;
+  std::vector<hipStream_t> streams;
+  std::vector<int> free_slots;
+  std::vector<std::unique_ptr<Request>> reqs;  // index = request id
+  std::deque<int64_t> queue;
+  std::vector<int64_t> running;
+  std::chrono::steady_clock::time_point t0;
+  std::string err;
+  int32_t peak_active = 0;
+  size_t peak_bytes = 0;
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  int fail(int code, const std::string &msg) {
+    err = msg;
+    return code;
+  }
+};
+
+namespace {
+
+void retire(sc_ensemble e, Request &R) {
+  if (R.h) sc_destroy(R.h);
+  R.h = nullptr;
+  if (R.slot >= 0) e->free_slots.push_back(R.slot);
+  R.slot = -1;
+  e->running.erase(std::remove(e->running.begin(), e->running.end(), R.id), e->running.end());
+  // inputs are no longer needed once the request has left the device
+  std::vector<double>().swap(R.r);
+}
+
+int enqueue_chunk(sc_ensemble e, Request &R) {
+  const int64_t k = std::min<int64_t>(e->prm.steps_per_launch, R.n_steps - R.enqueued);
+  if (k <= 0) return SC_OK;
+  const int rc = sc_step_async(R.h, (int32_t)k);
+  if (rc == SC_OK) R.enqueued += k;
+  return rc;
+}
+
+// Creates and loads R's system in slot `slot`; SC_ERR_NOMEM leaves R queued.
+int admit(sc_ensemble e, Request &R, int slot) {
+  sc_handle h = nullptr;
+  int rc = sc::system_create(&R.p, e->streams[slot], &e->arena, &h);
+  if (rc) return rc;
+  rc = sc_load_filaments(h, R.nf, R.site_off.data(), R.r.data(), R.fil.data());
+  if (rc) {
+    if (rc != SC_ERR_NOMEM) R.err = sc_last_error(h);
+    sc_destroy(h);
+    return rc;
+  }
+  R.h = h;
+  R.slot = slot;
+  (void)sc_device_bytes(h, &R.bytes);
+  R.status = SC_REQ_RUNNING;
+  R.t_admit = e->now();
+  return SC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_ensemble_create(const sc_ensemble_params *params, sc_ensemble *out) {
+  if (!out) return SC_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SC_ERR_HIP;
+  std::unique_ptr<sc_ensemble_s> e(new sc_ensemble_s());
+  if (params) e->prm = *params;
+  if (e->prm.max_active <= 0) e->prm.max_active = 16;
+  if (e->prm.steps_per_launch <= 0) e->prm.steps_per_launch = 64;
+  size_t budget = e->prm.hbm_budget;
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return SC_ERR_HIP;
+    budget = (size_t)(0.8 * (double)fr);
+  }
+  e->prm.hbm_budget = budget;
+  if (e->arena.init(budget) != SC_OK) return SC_ERR_NOMEM;
+  for (int k = 0; k < e->prm.max_active; ++k) {
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      for (hipStream_t s : e->streams) (void)hipStreamDestroy(s);
+      return SC_ERR_HIP;
+    }
+    e->streams.push_back(st);
+    e->free_slots.push_back(e->prm.max_active - 1 - k);
+  }
+  e->t0 = std::chrono::steady_clock::now();
+  *out = e.release();
+  return SC_OK;
+}
+
+int sc_ensemble_submit(sc_ensemble e, const sc_params *params, int32_t n_filaments,
+                       const int32_t *site_off, const double *r, const sc_filament_params *fil,
+                       int64_t n_steps, double t_submit, int64_t *request_id) {
+  if (!e) return SC_ERR_ARG;
+  if (!params || !site_off || !r || !fil || n_filaments <= 0 || n_steps < 0)
+    return e->fail(SC_ERR_ARG, "bad request");
+  std::unique_ptr<Request> R(new Request());
+  R->id = (int64_t)e->reqs.size();
+  R->p = *params;
+  R->nf = n_filaments;
+  R->site_off.assign(site_off, site_off + n_filaments + 1);
+  R->ns = R->site_off.back();
+  if (R->site_off[0] != 0 || R->ns <= 0) return e->fail(SC_ERR_ARG, "bad site offsets");
+  R->r.assign(r, r + 3 * R->ns);
+  R->fil.assign(fil, fil + n_filaments);
+  R->n_bins = count_bins(R->site_off);
+  R->n_steps = n_steps;
+  R->t_submit = t_submit < 0 ? e->now() : t_submit;
+  R->est_bytes = sc::system_estimate_bytes(&R->p, R->nf, R->ns, R->n_bins);
+  if (request_id) *request_id = R->id;
+  if (R->est_bytes > e->arena.capacity()) {
+    R->status = SC_ERR_NOMEM;
+    R->err = "request footprint exceeds the runner's device budget";
+    R->t_done = R->t_submit;
+  } else {
+    e->queue.push_back(R->id);
+  }
+  e->reqs.push_back(std::move(R));
+  return SC_OK;
+}
+
+int sc_ensemble_progress(sc_ensemble e, int32_t *n_running, int32_t *n_queued) {
+  if (!e) return SC_ERR_ARG;
+  // 1. running systems: finished chunks get the next one, finished
+  //    requests are retired
+  const std::vector<int64_t> run = e->running;
+  for (int64_t id : run) {
+    Request &R = *e->reqs[id];
+    int done = 0;
+    int rc = sc_step_poll(R.h, &done);
+    if (rc == SC_OK && done && R.enqueued < R.n_steps) rc = enqueue_chunk(e, R);
+    else if (rc == SC_OK && done) {
+      R.r_out.resize(3 * (size_t)R.ns);
+      rc = sc_get_sites(R.h, R.r_out.data(), nullptr, nullptr);
+      if (rc == SC_OK) {
+        R.status = SC_REQ_DONE;
+        R.t_done = e->now();
+        retire(e, R);
+        continue;
+      }
+    }
+    if (rc != SC_OK) {
+      R.status = rc;
+      R.err = sc_last_error(R.h);
+      R.t_done = e->now();
+      retire(e, R);
+    }
+  }
+  // 2. admission, first fit in arrival order
+  for (auto it = e->queue.begin(); it != e->queue.end() && !e->free_slots.empty();) {
+    Request &R = *e->reqs[*it];
+    if (R.est_bytes > e->arena.largest_free()) {
+      ++it;
+      continue;
+    }
+    const int slot = e->free_slots.back();
+    const int rc = admit(e, R, slot);
+    if (rc == SC_ERR_NOMEM) {
+      ++it;
+      continue;
+    }
+    e->free_slots.pop_back();
+    it = e->queue.erase(it);
+    if (rc != SC_OK) {
+      R.status = rc;
+      if (R.err.empty()) R.err = "admission failed";
+      R.t_done = e->now();
+      if (R.h) sc_destroy(R.h);
+      R.h = nullptr;
+      e->free_slots.push_back(slot);
+      continue;
+    }
+    e->running.push_back(R.id);
+    const int rc2 = enqueue_chunk(e, R);
+    if (rc2 != SC_OK) {
+      R.status = rc2;
+      R.err = sc_last_error(R.h);
+      R.t_done = e->now();
+      retire(e, R);
+    }
+    e->peak_active = std::max<int32_t>(e->peak_active, (int32_t)e->running.size());
+    e->peak_bytes = std::max(e->peak_bytes, e->arena.used());
+  }
+  if (n_running) *n_running = (int32_t)e->running.size();
+  if (n_queued) *n_queued = (int32_t)e->queue.size();
+  return SC_OK;
+}
+
+int sc_ensemble_drain(sc_ensemble e, double timeout_s) {
+  if (!e) return SC_ERR_ARG;
+  const double t_end = timeout_s > 0 ? e->now() + timeout_s : 0.0;
+  for (;;) {
+    int32_t nr = 0, nq = 0;
+    int rc = sc_ensemble_progress(e, &nr, &nq);
+    if (rc) return rc;
+    if (nr == 0 && nq == 0) return SC_OK;
+    if (nr == 0 && nq > 0) {
+      // nothing resident and the queue head does not fit an empty arena:
+      // only possible through fragmentation left by failed admissions
+      return e->fail(SC_ERR_NOMEM, "queued requests cannot be admitted");
+    }
+    if (timeout_s > 0 && e->now() > t_end) return e->fail(SC_ERR_STATE, "drain timed out");
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+int sc_ensemble_request(sc_ensemble e, int64_t id, sc_request_info *info) {
+  if (!e || !info) return SC_ERR_ARG;
+  if (id < 0 || id >= (int64_t)e->reqs.size()) return e->fail(SC_ERR_ARG, "unknown request");
+  const Request &R = *e->reqs[id];
+  info->status = R.status;
+  info->n_filaments = R.nf;
+  info->n_sites = R.ns;
+  info->n_bonds = R.ns - R.nf;
+  info->n_steps = R.n_steps;
+  // while running: steps enqueued so far (reading the device counters
+  // would wait for the handle's stream)
+  info->steps_done = R.status == SC_REQ_DONE ? R.n_steps
+                     : R.status == SC_REQ_RUNNING ? R.enqueued
+                                                  : 0;
+  info->t_submit = R.t_submit;
+  info->t_admit = R.t_admit;
+  info->t_done = R.t_done;
+  info->device_bytes = R.bytes;
+  return SC_OK;
+}
+
+int sc_ensemble_result(sc_ensemble e, int64_t id, double *r) {
+  if (!e || !r) return SC_ERR_ARG;
+  if (id < 0 || id >= (int64_t)e->reqs.size()) return e->fail(SC_ERR_ARG, "unknown request");
+  const Request &R = *e->reqs[id];
+  if (R.status < 0) return e->fail(R.status, "request failed: " + R.err);
+  if (R.status != SC_REQ_DONE) return e->fail(SC_ERR_STATE, "request not finished");
+  std::copy(R.r_out.begin(), R.r_out.end(), r);
+  return SC_OK;
+}
+
+int sc_ensemble_stats_get(sc_ensemble e, sc_ensemble_stats *out) {
+  if (!e || !out) return SC_ERR_ARG;
+  *out = sc_ensemble_stats{};
+  std::vector<double> lat;
+  double first = -1, last = -1;
+  for (const auto &p : e->reqs) {
+    const Request &R = *p;
+    ++out->submitted;
+    if (R.status == SC_REQ_DONE) {
+      ++out->completed;
+      lat.push_back(R.t_done - R.t_submit);
+      out->segment_steps += (double)(R.ns - R.nf) * (double)R.n_steps;
+      last = std::max(last, R.t_done);
+    } else if (R.status < 0) {
+      ++out->failed;
+    } else if (R.status == SC_REQ_RUNNING) {
+      ++out->running;
+    } else {
+      ++out->queued;
+    }
+    first = first < 0 ? R.t_submit : std::min(first, R.t_submit);
+  }
+  if (!lat.empty()) {
+    std::sort(lat.begin(), lat.end());
+    auto rank = [&](double q) {
+      // nearest-rank percentile
+      size_t k = (size_t)std::ceil(q * (double)lat.size());
+      if (k < 1) k = 1;
+      return lat[std::min(k, lat.size()) - 1];
+    };
+    out->latency_p50 = rank(0.50);
+    out->latency_p99 = rank(0.99);
+    double s = 0;
+    for (double v : lat) s += v;
+    out->latency_mean = s / (double)lat.size();
+    out->latency_max = lat.back();
+    out->span = last - first;
+  }
+  out->peak_active = e->peak_active;
+  out->peak_bytes = e->peak_bytes;
+  out->arena_bytes = e->arena.capacity();
+  return SC_OK;
+}
+
+double sc_ensemble_clock(sc_ensemble e) { return e ? e->now() : 0.0; }
+
+const char *sc_ensemble_last_error(sc_ensemble e) {
+  return e ? e->err.c_str() : "null ensemble";
+}
+
+int sc_ensemble_destroy(sc_ensemble e) {
+  if (!e) return SC_ERR_ARG;
+  for (auto &p : e->reqs)
+    if (p->h) {
+      sc_destroy(p->h);
+      p->h = nullptr;
+    }
+  for (hipStream_t s : e->streams) (void)hipStreamDestroy(s);
+  e->arena.release();
+  delete e;
+  return SC_OK;
+}
+
+}  // extern "C"

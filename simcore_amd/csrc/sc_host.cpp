@@ -522,7 +522,8 @@ int generate(double R, int nf, const int *n_sites, const double *length,
     const double bl = length[f] / (ns - 1);
     const double d = diameter[f];
     bool placed = false;
-    std::vector<double> pts(3 * (size_t)ns);
+    std::vector<double> pts(3 * (size_t)ns), own(6 * (size_t)ns);
+    const int self_skip = 2 + (int)std::ceil(d / bl);
     int64_t attempts_f = 0;
     for (int attempt = 0; attempt < 1000000 && !placed; ++attempt) {
       ++attempts_f;
@@ -549,6 +550,17 @@ int generate(double R, int nf, const int *n_sites, const double *length,
         for (int k = 0; k < 3; ++k) {
           c[k] = pts[3 * (i - 1) + k] + 0.5 * uu[k];
           uu[k] /= l;
+        }
+        // the filament's own earlier bonds more than a diameter of contour
+        // away (a long filament in a small periodic box can cross its own
+        // images)
+        for (int j = 0; j + self_skip < i && ok; ++j) {
+          const double *cj = &own[6 * j];
+          if (seg_dist2(c, uu, 0.5 * l, cj, cj + 3, 0.5 * l, L) < d * d) ok = false;
+        }
+        for (int k = 0; k < 3; ++k) {
+          own[6 * (i - 1) + k] = c[k];
+          own[6 * (i - 1) + 3 + k] = uu[k];
         }
         int cx = G.idx(c[0]), cy = G.idx(c[1]), cz = G.idx(c[2]);
         for (int dx = -1; dx <= 1 && ok; ++dx)

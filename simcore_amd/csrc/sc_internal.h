@@ -93,6 +93,11 @@ void launch_pair(const Dev &d, const Geo &g, int precision, int grid,
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
                       int bin_count, int bin_maxn, int precision,
                       hipStream_t st);
+// register-resident fast-path integrator; false when the bin is not one it
+// serves (FP64 precision or more than 64 sites), nothing launched then
+bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,
+                          int bin_count, int bin_maxn, int precision,
+                          hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t st);

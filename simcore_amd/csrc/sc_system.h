@@ -1,0 +1,53 @@
+// sc_system.h -- pieces of the system handle shared between the single-system
+// ABI (sc_api.cpp) and the ensemble runner (sc_ensemble.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <unordered_map>
+
+#include "../../include/simcore_amd.h"
+
+namespace sc {
+
+// First-fit sub-allocator over one device allocation.  Systems admitted by
+// the ensemble runner take all their buffers from it, so admission and
+// retirement never call hipMalloc/hipFree (hipFree waits for the whole
+// device) and the runner's footprint is fixed at creation.
+class Arena {
+ public:
+  static constexpr size_t kAlign = 256;
+  static size_t round(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+
+  int init(size_t bytes);  // SC_OK / SC_ERR_NOMEM
+  void release();
+  ~Arena() { release(); }
+
+  void *get(size_t bytes);  // nullptr when no free block fits
+  void put(void *p);
+  size_t capacity() const { return cap_; }
+  size_t used() const { return used_bytes_; }
+  size_t largest_free() const;
+
+ private:
+  char *base_ = nullptr;
+  size_t cap_ = 0;
+  size_t used_bytes_ = 0;
+  std::map<size_t, size_t> free_;             // offset -> size, address order
+  std::unordered_map<size_t, size_t> taken_;  // offset -> size
+};
+
+// sc_create with a caller-owned stream (not destroyed with the handle) and
+// an optional arena for every device buffer.
+int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
+                  sc_handle *out);
+
+// Device bytes sc_load_filaments allocates for a system of nf filaments and
+// ns sites under these parameters (before any candidate-capacity growth).
+size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns,
+                             int n_bins);
+
+}  // namespace sc

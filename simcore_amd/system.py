@@ -156,6 +156,16 @@ class FilamentSystem:
     def step(self, n=1):
         _check(lib.sc_step(self.h, int(n)), self.h, "sc_step")
 
+    def step_async(self, n=1):
+        """Enqueue n steps and return at once."""
+        _check(lib.sc_step_async(self.h, int(n)), self.h, "sc_step_async")
+
+    def poll(self):
+        """True when every enqueued step has completed (never blocks)."""
+        done = ctypes.c_int()
+        _check(lib.sc_step_poll(self.h, byref(done)), self.h, "sc_step_poll")
+        return bool(done.value)
+
     # -- taps ----------------------------------------------------------------
     def sites(self):
         r = np.empty(3 * self.ns)

@@ -48,3 +48,30 @@ def culled(b1, b2, site_off):
     bf = bond_filament(site_off)
     ex = (np.abs(b1.astype(np.int64) - b2) == 1) & (bf[b1] == bf[b2])
     return b1[~ex], b2[~ex]
+
+
+# site counts straddling the integrator's length bins (<=8, 16, 32, 64,
+# 128, 256 sites) and their edges
+SHORT_SITES = [3, 4, 7, 8, 9, 15, 16, 17, 31, 32, 33, 47, 63, 64]
+LONG_SITES = [65, 100, 127, 128, 129, 200, 255, 256]
+
+
+def uneven_system(sites=SHORT_SITES, copies=3, seed=21, phi=0.04, radius=None,
+                  delta=1e-4, noise_seed=777):
+    """Filaments of the given site counts (each `copies` times) in one
+    periodic box, lp in [10, 100] sigma, bond length 1, d = 1; the box is
+    sized for volume fraction phi unless `radius` is given."""
+    n = np.array(list(sites) * copies, dtype=np.int32)
+    rng = np.random.default_rng(seed)
+    lp = rng.uniform(10.0, 100.0, len(n))
+    R = box_for_fraction(n, 1.0, 1.0, phi) if radius is None else radius
+    p = sc.params_init(R, delta=delta, min_cell_length=1.0, seed=noise_seed)
+    fil = sc.generate_filaments(p, n, (n - 1).astype(float), lp, 1.0, 2.0, seed=seed)
+    return p, fil
+
+
+def long_system(seed=23, noise_seed=778):
+    """Every length class up to 256 sites, two of each, in a box wider than
+    the longest filament, plus short ones."""
+    return uneven_system(sites=SHORT_SITES + LONG_SITES, copies=2, seed=seed,
+                         radius=140.0, noise_seed=noise_seed)

@@ -114,3 +114,14 @@ def test_create_without_device_fails_cleanly():
     h = ctypes.c_void_p()
     rc = sc.lib.sc_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == -2 and not h.value
+
+
+def test_ensemble_create_without_device_fails_cleanly():
+    import ctypes
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    from simcore_amd._lib import ScEnsembleParams
+    prm = ScEnsembleParams(1 << 20, 2, 8)
+    h = ctypes.c_void_p()
+    rc = sc.lib.sc_ensemble_create(ctypes.byref(prm), ctypes.byref(h))
+    assert rc == -2 and not h.value

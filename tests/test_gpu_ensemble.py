@@ -1,0 +1,96 @@
+"""Asynchronous stepping and the ensemble runner on the device: a system's
+result is the same stepped alone, asynchronously, or among others."""
+import time
+
+import numpy as np
+import pytest
+
+import simcore_amd as sc
+from simcore_amd._lib import SC_ERR_NOMEM
+from tests.helpers import c0_system, long_system, uneven_system
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems():
+    out = []
+    p, f = c0_system(num=32, seed=5, noise_seed=101)
+    out.append((p, f, 60))
+    p, f = uneven_system(seed=31, noise_seed=202)
+    out.append((p, f, 45))
+    p, f = c0_system(num=64, seed=6, noise_seed=303)
+    out.append((p, f, 33))
+    p, f = long_system(seed=41, noise_seed=404)
+    out.append((p, f, 50))
+    return out
+
+
+def _alone(p, f, n, precision="mixed"):
+    with sc.FilamentSystem(p, f, precision=precision, use_graph=True) as s:
+        s.step(n)
+        return s.sites()[0]
+
+
+def test_async_steps_match_sync():
+    print("case: C0 32 filaments, 15+25 async steps vs 40 sync", flush=True)
+    p, f = c0_system(num=32, seed=5)
+    ref = _alone(p, f, 40)
+    with sc.FilamentSystem(p, f, precision="mixed", use_graph=True) as s:
+        s.step_async(15)
+        s.step_async(25)
+        t0 = time.time()
+        while not s.poll():
+            assert time.time() - t0 < 60
+            time.sleep(1e-4)
+        np.testing.assert_array_equal(s.sites()[0], ref)
+        assert s.counters()["i_step"] == 40
+
+
+def test_ensemble_results_same_alone_and_in_mix():
+    print("case: 4 systems alone vs ensemble (max_active 4, 16 steps per launch)", flush=True)
+    cases = _systems()
+    refs = [_alone(p, f, n) for p, f, n in cases]
+    with sc.Ensemble(hbm_budget=2 << 30, max_active=4, steps_per_launch=16) as ens:
+        ids = [ens.submit(p, f, n, precision="mixed") for p, f, n in cases]
+        ens.drain(timeout_s=120)
+        for rid, ref in zip(ids, refs):
+            assert ens.request(rid)["state"] == "done"
+            np.testing.assert_array_equal(ens.result(rid), ref)
+        st = ens.stats()
+        print(st)
+        assert st["completed"] == len(cases) and st["failed"] == 0
+        assert st["peak_active"] >= 2
+        assert st["latency_p50"] <= st["latency_p99"] <= st["latency_max"]
+        assert st["peak_bytes"] <= st["arena_bytes"]
+
+
+def test_ensemble_admission_by_footprint():
+    print("case: budget for one resident system at a time", flush=True)
+    cases = _systems()[:3]
+    sizes = []
+    for p, f, n in cases:
+        with sc.FilamentSystem(p, f, precision="mixed") as s:
+            sizes.append(s.device_bytes())
+    budget = int(1.3 * max(sizes))
+    with sc.Ensemble(hbm_budget=budget, max_active=8, steps_per_launch=8) as ens:
+        ids = [ens.submit(p, f, n, precision="mixed") for p, f, n in cases]
+        ens.drain(timeout_s=120)
+        st = ens.stats()
+        print(st, sizes)
+        assert st["completed"] == 3
+        assert st["peak_bytes"] <= budget
+        assert st["peak_active"] == 1
+        for rid, (p, f, n) in zip(ids, cases):
+            np.testing.assert_array_equal(ens.result(rid), _alone(p, f, n))
+
+
+def test_ensemble_rejects_request_larger_than_budget():
+    print("case: request larger than the whole budget", flush=True)
+    p, f = c0_system(num=64)
+    with sc.Ensemble(hbm_budget=1 << 18, max_active=2) as ens:
+        rid = ens.submit(p, f, 10)
+        ens.drain(timeout_s=10)
+        info = ens.request(rid)
+        assert info["status"] == SC_ERR_NOMEM
+        with pytest.raises(sc.SimcoreError):
+            ens.result(rid)

@@ -165,3 +165,68 @@ def test_mixed_trajectory_within_tolerance():
     ob1, ob2 = ora.candidates()
     np.testing.assert_array_equal(db1, ob1)
     np.testing.assert_array_equal(db2, ob2)
+
+
+def _uneven(kind, precision, use_graph=False):
+    from tests.helpers import long_system, uneven_system
+    p, fil = uneven_system() if kind == "short" else long_system()
+    dev = sc.FilamentSystem(p, fil, precision=precision, use_graph=use_graph)
+    return p, fil, dev
+
+
+@pytest.mark.parametrize("kind", ["short", "long"])
+def test_uneven_lengths_fp64_bit_exact(kind):
+    """Every integrator length bin (3..256 sites) in one system: FP64 path
+    bit-identical to the CPU path over 60 steps incl. rebuilds."""
+    print(f"case: uneven {kind} fp64 60 steps", flush=True)
+    p, fil, dev = _uneven(kind, "fp64")
+    ora = OracleSystem(p, fil)
+    dev.step(60)
+    assert ora.step(60) == 0
+    for a, b in zip(dev.sites(), ora.sites()):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(dev.tensions(), ora.tensions())
+    assert dev.counters()["n_rebuilds"] == ora.counters()["n_rebuilds"]
+    db1, db2 = dev.candidates()
+    ob1, ob2 = ora.candidates()
+    np.testing.assert_array_equal(db1, ob1)
+    np.testing.assert_array_equal(db2, ob2)
+
+
+@pytest.mark.parametrize("kind", ["short", "long"])
+def test_uneven_lengths_mixed_trajectory(kind):
+    """Mixed path on every length bin against the FP64 CPU path: positions
+    within 1e-8 sigma after one step and 1e-5 sigma after 100 steps,
+    tensions within 1e-5 relative after one step."""
+    print(f"case: uneven {kind} mixed 1 and 100 steps", flush=True)
+    p, fil, dev = _uneven(kind, "mixed")
+    ora = OracleSystem(p, fil)
+    dev.step(1)
+    assert ora.step(1) == 0
+    d1 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 1 step", d1)
+    assert d1 <= 1e-8
+    np.testing.assert_allclose(dev.tensions(), ora.tensions(), rtol=1e-5,
+                               atol=1e-5 * np.abs(ora.tensions()).max())
+    dev.step(1)
+    assert ora.step(1) == 0
+    d2 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 2 steps", d2)
+    assert d2 <= 1e-8
+    dev.step(98)
+    assert ora.step(98) == 0
+    d100 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 100 steps", d100)
+    assert d100 <= 1e-5
+    assert dev.counters()["n_rebuilds"] == ora.counters()["n_rebuilds"]
+
+
+def test_uneven_lengths_mixed_graph_matches_eager():
+    print("case: uneven short mixed graph vs eager 31 steps", flush=True)
+    p, fil, eager = _uneven("short", "mixed")
+    _, _, graph = _uneven("short", "mixed", use_graph=True)
+    eager.step(31)
+    graph.step(31)
+    for a, b in zip(eager.sites(), graph.sites()):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(eager.tensions(), graph.tensions())
