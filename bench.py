@@ -82,6 +82,150 @@ def workload(name, sc):
     return p, fil, desc
 
 
+def serving_requests(sc, n_requests, rate, seed=20240004):
+    """Synthetic serving trace: Poisson arrivals at `rate`/s of independent
+    systems with 1k-100k bonds (log-uniform) of 32-site filaments at volume
+    fraction 0.05, 2000 steps each, persistence length and noise seed drawn
+    per request (lp in [10, 1000] sigma, hence dt 1e-5 as for the
+    uneven-length system)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    out = []
+    for k in range(n_requests):
+        nb = float(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+        nf = max(1, int(round(nb / 31)))
+        lp = float(rng.uniform(10.0, 1000.0))
+        R = box_for_fraction([32] * nf, 1.0, 1.0, 0.05)
+        p = sc.params_init(R, delta=1e-5, min_cell_length=1.0,
+                           seed=int(rng.integers(1, 2 ** 62)))
+        fil = sc.generate_filaments(p, [32] * nf, 31.0, lp, 1.0, 2.0,
+                                    seed=int(rng.integers(1, 2 ** 31)))
+        out.append((float(t[k]), p, fil))
+    return out
+
+
+def run_serving(args, sc):
+    log(f"[bench] generating {args.requests} requests")
+    reqs = serving_requests(sc, args.requests, args.rate)
+    nbonds = np.array([f.nb for _, _, f in reqs])
+    log(f"[bench] bonds per request: mean {nbonds.mean():.0f}, min {nbonds.min()}, "
+        f"max {nbonds.max()}")
+    ens = sc.Ensemble(hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
+                      max_active=args.max_active, steps_per_launch=args.steps_per_launch)
+    # warm the code paths (graph capture, first launches) outside the trace
+    w = ens.submit(reqs[0][1], reqs[0][2], 20, precision=args.precision)
+    ens.drain(120)
+    ens.close()
+    ens = sc.Ensemble(hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
+                      max_active=args.max_active, steps_per_launch=args.steps_per_launch)
+    t0 = ens.clock()
+    k = 0
+    last_log = time.perf_counter()
+    while True:
+        now = ens.clock() - t0
+        while k < len(reqs) and reqs[k][0] <= now:
+            ens.submit(reqs[k][1], reqs[k][2], args.request_steps, precision=args.precision,
+                       t_submit=t0 + reqs[k][0])
+            k += 1
+        nr, nq = ens.progress()
+        if k == len(reqs) and nr == 0 and nq == 0:
+            break
+        if time.perf_counter() - last_log > 5.0:
+            last_log = time.perf_counter()
+            log(f"[bench] t={now:.1f}s submitted {k} running {nr} queued {nq}")
+        time.sleep(50e-6)
+    st = ens.stats()
+    ens.close()
+    del w
+    value = st["segment_steps"] / st["span"] if st["span"] > 0 else 0.0
+    offered = float(nbonds.mean()) * args.request_steps * args.rate
+    out = {
+        "metric": "segment-steps/s", "value": value, "unit": "segment-steps/s",
+        "n_gpus": 1, "steps": args.request_steps, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "ensemble serving: Poisson arrivals at %g/s of systems with "
+                               "1k-100k bonds (log-uniform, 32-site filaments, volume fraction "
+                               "0.05), %d steps each, lp U[10,1000] sigma, dt 1e-5, per-request "
+                               "seeds" % (args.rate, args.request_steps),
+                   "requests": args.requests, "max_active": args.max_active,
+                   "steps_per_launch": args.steps_per_launch,
+                   "hbm_budget_gb": args.hbm_budget_gb, "precision": args.precision},
+        "latency_p50_s": st["latency_p50"], "latency_p99_s": st["latency_p99"],
+        "latency_mean_s": st["latency_mean"], "latency_max_s": st["latency_max"],
+        "completed": st["completed"], "failed": st["failed"], "span_s": st["span"],
+        "offered_segment_steps_per_s": offered, "peak_active": st["peak_active"],
+        "peak_bytes": st["peak_bytes"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def copy_probe(nbytes, repeats=20):
+    import ctypes
+    import simcore_amd as sc
+    ms = ctypes.c_double()
+    rc = sc.lib.sc_copy_probe(int(nbytes), int(repeats), ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"sc_copy_probe failed: {rc}")
+    out = {"bytes": int(nbytes), "avg_ms": ms.value,
+           "GBps_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+    print(json.dumps(out), flush=True)
+
+
+def pmc_summary(root, tag, copy_bytes):
+    """Per-kernel FETCH_SIZE / WRITE_SIZE averages from rocprofv3 --pmc
+    runs under root/{FETCH_SIZE,WRITE_SIZE} and of the copy probe under
+    root/copy_{FETCH_SIZE,WRITE_SIZE}; each counter is scaled by
+    (copy bytes / what it reads on the copy) and written to
+    profiles/traffic_<tag>.json."""
+    import glob
+    import sqlite3
+
+    def per_kernel(d, counter):
+        dbs = glob.glob(os.path.join(root, d, "**", "*.db"), recursive=True)
+        acc = {}
+        for db in dbs:
+            con = sqlite3.connect(db)
+            q = ("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                 "where counter_name = ? group by kernel_name, dispatch_id")
+            for name, _, v in con.execute(q, (counter,)):
+                acc.setdefault(name.split("(")[0].replace("void ", ""), []).append(v)
+        return {k: (float(np.mean(v)), len(v)) for k, v in acc.items()}
+
+    out = {"tag": tag, "copy_bytes": copy_bytes, "unit": "bytes per launch", "kernels": {}}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cp = per_kernel("copy_" + counter, counter)
+        copy_kib = max(v[0] for k, v in cp.items() if "copy_probe" in k)
+        scale = copy_bytes / (copy_kib * 1024.0)
+        out[counter + "_scale"] = scale
+        for k, (v, n) in per_kernel(counter, counter).items():
+            e = out["kernels"].setdefault(k, {})
+            e[counter + "_KiB_raw"] = v
+            e[counter.lower() + "_bytes"] = v * 1024.0 * scale
+            e["launches"] = n
+    for k, e in out["kernels"].items():
+        if "fetch_size_bytes" in e and "write_size_bytes" in e:
+            e["traffic_bytes"] = e["fetch_size_bytes"] + e["write_size_bytes"]
+    path = os.path.join(ROOT, "profiles", f"traffic_{tag}.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+    print(json.dumps({"written": path, "fetch_scale": out["FETCH_SIZE_scale"],
+                      "write_scale": out["WRITE_SIZE_scale"]}), flush=True)
+
+
+def measured_traffic(tag, kernel):
+    path = os.path.join(ROOT, "profiles", f"traffic_{tag}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    for k, e in d.get("kernels", {}).items():
+        if k.endswith(kernel) and "traffic_bytes" in e:
+            return e["traffic_bytes"]
+    return None
+
+
 def pair_bytes(precision, nb, nc):
     """Algorithmic HBM bytes of one launch of the fused pair/scatter kernel:
     per bond its 64-byte state record read, its force and torque (6 FP64)
@@ -104,20 +248,42 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--fp64-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-baseline", action="store_true")
+    # serving workload (--workload c4)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--request-steps", type=int, default=2000)
+    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--steps-per-launch", type=int, default=64)
+    ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
+    # measurement helpers
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the streaming-copy probe of this many bytes")
+    ap.add_argument("--pmc-summary", default="",
+                    help="only summarise rocprofv3 --pmc runs under this directory")
+    ap.add_argument("--pmc-tag", default="dense620k_mixed")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(systems shard by ensemble member; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary, args.pmc_tag, args.copy_probe or (256 << 20))
+        return
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import simcore_amd as sc
+    if args.workload == "c4":
+        run_serving(args, sc)
+        return
 
     log(f"[bench] building workload {args.workload}")
     p, fil, desc = workload(args.workload, sc)
@@ -142,10 +308,14 @@ def main():
     bytes_pair = pair_bytes(args.precision, fil.nb, nc)
     achieved = bytes_pair / (pair_ms * 1e-3) / 1e9 if pair_ms > 0 else 0.0
     log(f"[bench] profile: {json.dumps(prof)} nc={nc}")
+    kname = "k_pair_fused<true>" if args.precision == "mixed" else "k_pair_fused<false>"
+    traffic = None
+    if args.workload == "c1":
+        traffic = measured_traffic(f"dense620k_{args.precision}", kname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_pair_fused<true>" if args.precision == "mixed"
-                          else "k_pair_fused<false>",
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic,
+                "kernel": kname,
                 "kernel_avg_ms": pair_ms, "algorithmic_bytes": bytes_pair,
                 "candidates": nc, "per_step_ms": {k: v[0] for k, v in prof.items()}}
     dev.close()

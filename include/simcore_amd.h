@@ -142,6 +142,19 @@ int sc_get_counters(sc_handle h, int64_t *i_step, int64_t *n_rebuilds,
  * unphysical-dr error path).  Bond state is re-initialised from them. */
 int sc_set_sites(sc_handle h, const double *r);
 
+/* On-device analyses of the current state.  sc_filament_stats: per filament
+ * the end-to-end vector head - tail (FilamentSpecies::RunMse2eAnalysis,
+ * src/filament_species.cpp:569-593) and the centre of mass, 3*nf doubles
+ * each (SoA); either pointer may be NULL.  sc_theta_histogram: cos(theta)
+ * of consecutive bonds binned as in RunThetaAnalysis (:595-610),
+ * bin = floor((1 + cos) * (n_bins / 2)) clamped to [0, n_bins-1], one row
+ * of n_bins per angle index i (bonds i, i+1 of a filament), rows
+ * [0, n_rows); counts are ADDED to hist (n_rows * n_bins int64), so calls
+ * accumulate samples. */
+int sc_filament_stats(sc_handle h, double *e2e, double *com);
+int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows,
+                       int64_t *hist);
+
 /* Device bytes this handle holds. */
 int sc_device_bytes(sc_handle h, size_t *bytes);
 /* Average per-launch duration (ms) of each kernel family recorded with HIP

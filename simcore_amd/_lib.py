@@ -30,6 +30,7 @@ EXPORTS = [
     "sc_profile_enable", "sc_profile_read", "sc_last_error", "sc_destroy",
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
     "sc_generate_filaments", "sc_copy_probe", "sc_step_async", "sc_step_poll",
+    "sc_filament_stats", "sc_theta_histogram",
     "sc_ensemble_create", "sc_ensemble_submit", "sc_ensemble_progress",
     "sc_ensemble_drain", "sc_ensemble_request", "sc_ensemble_result",
     "sc_ensemble_stats_get", "sc_ensemble_clock", "sc_ensemble_last_error",
@@ -120,6 +121,8 @@ def _load():
         "sc_get_dr_state": ([H, dp], c_int),
         "sc_get_counters": ([H, lp, lp, lp, ip], c_int),
         "sc_set_sites": ([H, dp], c_int),
+        "sc_filament_stats": ([H, dp, dp], c_int),
+        "sc_theta_histogram": ([H, c_int32, c_int32, lp], c_int),
         "sc_device_bytes": ([H, POINTER(c_size_t)], c_int),
         "sc_profile_enable": ([H, c_int], c_int),
         "sc_profile_read": ([H, dp, lp, c_int], c_int),

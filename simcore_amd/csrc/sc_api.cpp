@@ -753,6 +753,47 @@ int sc_set_sites(sc_handle h, const double *r) {
   return SC_OK;
 }
 
+int sc_filament_stats(sc_handle h, double *e2e, double *com) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!e2e && !com) return SC_OK;
+  const int64_t n = 3 * (int64_t)h->g.nf;
+  double *buf = nullptr;
+  HIPCHK(h, hipMalloc(&buf, sizeof(double) * 2 * n));
+  sc::launch_filament_stats(h->d, h->g, buf, buf + n, h->stream);
+  hipError_t e = hipMemcpyAsync(e2e ? e2e : com, e2e ? buf : buf + n, sizeof(double) * n,
+                                hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && e2e && com)
+    e = hipMemcpyAsync(com, buf + n, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(h, SC_ERR_HIP, std::string("sc_filament_stats: ") +
+                                                      hipGetErrorString(e));
+  return SC_OK;
+}
+
+int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows, int64_t *hist) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (n_bins <= 0 || n_rows <= 0 || !hist) return fail(h, SC_ERR_ARG, "bad histogram shape");
+  const size_t n = (size_t)n_bins * (size_t)n_rows;
+  unsigned long long *buf = nullptr;
+  HIPCHK(h, hipMalloc(&buf, sizeof(unsigned long long) * n));
+  std::vector<unsigned long long> host(n);
+  hipError_t e = hipMemsetAsync(buf, 0, sizeof(unsigned long long) * n, h->stream);
+  if (e == hipSuccess) {
+    sc::launch_theta_hist(h->d, h->g, n_bins, n_rows, buf, h->stream);
+    e = hipMemcpyAsync(host.data(), buf, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost,
+                       h->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(h, SC_ERR_HIP, std::string("sc_theta_histogram: ") +
+                                                      hipGetErrorString(e));
+  for (size_t k = 0; k < n; ++k) hist[k] += (int64_t)host[k];
+  return SC_OK;
+}
+
 int sc_device_bytes(sc_handle h, size_t *bytes) {
   if (!h || !bytes) return SC_ERR_ARG;
   *bytes = h->fixed_bytes + h->cand_bytes;

@@ -100,6 +100,11 @@ bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,
                           hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
+// analyses (sc_analysis.hip)
+void launch_filament_stats(const Dev &d, const Geo &g, double *e2e, double *com,
+                           hipStream_t st);
+void launch_theta_hist(const Dev &d, const Geo &g, int n_bins, int n_rows,
+                       unsigned long long *hist, hipStream_t st);
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t st);
 void launch_redo_dr(const Dev &d, const Geo &g, hipStream_t st);
 void launch_copy_probe(const void *src, void *dst, int64_t n32, hipStream_t st);

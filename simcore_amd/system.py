@@ -226,6 +226,23 @@ class FilamentSystem:
         return {"i_step": i_step.value, "n_rebuilds": nreb.value,
                 "n_candidates": nc.value, "n_cells_1d": n1d.value}
 
+    def filament_stats(self):
+        """(end-to-end vectors, centres of mass), each (3, nf)."""
+        e2e = np.empty(3 * self.nf)
+        com = np.empty(3 * self.nf)
+        _check(lib.sc_filament_stats(self.h, _dp(e2e), _dp(com)), self.h, "sc_filament_stats")
+        return e2e.reshape(3, -1), com.reshape(3, -1)
+
+    def theta_histogram(self, n_bins, n_rows, hist=None):
+        """cos(theta) histogram (n_rows, n_bins), accumulated into `hist`."""
+        if hist is None:
+            hist = np.zeros((n_rows, n_bins), dtype=np.int64)
+        assert hist.shape == (n_rows, n_bins) and hist.dtype == np.int64
+        _check(lib.sc_theta_histogram(self.h, int(n_bins), int(n_rows),
+                                      hist.ctypes.data_as(ctypes.POINTER(c_int64))),
+               self.h, "sc_theta_histogram")
+        return hist
+
     def set_sites(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         _check(lib.sc_set_sites(self.h, _dp(r)), self.h, "sc_set_sites")
