@@ -125,7 +125,8 @@ def run_serving(args, sc):
         now = ens.clock() - t0
         while k < len(reqs) and reqs[k][0] <= now:
             ens.submit(reqs[k][1], reqs[k][2], args.request_steps, precision=args.precision,
-                       t_submit=t0 + reqs[k][0])
+                       t_submit=t0 + reqs[k][0],
+                       spatial_order=args.precision == "mixed" and not args.caller_order)
             k += 1
         nr, nq = ens.progress()
         if k == len(reqs) and nr == 0 and nq == 0:
@@ -255,6 +256,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-baseline", action="store_true")
+    ap.add_argument("--caller-order", action="store_true",
+                    help="keep filaments in generation order (no spatial storage order)")
     # serving workload (--workload c4)
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=20.0)
@@ -288,7 +291,9 @@ def main():
     log(f"[bench] building workload {args.workload}")
     p, fil, desc = workload(args.workload, sc)
     log(f"[bench] {fil.nf} filaments, {fil.nb} bonds")
-    dev = sc.FilamentSystem(p, fil, precision=args.precision, use_graph=True)
+    spatial = args.precision == "mixed" and not args.caller_order
+    dev = sc.FilamentSystem(p, fil, precision=args.precision, use_graph=True,
+                            spatial_order=spatial)
     dev.step(args.warmup)
     log(f"[bench] warmup {args.warmup} steps done, counters {dev.counters()}")
     c0 = dev.counters()
@@ -365,7 +370,7 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if args.precision == "mixed" else "f64",
         "data": "synthetic",
-        "config": dict(desc, precision=args.precision, graph=True),
+        "config": dict(desc, precision=args.precision, graph=True, spatial_order=spatial),
         "roofline": roofline, "cpu_baseline": cpu,
     }
     out.update(extra)

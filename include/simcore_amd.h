@@ -60,7 +60,13 @@ typedef struct sc_params {
   int stoch_flag;          /* params.stoch_flag                            */
   int precision;           /* SC_PREC_FP64 or SC_PREC_MIXED                */
   int use_graph;           /* capture the step in a hipGraph               */
-  int reserved;
+  int spatial_order;       /* 1: store filaments in space-filling-curve order
+                              of their centres (load time) so that pair
+                              partners sit close in memory; getters still
+                              use the caller's order and the noise stream is
+                              keyed by the caller's site indices, but the
+                              candidate list ORDER (hence FP64 summation
+                              order) differs from the reference's          */
   uint64_t seed;           /* thermal-noise stream key                     */
   double capacity_factor;  /* initial candidate capacity / bonds (0: 4)    */
 } sc_params;
@@ -154,6 +160,20 @@ int sc_set_sites(sc_handle h, const double *r);
 int sc_filament_stats(sc_handle h, double *e2e, double *com);
 int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows,
                        int64_t *hist);
+
+/* Checkpoint / resume.  The reference's checkpoint (Species::WriteCheckpoints,
+ * include/simcore/species.hpp:238-258; Mesh::WriteCheckpoint,
+ * src/mesh.cpp:506-514) stores object specs plus GSL RNG states; the noise
+ * here is a counter-based stream keyed by the step index, so the complete
+ * dynamic state is: sites, previous positions, random forces, displacement
+ * bookkeeping, bond state and forces, tensions, normalisation switches,
+ * the candidate list with its sticky forces and incidence lists, and the
+ * step/rebuild counters.  sc_checkpoint_save with buf == NULL reports the
+ * size.  sc_checkpoint_load restores into a handle created with the same
+ * parameters and loaded with the same filament layout (checked); stepping
+ * then continues bit-identically to the saved handle. */
+int sc_checkpoint_save(sc_handle h, void *buf, size_t cap, size_t *size);
+int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
 
 /* Device bytes this handle holds. */
 int sc_device_bytes(sc_handle h, size_t *bytes);
@@ -272,6 +292,24 @@ int sc_generate_filaments(const sc_params *params, int32_t n_filaments,
                           const double *diameter, double friction_ratio,
                           uint64_t seed, int32_t *site_off, double *r,
                           sc_filament_params *fil, int64_t *n_sites_total);
+
+/* ---- trajectory files ------------------------------------------------------ */
+/* The reference's binary .posit / .spec files for a filament species, from
+ * caller arrays (host only; pair with sc_get_sites).  File names follow
+ * SpeciesBase::InitPositFile: <run>_filament_<species>.posit / .spec.
+ * sc_output_header creates the file with its header (int n_steps, int
+ * n_posit or n_spec, double delta; src/species_base.cpp:13-27, 55-70);
+ * sc_spec_frame / sc_posit_frame append one frame in the layout of
+ * Species::WriteSpecs / WritePosits (include/simcore/species.hpp:221-236),
+ * Filament::WriteSpec (src/mesh.cpp:482-492, src/filament.cpp:1303-1318)
+ * and Filament::WritePosit (src/filament.cpp:1338-1352). */
+int sc_output_header(const char *path, int32_t n_steps, int32_t n_every,
+                     double delta);
+int sc_spec_frame(const char *path, int32_t n_filaments, const int32_t *site_off,
+                  const double *r, const sc_filament_params *fil);
+int sc_posit_frame(const char *path, const sc_params *params,
+                   int32_t n_filaments, const int32_t *site_off,
+                   const double *r, const sc_filament_params *fil);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,8 @@ EXPORTS = [
     "sc_profile_enable", "sc_profile_read", "sc_last_error", "sc_destroy",
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
     "sc_generate_filaments", "sc_copy_probe", "sc_step_async", "sc_step_poll",
-    "sc_filament_stats", "sc_theta_histogram",
+    "sc_filament_stats", "sc_theta_histogram", "sc_output_header", "sc_spec_frame",
+    "sc_posit_frame", "sc_checkpoint_save", "sc_checkpoint_load",
     "sc_ensemble_create", "sc_ensemble_submit", "sc_ensemble_progress",
     "sc_ensemble_drain", "sc_ensemble_request", "sc_ensemble_result",
     "sc_ensemble_stats_get", "sc_ensemble_clock", "sc_ensemble_last_error",
@@ -49,7 +50,7 @@ class ScParams(ctypes.Structure):
         ("system_radius", c_double), ("min_cell_length", c_double),
         ("delta", c_double), ("wca_eps", c_double), ("wca_sig", c_double),
         ("f_cutoff", c_double), ("stoch_flag", c_int), ("precision", c_int),
-        ("use_graph", c_int), ("reserved", c_int), ("seed", c_uint64),
+        ("use_graph", c_int), ("spatial_order", c_int), ("seed", c_uint64),
         ("capacity_factor", c_double),
     ]
 
@@ -122,6 +123,8 @@ def _load():
         "sc_get_counters": ([H, lp, lp, lp, ip], c_int),
         "sc_set_sites": ([H, dp], c_int),
         "sc_filament_stats": ([H, dp, dp], c_int),
+        "sc_checkpoint_save": ([H, c_void_p, c_size_t, POINTER(c_size_t)], c_int),
+        "sc_checkpoint_load": ([H, c_void_p, c_size_t], c_int),
         "sc_theta_histogram": ([H, c_int32, c_int32, lp], c_int),
         "sc_device_bytes": ([H, POINTER(c_size_t)], c_int),
         "sc_profile_enable": ([H, c_int], c_int),
@@ -142,6 +145,10 @@ def _load():
         "sc_ensemble_destroy": ([H], c_int),
         "sc_params_from_yaml": ([c_char_p, POINTER(ScParams), dp], c_int),
         "sc_params_init": ([POINTER(ScParams), c_double], c_int),
+        "sc_output_header": ([c_char_p, c_int32, c_int32, c_double], c_int),
+        "sc_spec_frame": ([c_char_p, c_int32, ip, dp, POINTER(ScFilamentParams)], c_int),
+        "sc_posit_frame": ([c_char_p, POINTER(ScParams), c_int32, ip, dp,
+                            POINTER(ScFilamentParams)], c_int),
         "sc_filament_layout": ([c_double, c_double, c_double, c_double, ip, dp], c_int),
         "sc_generate_filaments": ([POINTER(ScParams), c_int32, ip, dp, dp, dp, c_double,
                                    c_uint64, ip, dp, POINTER(ScFilamentParams), lp], c_int),

@@ -35,7 +35,12 @@ struct sc_system {
   hipStream_t stream = nullptr;
   std::string err;
   bool loaded = false;
-  std::vector<int32_t> site_off;
+  std::vector<int32_t> site_off;  // storage order
+  // spatial order: storage filament f holds the caller's filament perm[f];
+  // site/bond maps send storage indices to the caller's
+  bool reordered = false;
+  std::vector<int32_t> perm;
+  std::vector<int64_t> site_map, bond_map;
   std::vector<Bin> bins;
   int pair_grid = 1024;
   // graphs: [0] odd step (rebuild stage included), [1] even step (a rebuild
@@ -255,6 +260,53 @@ void get3(double *dst, const double *src_dev, int64_t n) {
   (void)hipMemcpy(dst, src_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToHost);
 }
 
+// storage-order array of `comp` components x n entries -> caller order
+template <typename T>
+void to_caller(const sc_system *h, const std::vector<int64_t> &map, const T *src, T *dst,
+               int64_t n, int comp) {
+  if (!h->reordered) {
+    std::copy(src, src + comp * n, dst);
+    return;
+  }
+  for (int k = 0; k < comp; ++k)
+    for (int64_t i = 0; i < n; ++i) dst[k * n + map[i]] = src[k * n + i];
+}
+
+// device SoA site/bond array -> caller-order host array
+int fetch_mapped(sc_system *h, const double *dev, double *out, int64_t n, int comp,
+                 bool sites) {
+  std::vector<double> tmp((size_t)comp * n);
+  HIPCHK(h, hipMemcpy(tmp.data(), dev, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+  to_caller(h, sites ? h->site_map : h->bond_map, tmp.data(), out, n, comp);
+  return SC_OK;
+}
+
+// Space-filling-curve order of the filaments' centres: 10-bit Morton code
+// of the wrapped scaled centre per dimension, stable in caller order.
+std::vector<int32_t> spatial_permutation(const sc_params &p, int32_t nf, const int32_t *so,
+                                         const double *r) {
+  const int64_t ns = so[nf];
+  std::vector<uint32_t> key(nf);
+  for (int f = 0; f < nf; ++f) {
+    double c[3] = {0, 0, 0};
+    for (int64_t i = so[f]; i < so[f + 1]; ++i)
+      for (int k = 0; k < 3; ++k) c[k] += r[k * ns + i];
+    uint32_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      double sk = 0;
+      for (int j = 0; j < 3; ++j) sk += p.unit_cell_inv[3 * k + j] * c[j] / (so[f + 1] - so[f]);
+      sk -= std::floor(sk + 0.5);
+      uint32_t q = (uint32_t)std::min(1023.0, std::max(0.0, std::floor((sk + 0.5) * 1024.0)));
+      for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + k);
+    }
+    key[f] = code;
+  }
+  std::vector<int32_t> perm(nf);
+  for (int f = 0; f < nf; ++f) perm[f] = f;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  return perm;
+}
+
 // Enqueues `count` steps after step index `from` (the next step is
 // from+1), then the Ctl mirror copy and the completion event.
 int enqueue_range(sc_system *h, int64_t from, int64_t count) {
@@ -375,8 +427,8 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   const int64_t n_cells = n1d * n1d * n1d;
   const int64_t nb = ns - nf;
   auto R = [](int64_t n) { return Arena::round((size_t)(n > 0 ? n : 1)); };
-  size_t b = R(sizeof(Ctl)) + R(4 * (nf + 1)) + R(64 * (int64_t)nf) + R(8 * (int64_t)nf) +
-             R(4 * nb);
+  size_t b = R(sizeof(Ctl)) + R(4 * (nf + 1)) + R(4 * (int64_t)nf) + R(64 * (int64_t)nf) +
+             R(8 * (int64_t)nf) + R(4 * nb);
   b += 4 * R(24 * ns) + 2 * R(8 * ns);                    // r, rp, fr, dr0, acc, acc_prev
   b += R(64 * nb) + 2 * R(24 * nb) + R(8 * nb);           // record, F, T, tension
   b += R(4 * nb) + R(4 * n_cells) + R(4 * (n_cells + 1));  // keys, cell count/start
@@ -414,10 +466,45 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   g.nf = nf;
   g.ns = site_off[nf];
   g.nb = g.ns - nf;
+  // storage order (caller order unless spatial_order is set)
+  std::vector<int32_t> so_st, noise_off(nf);
+  std::vector<double> r_st;
+  std::vector<sc_filament_params> fil_st;
+  if (h->p.spatial_order) {
+    const int64_t ns_ = g.ns, nb_ = g.nb;
+    h->perm = spatial_permutation(h->p, nf, site_off, r);
+    h->reordered = true;
+    so_st.assign(nf + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+      const int u = h->perm[f];
+      so_st[f + 1] = so_st[f] + (site_off[u + 1] - site_off[u]);
+    }
+    r_st.resize(3 * (size_t)ns_);
+    fil_st.resize(nf);
+    h->site_map.resize(ns_);
+    h->bond_map.resize(nb_);
+    for (int f = 0; f < nf; ++f) {
+      const int u = h->perm[f];
+      fil_st[f] = fil[u];
+      noise_off[f] = site_off[u];
+      const int n = so_st[f + 1] - so_st[f];
+      for (int i = 0; i < n; ++i) {
+        h->site_map[so_st[f] + i] = site_off[u] + i;
+        for (int k = 0; k < 3; ++k) r_st[k * ns_ + so_st[f] + i] = r[k * ns_ + site_off[u] + i];
+        if (i < n - 1) h->bond_map[so_st[f] - f + i] = site_off[u] - u + i;
+      }
+    }
+    site_off = so_st.data();
+    r = r_st.data();
+    fil = fil_st.data();
+  } else {
+    for (int f = 0; f < nf; ++f) noise_off[f] = site_off[f];
+  }
   h->site_off.assign(site_off, site_off + nf + 1);
   const int64_t ns = g.ns, nb = g.nb;
   if ((rc = dalloc(h, &d.ctl, 1, false))) return rc;
   if ((rc = dalloc(h, &d.site_off, nf + 1, false))) return rc;
+  if ((rc = dalloc(h, &d.noise_off, nf, false))) return rc;
   if ((rc = dalloc(h, &d.fpar, 8 * (size_t)nf, false))) return rc;
   if ((rc = dalloc(h, &d.norm, 2 * (size_t)nf, false))) return rc;
   if ((rc = dalloc(h, &d.bond_fil, nb, false))) return rc;
@@ -465,6 +552,7 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     for (int i = 0; i < n - 1; ++i) bond_fil[site_off[f] - f + i] = f;
   }
   HIPCHK(h, hipMemcpyAsync(d.site_off, site_off, sizeof(int32_t) * (nf + 1), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.noise_off, noise_off.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(d.fpar, fpar.data(), sizeof(double) * fpar.size(), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(d.norm, norm.data(), sizeof(int32_t) * norm.size(), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(d.bond_fil, bond_fil.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, h->stream));
@@ -599,9 +687,9 @@ int sc_get_sites(sc_handle h, double *r, double *r_prev, double *f_rand) {
   int rc = check_handle(h, true);
   if (rc) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (r) get3(r, h->d.r, h->g.ns);
-  if (r_prev) get3(r_prev, h->d.rp, h->g.ns);
-  if (f_rand) get3(f_rand, h->d.fr, h->g.ns);
+  if (r && (rc = fetch_mapped(h, h->d.r, r, h->g.ns, 3, true))) return rc;
+  if (r_prev && (rc = fetch_mapped(h, h->d.rp, r_prev, h->g.ns, 3, true))) return rc;
+  if (f_rand && (rc = fetch_mapped(h, h->d.fr, f_rand, h->g.ns, 3, true))) return rc;
   return SC_OK;
 }
 
@@ -610,18 +698,21 @@ int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u, double *len)
   if (rc) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const int64_t nb = h->g.nb, ns = h->g.ns;
-  std::vector<double> rec(8 * (size_t)nb);
+  std::vector<double> rec(8 * (size_t)nb), sp(3 * (size_t)nb), uu(3 * (size_t)nb), ln(nb);
   HIPCHK(h, hipMemcpy(rec.data(), h->d.br, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
-  for (int64_t b = 0; b < nb; ++b)
+  for (int64_t b = 0; b < nb; ++b) {
     for (int k = 0; k < 3; ++k) {
-      if (spos) spos[k * nb + b] = rec[8 * b + k];
-      if (u) u[k * nb + b] = rec[8 * b + 3 + k];
+      sp[k * nb + b] = rec[8 * b + k];
+      uu[k * nb + b] = rec[8 * b + 3 + k];
     }
-  if (len)
-    for (int64_t b = 0; b < nb; ++b) len[b] = rec[8 * b + 6];
+    ln[b] = rec[8 * b + 6];
+  }
+  if (spos) to_caller(h, h->bond_map, sp.data(), spos, nb, 3);
+  if (u) to_caller(h, h->bond_map, uu.data(), u, nb, 3);
+  if (len) to_caller(h, h->bond_map, ln.data(), len, nb, 1);
   if (pos) {
     // bond centre r1 + 0.5 (r2 - r1) (src/bond.cpp:27-34), from the sites
-    std::vector<double> r(3 * (size_t)ns);
+    std::vector<double> r(3 * (size_t)ns), ps(3 * (size_t)nb);
     HIPCHK(h, hipMemcpy(r.data(), h->d.r, sizeof(double) * r.size(), hipMemcpyDeviceToHost));
     const int nf = h->g.nf;
     for (int f = 0; f < nf; ++f) {
@@ -630,10 +721,11 @@ int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u, double *len)
         const int64_t b = i - f;
         for (int k = 0; k < 3; ++k) {
           const double o = r[k * ns + i + 1] - r[k * ns + i];
-          pos[k * nb + b] = r[k * ns + i] + 0.5 * o;
+          ps[k * nb + b] = r[k * ns + i] + 0.5 * o;
         }
       }
     }
+    to_caller(h, h->bond_map, ps.data(), pos, nb, 3);
   }
   return SC_OK;
 }
@@ -642,8 +734,8 @@ int sc_get_bond_forces(sc_handle h, double *f, double *t) {
   int rc = check_handle(h, true);
   if (rc) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (f) get3(f, h->d.bf, h->g.nb);
-  if (t) get3(t, h->d.bt, h->g.nb);
+  if (f && (rc = fetch_mapped(h, h->d.bf, f, h->g.nb, 3, false))) return rc;
+  if (t && (rc = fetch_mapped(h, h->d.bt, t, h->g.nb, 3, false))) return rc;
   return SC_OK;
 }
 
@@ -652,8 +744,7 @@ int sc_get_tensions(sc_handle h, double *tension) {
   if (rc) return rc;
   if (!tension) return fail(h, SC_ERR_ARG, "null output");
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(tension, h->d.tension, sizeof(double) * h->g.nb, hipMemcpyDeviceToHost));
-  return SC_OK;
+  return fetch_mapped(h, h->d.tension, tension, h->g.nb, 1, false);
 }
 
 int sc_get_cell_keys(sc_handle h, int32_t *keys) {
@@ -661,7 +752,9 @@ int sc_get_cell_keys(sc_handle h, int32_t *keys) {
   if (rc) return rc;
   if (!keys) return fail(h, SC_ERR_ARG, "null output");
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(keys, h->d.keys, sizeof(int32_t) * h->g.nb, hipMemcpyDeviceToHost));
+  std::vector<int32_t> tmp(h->g.nb);
+  HIPCHK(h, hipMemcpy(tmp.data(), h->d.keys, sizeof(int32_t) * h->g.nb, hipMemcpyDeviceToHost));
+  to_caller(h, h->bond_map, tmp.data(), keys, h->g.nb, 1);
   return SC_OK;
 }
 
@@ -677,8 +770,10 @@ int sc_get_candidates(sc_handle h, int32_t *b1, int32_t *b2, size_t cap, size_t 
   if (c.nc)
     HIPCHK(h, hipMemcpy(pr.data(), h->d.pairs, sizeof(int2) * c.nc, hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < c.nc; ++i) {
-    if (b1) b1[i] = pr[i].x;
-    if (b2) b2[i] = pr[i].y;
+    const int64_t x = h->reordered ? h->bond_map[pr[i].x] : pr[i].x;
+    const int64_t y = h->reordered ? h->bond_map[pr[i].y] : pr[i].y;
+    if (b1) b1[i] = (int32_t)x;
+    if (b2) b2[i] = (int32_t)y;
   }
   return SC_OK;
 }
@@ -715,8 +810,7 @@ int sc_get_dr_state(sc_handle h, double *acc) {
   if (rc) return rc;
   if (!acc) return fail(h, SC_ERR_ARG, "null output");
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipMemcpy(acc, h->d.acc, sizeof(double) * h->g.ns, hipMemcpyDeviceToHost));
-  return SC_OK;
+  return fetch_mapped(h, h->d.acc, acc, h->g.ns, 1, true);
 }
 
 int sc_get_counters(sc_handle h, int64_t *i_step, int64_t *n_rebuilds,
@@ -738,6 +832,14 @@ int sc_set_sites(sc_handle h, const double *r) {
   if (!r) return fail(h, SC_ERR_ARG, "null input");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::vector<double> st;
+  if (h->reordered) {
+    const int64_t ns = h->g.ns;
+    st.resize(3 * (size_t)ns);
+    for (int k = 0; k < 3; ++k)
+      for (int64_t i = 0; i < ns; ++i) st[k * ns + i] = r[k * ns + h->site_map[i]];
+    r = st.data();
+  }
   HIPCHK(h, hipMemcpy(h->d.r, r, sizeof(double) * 3 * h->g.ns, hipMemcpyHostToDevice));
   sc::launch_bond_init(h->d, h->g, h->p.precision, h->stream);
   Ctl c = read_ctl(h);
@@ -761,14 +863,20 @@ int sc_filament_stats(sc_handle h, double *e2e, double *com) {
   double *buf = nullptr;
   HIPCHK(h, hipMalloc(&buf, sizeof(double) * 2 * n));
   sc::launch_filament_stats(h->d, h->g, buf, buf + n, h->stream);
-  hipError_t e = hipMemcpyAsync(e2e ? e2e : com, e2e ? buf : buf + n, sizeof(double) * n,
-                                hipMemcpyDeviceToHost, h->stream);
-  if (e == hipSuccess && e2e && com)
-    e = hipMemcpyAsync(com, buf + n, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream);
+  std::vector<double> tmp(2 * (size_t)n);
+  hipError_t e = hipMemcpyAsync(tmp.data(), buf, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
+                                h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(buf);
   if (e != hipSuccess) return fail(h, SC_ERR_HIP, std::string("sc_filament_stats: ") +
                                                       hipGetErrorString(e));
+  const int64_t nf = h->g.nf;
+  for (int k = 0; k < 3; ++k)
+    for (int64_t f = 0; f < nf; ++f) {
+      const int64_t u = h->reordered ? h->perm[f] : f;
+      if (e2e) e2e[k * nf + u] = tmp[k * nf + f];
+      if (com) com[k * nf + u] = tmp[n + k * nf + f];
+    }
   return SC_OK;
 }
 
@@ -791,6 +899,127 @@ int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows, int64_t *his
   if (e != hipSuccess) return fail(h, SC_ERR_HIP, std::string("sc_theta_histogram: ") +
                                                       hipGetErrorString(e));
   for (size_t k = 0; k < n; ++k) hist[k] += (int64_t)host[k];
+  return SC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct CkHeader {
+  char magic[8];
+  int32_t version, precision, spatial_order, nf;
+  int64_t ns, nb, nc, n_cells;
+  Ctl ctl;
+};
+
+constexpr char kCkMagic[8] = {'S', 'C', 'C', 'K', 'P', 'T', '0', '1'};
+
+// One field of the checkpoint: `n` elements of `T` at device address `dev`.
+struct CkField {
+  void *dev;
+  size_t bytes;
+};
+
+std::vector<CkField> ck_fields(sc_system *h, int64_t nc) {
+  const Dev &d = h->d;
+  const int64_t ns = h->g.ns, nb = h->g.nb, nf = h->g.nf;
+  const size_t slot = h->p.precision == SC_PREC_FP64 ? sizeof(double4) : sizeof(float4);
+  void *slots = h->p.precision == SC_PREC_FP64 ? (void *)d.slot64 : (void *)d.slot32;
+  return {{d.r, 24 * (size_t)ns},        {d.rp, 24 * (size_t)ns},
+          {d.fr, 24 * (size_t)ns},       {d.dr0, 24 * (size_t)ns},
+          {d.acc, 8 * (size_t)ns},       {d.acc_prev, 8 * (size_t)ns},
+          {d.br, 64 * (size_t)nb},       {d.bf, 24 * (size_t)nb},
+          {d.bt, 24 * (size_t)nb},       {d.tension, 8 * (size_t)nb},
+          {d.keys, 4 * (size_t)nb},      {d.norm, 8 * (size_t)nf},
+          {d.pairs, 8 * (size_t)nc},     {slots, slot * (size_t)nc},
+          {d.inc_start, 4 * (size_t)(nb + 1)}, {d.inc, 16 * (size_t)nc}};
+}
+
+size_t ck_size(sc_system *h, int64_t nc) {
+  size_t b = sizeof(CkHeader) + 4 * (size_t)(h->g.nf + 1) + 4 * (size_t)h->g.nf;
+  for (const CkField &f : ck_fields(h, nc)) b += f.bytes;
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_checkpoint_save(sc_handle h, void *buf, size_t cap, size_t *size) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const Ctl c = read_ctl(h);
+  const size_t need = ck_size(h, c.nc);
+  if (size) *size = need;
+  if (!buf) return SC_OK;
+  if (cap < need) return fail(h, SC_ERR_ARG, "checkpoint buffer too small");
+  char *p = static_cast<char *>(buf);
+  CkHeader hd{};
+  std::memcpy(hd.magic, kCkMagic, 8);
+  hd.version = 1;
+  hd.precision = h->p.precision;
+  hd.spatial_order = h->reordered ? 1 : 0;
+  hd.nf = h->g.nf;
+  hd.ns = h->g.ns;
+  hd.nb = h->g.nb;
+  hd.nc = c.nc;
+  hd.n_cells = h->g.n_cells;
+  hd.ctl = c;
+  std::memcpy(p, &hd, sizeof(hd));
+  p += sizeof(hd);
+  std::memcpy(p, h->site_off.data(), 4 * (size_t)(h->g.nf + 1));
+  p += 4 * (size_t)(h->g.nf + 1);
+  for (int f = 0; f < h->g.nf; ++f) {
+    const int32_t u = h->reordered ? h->perm[f] : f;
+    std::memcpy(p, &u, 4);
+    p += 4;
+  }
+  for (const CkField &f : ck_fields(h, c.nc)) {
+    if (f.bytes) HIPCHK(h, hipMemcpy(p, f.dev, f.bytes, hipMemcpyDeviceToHost));
+    p += f.bytes;
+  }
+  return SC_OK;
+}
+
+int sc_checkpoint_load(sc_handle h, const void *buf, size_t size) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!buf || size < sizeof(CkHeader)) return fail(h, SC_ERR_ARG, "checkpoint too short");
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
+  CkHeader hd;
+  std::memcpy(&hd, buf, sizeof(hd));
+  if (std::memcmp(hd.magic, kCkMagic, 8) != 0 || hd.version != 1)
+    return fail(h, SC_ERR_ARG, "not a checkpoint of this library");
+  if (hd.precision != h->p.precision || hd.spatial_order != (h->reordered ? 1 : 0) ||
+      hd.nf != h->g.nf || hd.ns != h->g.ns || hd.nb != h->g.nb || hd.n_cells != h->g.n_cells)
+    return fail(h, SC_ERR_ARG, "checkpoint of a different system layout");
+  if (size != ck_size(h, hd.nc)) return fail(h, SC_ERR_ARG, "checkpoint size mismatch");
+  const char *p = static_cast<const char *>(buf) + sizeof(hd);
+  if (std::memcmp(p, h->site_off.data(), 4 * (size_t)(h->g.nf + 1)) != 0)
+    return fail(h, SC_ERR_ARG, "checkpoint filament sizes differ");
+  p += 4 * (size_t)(h->g.nf + 1);
+  for (int f = 0; f < h->g.nf; ++f) {
+    int32_t u;
+    std::memcpy(&u, p, 4);
+    p += 4;
+    if (u != (h->reordered ? h->perm[f] : f))
+      return fail(h, SC_ERR_ARG, "checkpoint storage order differs");
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (hd.nc > h->g.cap) {
+    if ((rc = alloc_cand(h, hd.nc + hd.nc / 4 + 1024))) return rc;
+    h->pair_grid = (int)std::min<int64_t>((h->g.cap + 255) / 256, 8192);
+  }
+  for (const CkField &f : ck_fields(h, hd.nc)) {
+    if (f.bytes) HIPCHK(h, hipMemcpy(f.dev, p, f.bytes, hipMemcpyHostToDevice));
+    p += f.bytes;
+  }
+  if ((rc = write_ctl(h, hd.ctl))) return rc;
+  *h->ctl_host = hd.ctl;
+  h->i_step_host = hd.ctl.i_step;
   return SC_OK;
 }
 

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(64) void k_integrate(Dev d, Geo g,
         const int i = gl + G * k;
         if (i >= n) continue;
         double xi[3];
-        noise_xi(g.seed, s0 + i, mstep, xi);
+        noise_xi(g.seed, (int64_t)d.noise_off[f] + i, mstep, xi);
         const double t0 = TN[0][i], t1 = TN[1][i], t2 = TN[2][i];
         double xt[3];
         if (EXACT) {

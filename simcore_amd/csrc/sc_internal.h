@@ -57,6 +57,7 @@ struct Dev {
   Ctl *ctl;
   // filaments
   int32_t *site_off;       // nf+1
+  int32_t *noise_off;      // nf: site index the noise stream is keyed by
   double *fpar;            // nf*8: bl, kappa, gpar, gperp, sig_par, sig_perp, diam, 0
   int32_t *norm;           // nf*2: normalize_switch_, n_normalize_
   int32_t *bond_fil;       // nb

@@ -122,11 +122,23 @@ __global__ __launch_bounds__(256) void k_pair_fused(Dev d, Geo g) {
   const BondRec own = load_rec(d.br, b);
   double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
   const int32_t q0 = d.inc_start[b], q1 = d.inc_start[b + 1];
+  // software pipeline: the next incidence's entry and partner record are in
+  // flight while the current pair is computed
+  int2 ent_n = make_int2(0, 0);
+  BondRec oth_n{};
+  if (q0 < q1) {
+    ent_n = d.inc[q0];
+    oth_n = load_rec(d.br, ent_n.y);
+  }
   for (int32_t q = q0; q < q1; ++q) {
-    const int2 ent = d.inc[q];
+    const int2 ent = ent_n;
+    const BondRec oth = oth_n;
+    if (q + 1 < q1) {
+      ent_n = d.inc[q + 1];
+      oth_n = load_rec(d.br, ent_n.y);
+    }
     const int32_t c = ent.x >> 1;
     const bool second = (ent.x & 1) != 0;  // this bond is obj2 of candidate c
-    const BondRec oth = load_rec(d.br, ent.y);
     const BondRec &r1 = second ? oth : own;
     const BondRec &r2 = second ? own : oth;
     double fc[3], lam, mu;

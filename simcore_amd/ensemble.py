@@ -53,12 +53,14 @@ class Ensemble:
         return lib.sc_ensemble_clock(self.h)
 
     def submit(self, params, filaments, n_steps, precision=None, use_graph=True,
-               t_submit=-1.0):
+               t_submit=-1.0, spatial_order=None):
         """Queue one system; returns its request id."""
         p = ScParams()
         ctypes.pointer(p)[0] = params
         if precision is not None:
             p.precision = {"fp64": 0, "mixed": 1}[precision]
+        if spatial_order is not None:
+            p.spatial_order = int(bool(spatial_order))
         p.use_graph = int(bool(use_graph))
         so = np.ascontiguousarray(filaments.site_off, dtype=np.int32)
         r = np.ascontiguousarray(filaments.r, dtype=np.float64)

@@ -118,11 +118,12 @@ class FilamentSystem:
     """One system's device state (sc_handle)."""
 
     def __init__(self, params, filaments, precision="fp64", use_graph=False,
-                 capacity_factor=0.0):
+                 capacity_factor=0.0, spatial_order=False):
         p = ScParams()
         ctypes.pointer(p)[0] = params
         p.precision = {"fp64": SC_PREC_FP64, "mixed": SC_PREC_MIXED}[precision]
         p.use_graph = int(bool(use_graph))
+        p.spatial_order = int(bool(spatial_order))
         p.capacity_factor = capacity_factor
         self.params = p
         self.filaments = filaments
@@ -225,6 +226,20 @@ class FilamentSystem:
                self.h, "sc_get_counters")
         return {"i_step": i_step.value, "n_rebuilds": nreb.value,
                 "n_candidates": nc.value, "n_cells_1d": n1d.value}
+
+    def checkpoint(self):
+        """Complete dynamic state as bytes (sc_checkpoint_save)."""
+        n = c_size_t()
+        _check(lib.sc_checkpoint_save(self.h, None, 0, byref(n)), self.h, "sc_checkpoint_save")
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib.sc_checkpoint_save(self.h, buf, n.value, byref(n)), self.h,
+               "sc_checkpoint_save")
+        return buf.raw
+
+    def restore(self, blob):
+        """Continue from a checkpoint of a system with the same layout."""
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        _check(lib.sc_checkpoint_load(self.h, buf, len(blob)), self.h, "sc_checkpoint_load")
 
     def filament_stats(self):
         """(end-to-end vectors, centres of mass), each (3, nf)."""

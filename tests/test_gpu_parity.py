@@ -230,3 +230,72 @@ def test_uneven_lengths_mixed_graph_matches_eager():
     for a, b in zip(eager.sites(), graph.sites()):
         np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(eager.tensions(), graph.tensions())
+
+
+def test_spatial_order_getters_use_caller_order():
+    """Filaments stored in space-filling-curve order: every getter still
+    reports the caller's order (bit-identical FP64 bond state and cell keys
+    at load, the same candidate set)."""
+    print("case: C0 fp64 spatial order, load-time state", flush=True)
+    p, fil = c0_system()
+    dev = sc.FilamentSystem(p, fil, precision="fp64", spatial_order=True)
+    ora = OracleSystem(p, fil)
+    np.testing.assert_array_equal(dev.sites()[0], fil.r)
+    np.testing.assert_array_equal(dev.cell_keys(), ora.cell_keys())
+    for a, b in zip(dev.bonds(), ora.bonds()):
+        np.testing.assert_array_equal(a, b)
+    ob1, ob2 = culled(*ora.candidates(), fil.site_off)
+    db1, db2 = dev.candidates()
+    assert set(zip(db1.tolist(), db2.tolist())) == set(zip(ob1.tolist(), ob2.tolist()))
+
+
+@pytest.mark.parametrize("kind", ["c0", "short", "long"])
+def test_spatial_order_mixed_trajectory(kind):
+    """Mixed path with spatial storage order against the CPU path: same
+    tolerances as in caller order (the noise is keyed by caller sites)."""
+    print(f"case: {kind} mixed spatial order 1 and 100 steps", flush=True)
+    from tests.helpers import long_system, uneven_system
+    p, fil = {"c0": c0_system, "short": uneven_system, "long": long_system}[kind]()
+    dev = sc.FilamentSystem(p, fil, precision="mixed", use_graph=True, spatial_order=True)
+    ora = OracleSystem(p, fil)
+    dev.step(1)
+    assert ora.step(1) == 0
+    d1 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 1 step", d1)
+    assert d1 <= 1e-8
+    fd, _ = dev.bond_forces()
+    fo, _ = ora.bond_forces()
+    assert np.linalg.norm(fd - fo) / np.linalg.norm(fo) <= 1e-5
+    dev.step(99)
+    assert ora.step(99) == 0
+    d100 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 100 steps", d100)
+    assert d100 <= 1e-5
+    assert dev.counters()["n_rebuilds"] == ora.counters()["n_rebuilds"]
+    db1, db2 = dev.candidates()
+    ob1, ob2 = ora.candidates()
+    assert set(zip(db1.tolist(), db2.tolist())) == set(zip(ob1.tolist(), ob2.tolist()))
+
+
+@pytest.mark.parametrize("precision,spatial", [("fp64", False), ("mixed", True)])
+def test_checkpoint_resume_is_bit_identical(precision, spatial):
+    """Save after 31 steps (odd: next is a full step), continue 40 steps;
+    a fresh handle restored from the checkpoint reaches the same state."""
+    print(f"case: C0 {precision} spatial={spatial} checkpoint at 31, +40 steps", flush=True)
+    p, fil = c0_system()
+    a = sc.FilamentSystem(p, fil, precision=precision, use_graph=True, spatial_order=spatial)
+    a.step(31)
+    blob = a.checkpoint()
+    a.step(40)
+    b = sc.FilamentSystem(p, fil, precision=precision, use_graph=True, spatial_order=spatial)
+    b.restore(blob)
+    assert b.counters()["i_step"] == 31
+    b.step(40)
+    for x, y in zip(a.sites(), b.sites()):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a.tensions(), b.tensions())
+    assert a.counters() == b.counters()
+    if precision == "fp64":
+        ora = OracleSystem(p, fil)
+        assert ora.step(71) == 0
+        np.testing.assert_array_equal(b.sites()[0], ora.sites()[0])
