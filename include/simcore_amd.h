@@ -175,6 +175,21 @@ int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows,
 int sc_checkpoint_save(sc_handle h, void *buf, size_t cap, size_t *size);
 int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
 
+/* Autotuner (no counterpart in the reference): times the launch shapes
+ * that leave every result unchanged -- threads per block of the pair
+ * kernel (128/256/512), its XCD-contiguous block order on or off, and
+ * waves (filaments) per block of the integrator (1/2/4) -- over `steps`
+ * steps each from a checkpoint of the current state, keeps the fastest and
+ * restores the state, so the trajectory is the one without tuning.  The
+ * cell size is not tuned: in the reference it sets the rebuild threshold
+ * and the candidate set (src/cell_list.cpp:9-64), i.e. it is part of the
+ * model. */
+typedef struct sc_tune_report {
+  int32_t pair_block, pair_xcd, integrate_waves, n_trials;
+  double step_ms_start, step_ms_best;
+} sc_tune_report;
+int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report);
+
 /* Device bytes this handle holds. */
 int sc_device_bytes(sc_handle h, size_t *bytes);
 /* Average per-launch duration (ms) of each kernel family recorded with HIP

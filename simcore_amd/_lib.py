@@ -31,7 +31,7 @@ EXPORTS = [
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
     "sc_generate_filaments", "sc_copy_probe", "sc_step_async", "sc_step_poll",
     "sc_filament_stats", "sc_theta_histogram", "sc_output_header", "sc_spec_frame",
-    "sc_posit_frame", "sc_checkpoint_save", "sc_checkpoint_load",
+    "sc_posit_frame", "sc_checkpoint_save", "sc_checkpoint_load", "sc_autotune",
     "sc_ensemble_create", "sc_ensemble_submit", "sc_ensemble_progress",
     "sc_ensemble_drain", "sc_ensemble_request", "sc_ensemble_result",
     "sc_ensemble_stats_get", "sc_ensemble_clock", "sc_ensemble_last_error",
@@ -85,6 +85,12 @@ class ScEnsembleStats(ctypes.Structure):
                 ("arena_bytes", c_size_t)]
 
 
+class ScTuneReport(ctypes.Structure):
+    _fields_ = [("pair_block", c_int32), ("pair_xcd", c_int32),
+                ("integrate_waves", c_int32), ("n_trials", c_int32),
+                ("step_ms_start", c_double), ("step_ms_best", c_double)]
+
+
 class SimcoreError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"simcore_amd error {code}: {msg}")
@@ -125,6 +131,7 @@ def _load():
         "sc_filament_stats": ([H, dp, dp], c_int),
         "sc_checkpoint_save": ([H, c_void_p, c_size_t, POINTER(c_size_t)], c_int),
         "sc_checkpoint_load": ([H, c_void_p, c_size_t], c_int),
+        "sc_autotune": ([H, c_int32, POINTER(ScTuneReport)], c_int),
         "sc_theta_histogram": ([H, c_int32, c_int32, lp], c_int),
         "sc_device_bytes": ([H, POINTER(c_size_t)], c_int),
         "sc_profile_enable": ([H, c_int], c_int),

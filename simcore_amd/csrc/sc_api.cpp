@@ -42,7 +42,7 @@ struct sc_system {
   std::vector<int32_t> perm;
   std::vector<int64_t> site_map, bond_map;
   std::vector<Bin> bins;
-  int pair_grid = 1024;
+  sc::Tune tune;
   // graphs: [0] odd step (rebuild stage included), [1] even step (a rebuild
   // is only ever decided after a full step, i.e. at odd steps), [2] the
   // odd+even pair
@@ -195,7 +195,7 @@ void enqueue_step(sc_system *h, bool odd) {
     (void)hipEventRecord(h->ev[7], st);
     (void)hipEventRecord(h->ev[0], st);
   }
-  sc::launch_pair(d, g, h->p.precision, h->pair_grid, st);
+  sc::launch_pair(d, g, h->p.precision, h->tune, st);
   if (pf) {
     (void)hipEventRecord(h->ev[1], st);
     (void)hipEventRecord(h->ev[2], st);
@@ -205,7 +205,7 @@ void enqueue_step(sc_system *h, bool odd) {
     (void)hipEventRecord(h->ev[4], st);
   }
   for (const Bin &b : h->bins)
-    sc::launch_integrate(d, g, b.list, b.count, b.maxn, h->p.precision, st);
+    sc::launch_integrate(d, g, b.list, b.count, b.maxn, h->p.precision, h->tune, st);
   if (pf) (void)hipEventRecord(h->ev[5], st);
   sc::launch_commit(d, st);
 }
@@ -595,10 +595,6 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   double cf = h->p.capacity_factor > 0 ? h->p.capacity_factor : 4.0;
   int64_t cap = (int64_t)(cf * nb) + 4096;
   if ((rc = alloc_cand(h, cap))) return rc;
-  {
-    int64_t blocks = (cap + 255) / 256;
-    h->pair_grid = (int)std::min<int64_t>(std::max<int64_t>(blocks, 1), 8192);
-  }
   sc::launch_bond_init(d, g, h->p.precision, h->stream);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipGetLastError());
@@ -667,7 +663,6 @@ int sc_step_poll(sc_handle h, int *done) {
       h->pending = false;
       return rc;
     }
-    h->pair_grid = (int)std::min<int64_t>((ncap + 255) / 256, 8192);
     c.abort = 0;
     c.overflow = 0;
     if ((rc = write_ctl(h, c))) {
@@ -1011,7 +1006,6 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (hd.nc > h->g.cap) {
     if ((rc = alloc_cand(h, hd.nc + hd.nc / 4 + 1024))) return rc;
-    h->pair_grid = (int)std::min<int64_t>((h->g.cap + 255) / 256, 8192);
   }
   for (const CkField &f : ck_fields(h, hd.nc)) {
     if (f.bytes) HIPCHK(h, hipMemcpy(f.dev, p, f.bytes, hipMemcpyHostToDevice));
@@ -1020,6 +1014,87 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size) {
   if ((rc = write_ctl(h, hd.ctl))) return rc;
   *h->ctl_host = hd.ctl;
   h->i_step_host = hd.ctl.i_step;
+  return SC_OK;
+}
+
+int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
+  if (steps < 2) steps = 2;
+  steps += steps & 1;  // whole midstep/full-step pairs
+  size_t n = 0;
+  if ((rc = sc_checkpoint_save(h, nullptr, 0, &n))) return rc;
+  std::vector<char> blob(n);
+  if ((rc = sc_checkpoint_save(h, blob.data(), n, &n))) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  const bool prof = h->prof;
+  h->prof = false;
+  // time `steps` steps of configuration t from the saved state
+  auto trial = [&](const sc::Tune &t, double *ms) -> int {
+    h->tune = t;
+    drop_graph(h);
+    int r = sc_checkpoint_load(h, blob.data(), blob.size());
+    if (r) return r;
+    if (h->p.use_graph && (r = capture_graphs(h))) return r;
+    // one untimed pair of steps (first launches of fresh graphs)
+    if ((r = enqueue_range(h, h->i_step_host, 2))) return r;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->pending = false;
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+    if ((r = enqueue_range(h, h->i_step_host + 2, steps))) return r;
+    HIPCHK(h, hipEventRecord(e1, h->stream));
+    HIPCHK(h, hipEventSynchronize(e1));
+    h->pending = false;
+    if (h->ctl_host->abort) return fail(h, SC_ERR_STATE, "step aborted during tuning");
+    float f = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&f, e0, e1));
+    *ms = f / steps;
+    return SC_OK;
+  };
+  const sc::Tune start = h->tune;
+  double best_ms = 0, start_ms = 0;
+  sc::Tune best = start;
+  int trials = 0;
+  rc = trial(start, &start_ms);
+  best_ms = start_ms;
+  const int blocks[] = {128, 256, 512};
+  const int waves[] = {1, 2, 4};
+  for (int bs : blocks)
+    for (int x = 0; x < 2 && rc == SC_OK; ++x)
+      for (int w : waves) {
+        if (rc) break;
+        sc::Tune t;
+        t.pair_block = bs;
+        t.pair_xcd = x;
+        t.int_waves = w;
+        double ms = 0;
+        rc = trial(t, &ms);
+        ++trials;
+        if (rc == SC_OK && ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+  // leave the system where it was, with the chosen shapes
+  h->tune = rc ? start : best;
+  drop_graph(h);
+  const int rc2 = sc_checkpoint_load(h, blob.data(), blob.size());
+  h->prof = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (report) {
+    report->pair_block = best.pair_block;
+    report->pair_xcd = best.pair_xcd;
+    report->integrate_waves = best.int_waves;
+    report->n_trials = trials;
+    report->step_ms_start = start_ms;
+    report->step_ms_best = best_ms;
+  }
   return SC_OK;
 }
 

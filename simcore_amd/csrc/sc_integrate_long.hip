@@ -1,0 +1,566 @@
+// sc_integrate_long.hip -- register-resident fast-path integrator for
+// filaments of 65 to 256 sites (Filament::UpdatePosition,
+// src/filament.cpp:394-428).
+//
+// One wavefront per filament; lane l owns the S consecutive sites
+// lS .. lS+S-1 (S = 2 or 4), so a neighbour is either in the lane's own
+// registers or one lane-shuffle away.  Tridiagonal systems: log2(S) levels
+// of cyclic reduction inside each lane leave one row per lane, solved by
+// parallel cyclic reduction across the 64 lanes, then back-substituted
+// level by level.  Metric determinants: 2x2 matrix products, sequential
+// inside a lane, scanned across lanes.  Renormalisation: prefix sums the
+// same way.  No LDS, no barriers.  Same reduced algebra as the short-filament
+// kernel (sc_integrate_reg.hip).
+#include <hip/hip_runtime.h>
+
+#include "sc_internal.h"
+#include "sc_lanes.h"
+#include "sc_math.h"
+
+namespace sc {
+
+using namespace lanes;
+
+namespace {
+
+// o[k] = value at site lS + k + DELTA (fb outside the wave's 64*S sites)
+template <int S, int DELTA>
+__device__ __forceinline__ void site_at(const double (&v)[S], double (&o)[S], int lane,
+                                        double fb) {
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int t = k + DELTA;
+    const int q = t >= 0 ? t / S : -((-t + S - 1) / S);
+    const int ks = t - q * S;
+    if (q == 0) {
+      o[k] = v[ks];
+    } else {
+      const int src = lane + q;
+      const bool ok = src >= 0 && src < 64;
+      const double y = __shfl(v[ks], ok ? src : lane, 64);
+      o[k] = ok ? y : fb;
+    }
+  }
+}
+
+template <int S, int DELTA>
+__device__ __forceinline__ void site_at3(const double (&v)[S][3], double (&o)[S][3], int lane) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double a[S], b[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) a[k] = v[k][q];
+    site_at<S, DELTA>(a, b, lane, 0.0);
+#pragma unroll
+    for (int k = 0; k < S; ++k) o[k][q] = b[k];
+  }
+}
+
+// Solves rows i = lS + k < m (lower a: x[i-1], diagonal b, upper c:
+// x[i+1], right-hand side d); rows >= m are inert.  Result in x.
+template <int S>
+__device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], double (&c)[S],
+                                              double (&d)[S], int m, int lane,
+                                              double (&x)[S]) {
+#pragma unroll
+  for (int k = 0; k < S; ++k)
+    if (lane * S + k >= m) {
+      a[k] = 0.0;
+      b[k] = 1.0;
+      c[k] = 0.0;
+      d[k] = 0.0;
+    }
+  // cyclic reduction inside the lane: at stride st the rows k = 0 mod 2st
+  // absorb their neighbours k - st (the previous lane's S - st for k = 0)
+  // and k + st; the rows k = st mod 2st keep their coefficients for the
+  // back substitution
+#pragma unroll
+  for (int st = 1; st < S; st <<= 1) {
+    const bool first = lane == 0;
+    const double pa = first ? 0.0 : __shfl(a[S - st], first ? lane : lane - 1, 64);
+    const double pb = first ? 1.0 : __shfl(b[S - st], first ? lane : lane - 1, 64);
+    const double pc = first ? 0.0 : __shfl(c[S - st], first ? lane : lane - 1, 64);
+    const double pd = first ? 0.0 : __shfl(d[S - st], first ? lane : lane - 1, 64);
+#pragma unroll
+    for (int k = 0; k < S; k += 2 * st) {
+      const double la = k == 0 ? pa : a[k - st], lb = k == 0 ? pb : b[k - st];
+      const double lc = k == 0 ? pc : c[k - st], ld = k == 0 ? pd : d[k - st];
+      const double ua = a[k + st], ub = b[k + st], uc = c[k + st], ud = d[k + st];
+      const double al = -a[k] / lb, ga = -c[k] / ub;
+      a[k] = al * la;
+      c[k] = ga * uc;
+      b[k] = b[k] + al * lc + ga * ua;
+      d[k] = d[k] + al * ld + ga * ud;
+    }
+  }
+  // one row per lane left: parallel cyclic reduction across the lanes
+  const int m_red = (m + S - 1) / S;
+  x[0] = pcr<64>(a[0], b[0], c[0], d[0], m_red, lane, lane);
+#pragma unroll
+  for (int st = S / 2; st >= 1; st >>= 1) {
+    const double nx = __shfl(x[0], lane < 63 ? lane + 1 : lane, 64);
+#pragma unroll
+    for (int k = st; k < S; k += 2 * st) {
+      const double xl = x[k - st];
+      const double xu = k + st < S ? x[k + st] : (lane < 63 ? nx : 0.0);
+      x[k] = (d[k] - a[k] * xl - c[k] * xu) / b[k];
+    }
+  }
+}
+
+__device__ __forceinline__ void mat_mul(const double *P, const double *Q, double *R) {
+  R[0] = P[0] * Q[0] + P[1] * Q[2];
+  R[1] = P[0] * Q[1] + P[1] * Q[3];
+  R[2] = P[2] * Q[0] + P[3] * Q[2];
+  R[3] = P[2] * Q[1] + P[3] * Q[3];
+}
+
+// Prefix (forward) products of per-site 2x2 matrices M (newest on the
+// left): out[k] = M_{lS+k} ... M_0 applied to [2, 1], first component.
+template <int S>
+__device__ __forceinline__ void det_forward(const double (&M)[S][4], double (&out)[S],
+                                            int lane) {
+  double L[S][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) L[0][q] = M[0][q];
+#pragma unroll
+  for (int k = 1; k < S; ++k) mat_mul(M[k], L[k - 1], L[k]);
+  double A[4] = {L[S - 1][0], L[S - 1][1], L[S - 1][2], L[S - 1][3]};
+  mat_scan<64>(A, lane, lane, -1);  // A_l ... A_0
+  double E[4];
+  const double id[4] = {1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double y = __shfl(A[q], lane > 0 ? lane - 1 : lane, 64);
+    E[q] = lane > 0 ? y : id[q];
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    double P[4];
+    mat_mul(L[k], E, P);
+    out[k] = P[0] * 2.0 + P[1] * 1.0;
+  }
+}
+
+// Suffix products: out[k] = N_{lS+k} ... N_last applied to [2, 1].
+template <int S>
+__device__ __forceinline__ void det_backward(const double (&N)[S][4], double (&out)[S],
+                                             int lane) {
+  double R[S][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) R[S - 1][q] = N[S - 1][q];
+#pragma unroll
+  for (int k = S - 2; k >= 0; --k) mat_mul(N[k], R[k + 1], R[k]);
+  double B[4] = {R[0][0], R[0][1], R[0][2], R[0][3]};
+  mat_scan<64>(B, lane, lane, +1);  // B_l ... B_63
+  double F[4];
+  const double id[4] = {1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double y = __shfl(B[q], lane < 63 ? lane + 1 : lane, 64);
+    F[q] = lane < 63 ? y : id[q];
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    double P[4];
+    mat_mul(R[k], F, P);
+    out[k] = P[0] * 2.0 + P[1] * 1.0;
+  }
+}
+
+}  // namespace
+
+template <int S>
+__global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int32_t *list,
+                                                        int count, int prec) {
+  const Ctl *ctl = d.ctl;
+  if (ctl->abort) return;
+  const int lane = threadIdx.x & 63;
+  const int fidx = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const bool factive = fidx < count;
+  const int f = factive ? list[fidx] : 0;
+  const int64_t s0 = factive ? d.site_off[f] : 0;
+  const int n = factive ? (int)(d.site_off[f + 1] - s0) : 0;
+  const int64_t b0 = s0 - f;
+  const int64_t ns = g.ns;
+  const bool mid = ctl->midstep != 0;
+  const int64_t step = ctl->cur_step;
+  const int m = n - 1;
+  double bl = 1.0, kappa = 0.0, gpar = 1.0, gperp = 1.0, sgpar = 0.0, sgperp = 0.0;
+  int64_t noise0 = 0;
+  if (factive) {
+    const double *fp = d.fpar + 8 * f;
+    bl = fp[0];
+    kappa = fp[1];
+    gpar = fp[2];
+    gperp = fp[3];
+    sgpar = fp[4];
+    sgperp = fp[5];
+    noise0 = d.noise_off[f];
+  }
+  const int base = lane * S;
+
+  // ---- loads -------------------------------------------------------------
+  double rp[S][3], u[S][3], Fb[S][3], Tb[S][3], fr[S][3], dr0[S][3], acc[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int i = base + k;
+    const int64_t si = s0 + i;
+    for (int q = 0; q < 3; ++q) {
+      rp[k][q] = 0.0;
+      u[k][q] = q == 0 ? 1.0 : 0.0;
+      Fb[k][q] = 0.0;
+      Tb[k][q] = 0.0;
+      fr[k][q] = 0.0;
+      dr0[k][q] = 0.0;
+    }
+    acc[k] = 0.0;
+    if (i < n) {
+      for (int q = 0; q < 3; ++q) rp[k][q] = mid ? d.r[q * ns + si] : d.rp[q * ns + si];
+      const int64_t jb = b0 + (i < m ? i : m - 1);
+      const double4 *rec = reinterpret_cast<const double4 *>(d.br + 8 * jb);
+      const double4 r0 = rec[0], r1 = rec[1];
+      u[k][0] = r0.w;
+      u[k][1] = r1.x;
+      u[k][2] = r1.y;
+      if (i < m)
+        for (int q = 0; q < 3; ++q) {
+          Fb[k][q] = d.bf[q * g.nb + jb];
+          Tb[k][q] = d.bt[q * g.nb + jb];
+        }
+      if (!mid && g.stoch)
+        for (int q = 0; q < 3; ++q) fr[k][q] = d.fr[q * ns + si];
+      if (!mid) {
+        acc[k] = d.acc[si];
+        for (int q = 0; q < 3; ++q) dr0[k][q] = d.dr0[q * ns + si];
+      }
+    }
+  }
+
+  // ---- ApplyInteractionForces (src/filament.cpp:992-1048) ---------------
+  const double linv = 1.0 / bl;
+  double sf[S][3], pt[S][3], sfm[S][3], ptm[S][3], fs[S][3];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    cross3(u[k], Tb[k], pt[k]);
+    for (int q = 0; q < 3; ++q) {
+      pt[k][q] *= linv;
+      sf[k][q] = 0.5 * Fb[k][q];
+    }
+  }
+  site_at3<S, -1>(sf, sfm, lane);
+  site_at3<S, -1>(pt, ptm, lane);
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int i = base + k;
+    for (int q = 0; q < 3; ++q) fs[k][q] = 0.0;
+    if (i >= 1)
+      for (int q = 0; q < 3; ++q) fs[k][q] = fs[k][q] + sfm[k][q] + (-ptm[k][q]);
+    if (i <= n - 2)
+      for (int q = 0; q < 3; ++q) fs[k][q] = fs[k][q] + sf[k][q] + pt[k][q];
+  }
+
+  // ---- angles and tangents ----------------------------------------------
+  double um[S][3], up[S][3], t[S][3], cs[S];
+  site_at3<S, -1>(u, um, lane);
+  site_at3<S, 1>(u, up, lane);
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int i = base + k;
+    cs[k] = i < n - 2 ? dot(u[k], up[k]) : 0.0;
+    if (i == 0 || i >= n - 1) {
+      for (int q = 0; q < 3; ++q) t[k][q] = u[k][q];
+    } else {
+      for (int q = 0; q < 3; ++q) t[k][q] = um[k][q] + u[k][q];
+      const double inv = 1.0 / sqrt(dot(t[k], t[k]));
+      for (int q = 0; q < 3; ++q) t[k][q] *= inv;
+    }
+  }
+  double csm[S];
+  site_at<S, -1>(cs, csm, lane, 0.0);
+
+  // ---- random forces (midstep) ------------------------------------------
+  if (mid) {
+    if (g.stoch) {
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        const int i = base + k;
+        double xi[3] = {0.0, 0.0, 0.0};
+        if (i < n) noise_xi(g.seed, noise0 + i, (step - 1) >> 1, xi);
+        const double txi = dot(t[k], xi);
+        for (int q = 0; q < 3; ++q)
+          fr[k][q] = sgperp * xi[q] + (sgpar - sgperp) * t[k][q] * txi;
+      }
+      double frn[S][3], a[S], b[S], c[S], rh[S], Tp[S], Tm[S];
+      site_at3<S, 1>(fr, frn, lane);
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        const int i = base + k;
+        double df[3];
+        for (int q = 0; q < 3; ++q) df[q] = frn[k][q] - fr[k][q];
+        rh[k] = dot(df, u[k]);
+        a[k] = i >= 1 ? -csm[k] : 0.0;
+        b[k] = 2.0;
+        c[k] = i < m - 1 ? -cs[k] : 0.0;
+      }
+      solve_blocked<S>(a, b, c, rh, m, lane, Tp);
+      site_at<S, -1>(Tp, Tm, lane, 0.0);
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        const int i = base + k;
+        if (i == 0) {
+          for (int q = 0; q < 3; ++q) fr[k][q] = fr[k][q] + Tp[k] * u[k][q];
+        } else if (i == n - 1) {
+          for (int q = 0; q < 3; ++q) fr[k][q] = fr[k][q] - Tm[k] * um[k][q];
+        } else {
+          for (int q = 0; q < 3; ++q) fr[k][q] = fr[k][q] + Tp[k] * u[k][q] - Tm[k] * um[k][q];
+        }
+        if (i < n)
+          for (int q = 0; q < 3; ++q) d.fr[q * ns + s0 + i] = fr[k][q];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+      if (base + k < n)
+        for (int q = 0; q < 3; ++q) d.rp[q * ns + s0 + base + k] = rp[k][q];
+  }
+  if (g.stoch)
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+      for (int q = 0; q < 3; ++q) fs[k][q] += fr[k][q];
+
+  // ---- bending (src/filament.cpp:608-776), reduced form ------------------
+  {
+    double csmm[S], M[S][4], N[S][4], det_t[S], det_b[S], db3[S], ke[S];
+    site_at<S, -1>(csm, csmm, lane, 0.0);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int i = base + k;
+      const bool sm = i >= 2 && i < n;
+      M[k][0] = sm ? 2.0 : 1.0;
+      M[k][1] = sm ? -csmm[k] * csmm[k] : 0.0;
+      M[k][2] = sm ? 1.0 : 0.0;
+      M[k][3] = sm ? 0.0 : 1.0;
+      const bool sn = i >= 1 && i <= n - 2;
+      N[k][0] = sn ? 2.0 : 1.0;
+      N[k][1] = sn ? -csm[k] * csm[k] : 0.0;
+      N[k][2] = sn ? 1.0 : 0.0;
+      N[k][3] = sn ? 0.0 : 1.0;
+    }
+    det_forward<S>(M, det_t, lane);
+    det_backward<S>(N, det_b, lane);
+    // det_t at the last site, det_b three sites ahead (det_b[n] = 1)
+    const int last = n > 0 ? n - 1 : 0;
+    double pick = 0.0;
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+      if (k == last % S) pick = det_t[k];
+    const double det_g = __shfl(pick, last / S, 64);
+    site_at<S, 3>(det_b, db3, lane, 1.0);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int i = base + k;
+      if (i == 0) det_t[k] = 1.0;
+      if (i + 3 >= n) db3[k] = 1.0;
+      ke[k] = 0.0;
+      if (i < n - 2) {
+        const double ginv = cs[k] * det_t[k] * db3[k] * (1.0 / det_g);
+        ke[k] = (kappa + bl * ginv) * (linv * linv);
+      }
+    }
+    double kem[S], kemm[S], umm[S][3];
+    site_at<S, -1>(ke, kem, lane, 0.0);
+    site_at<S, -2>(ke, kemm, lane, 0.0);
+    site_at3<S, -2>(u, umm, lane);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int i = base + k;
+      double f3[3] = {0.0, 0.0, 0.0}, pa[3], pb[3];
+      if (i > 1 && i < n) {
+        proj_perp(umm[k], um[k], pa);
+        for (int q = 0; q < 3; ++q) f3[q] += kemm[k] * pa[q];
+      }
+      if (i > 0 && i < n - 1) {
+        proj_perp(u[k], um[k], pa);
+        proj_perp(um[k], u[k], pb);
+        for (int q = 0; q < 3; ++q) f3[q] += kem[k] * (pa[q] - pb[q]);
+      }
+      if (i < n - 2) {
+        proj_perp(up[k], u[k], pa);
+        for (int q = 0; q < 3; ++q) f3[q] -= ke[k] * pa[q];
+      }
+      for (int q = 0; q < 3; ++q) fs[k][q] += f3[q];
+    }
+  }
+
+  // ---- tensions (src/filament.cpp:778-838) -------------------------------
+  const double igpar = 1.0 / gpar, igperp = 1.0 / gperp, dg = igpar - igperp;
+  double T[S], Tm[S];
+  {
+    double tn[S][3], fn[S][3], hu[S], hun[S], a[S], b[S], c[S], rh[S];
+    site_at3<S, 1>(t, tn, lane);
+    site_at3<S, 1>(fs, fn, lane);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      double m1[3], m2[3], fd[3];
+      mob(igperp, dg, t[k], fs[k], m1);
+      mob(igperp, dg, tn[k], fn[k], m2);
+      for (int q = 0; q < 3; ++q) fd[q] = m2[q] - m1[q];
+      rh[k] = dot(u[k], fd);
+      const double a1 = dot(t[k], u[k]), a2 = dot(tn[k], u[k]);
+      b[k] = 2.0 * igperp + dg * (a1 * a1 + a2 * a2);
+      hu[k] = -igperp * dot(u[k], um[k]) - dg * (dot(t[k], um[k]) * dot(t[k], u[k]));
+    }
+    site_at<S, 1>(hu, hun, lane, 0.0);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int i = base + k;
+      a[k] = i >= 1 ? hu[k] : 0.0;
+      c[k] = i < m - 1 ? hun[k] : 0.0;
+    }
+    solve_blocked<S>(a, b, c, rh, m, lane, T);
+  }
+  site_at<S, -1>(T, Tm, lane, 0.0);
+#pragma unroll
+  for (int k = 0; k < S; ++k)
+    if (base + k < m) d.tension[b0 + base + k] = T[k];
+
+  // ---- positions (src/filament.cpp:840-927) ------------------------------
+  const double delta = mid ? 0.5 * g.delta : g.delta;
+  double rn[S][3], rnn[S][3], uo[S][3];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int i = base + k;
+    if (i == 0) {
+      for (int q = 0; q < 3; ++q) fs[k][q] += T[k] * u[k][q];
+    } else if (i == n - 1) {
+      for (int q = 0; q < 3; ++q) fs[k][q] += -Tm[k] * um[k][q];
+    } else {
+      for (int q = 0; q < 3; ++q) fs[k][q] += T[k] * u[k][q] - Tm[k] * um[k][q];
+    }
+    double mv[3];
+    mob(igperp, dg, t[k], fs[k], mv);
+    for (int q = 0; q < 3; ++q) rn[k][q] = rp[k][q] + delta * mv[q];
+  }
+  site_at3<S, 1>(rn, rnn, lane);
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int i = base + k;
+    double rd[3];
+    for (int q = 0; q < 3; ++q) rd[q] = rnn[k][q] - rn[k][q];
+    const double mm = sqrt(dot(rd, rd));
+    const double inv = 1.0 / mm;
+    for (int q = 0; q < 3; ++q) uo[k][q] = rd[q] * inv;
+    if (i < m) {
+      const double diff = bl - mm;
+      if ((diff < 0 ? -diff : diff) / bl > 1e-3) bad = 1;
+    }
+  }
+  if (__ballot(bad) != 0) {
+    // bond renormalisation (src/filament.cpp:904-924): cumulative sums of
+    // l u along the chain, tail to head or head to tail by the switch
+    int32_t sw = factive ? d.norm[2 * f] : 1;
+    sw = !sw;
+    double st3[S][3];
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+      for (int q = 0; q < 3; ++q) st3[k][q] = base + k < m ? bl * uo[k][q] : 0.0;
+    const int last = n > 0 ? n - 1 : 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double loc[S], tot;
+      if (sw) {
+        // r_i = r_0 + sum_{j < i} l u_j
+        double run = 0.0;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+          loc[k] = run;  // exclusive inside the lane
+          run += st3[k][q];
+        }
+        tot = run;
+        const double incl = sum_scan<64>(tot, lane, lane, -1);
+        const double y = __shfl(incl, lane > 0 ? lane - 1 : lane, 64);
+        const double before = lane > 0 ? y : 0.0;
+        const double r0 = __shfl(rn[0][q], 0, 64);
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+          if (base + k >= 1) rn[k][q] = r0 + (before + loc[k]);
+      } else {
+        // r_i = r_{n-1} - sum_{j >= i} l u_j
+        double run = 0.0;
+#pragma unroll
+        for (int k = S - 1; k >= 0; --k) {
+          run += st3[k][q];
+          loc[k] = run;  // inclusive suffix inside the lane
+        }
+        tot = run;
+        const double incl = sum_scan<64>(tot, lane, lane, +1);
+        const double y = __shfl(incl, lane < 63 ? lane + 1 : lane, 64);
+        const double after = lane < 63 ? y : 0.0;
+        double pick = 0.0;
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+          if (k == last % S) pick = rn[k][q];
+        const double rl = __shfl(pick, last / S, 64);
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+          if (base + k < n - 1) rn[k][q] = rl - (after + loc[k]);
+      }
+    }
+    site_at3<S, 1>(rn, rnn, lane);
+    if (lane == 0 && factive) {
+      d.norm[2 * f] = sw;
+      d.norm[2 * f + 1] = 1;
+    }
+  } else if (lane == 0 && factive) {
+    d.norm[2 * f + 1] += 1;
+  }
+
+  // ---- write back, bond records, rebuild bookkeeping ---------------------
+  double lmax = 0.0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int i = base + k;
+    if (i >= n) continue;
+    const int64_t si = s0 + i;
+    for (int q = 0; q < 3; ++q) d.r[q * ns + si] = rn[k][q];
+    if (i < m) bond_reinit(d, g, b0 + i, rn[k], rnn[k], prec);
+    if (!mid) {
+      double a = acc[k];
+      d.acc_prev[si] = a;
+      for (int rep = 0; rep < 2; ++rep)
+        for (int q = 0; q < 3; ++q) {
+          const double dd = rn[k][q] - dr0[k][q];
+          a += dd * dd;
+        }
+      d.acc[si] = a;
+      lmax = a > lmax ? a : lmax;
+    }
+  }
+  if (!mid) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(lmax, o, 64);
+      lmax = y > lmax ? y : lmax;
+    }
+    if (lane == 0)
+      atomicMax((unsigned long long *)&d.ctl->dr_bits,
+                (unsigned long long)__double_as_longlong(lmax));
+  }
+}
+
+bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list, int bin_count,
+                           int bin_maxn, int precision, int waves, hipStream_t st) {
+  if (precision == SC_PREC_FP64 || bin_maxn <= 64 || bin_maxn > 256) return false;
+  if (bin_count <= 0) return true;
+  dim3 grid((bin_count + waves - 1) / waves), block(64 * waves);
+  if (bin_maxn <= 128)
+    hipLaunchKernelGGL(k_integrate_long<2>, grid, block, 0, st, d, g, bin_list, bin_count,
+                       precision);
+  else
+    hipLaunchKernelGGL(k_integrate_long<4>, grid, block, 0, st, d, g, bin_list, bin_count,
+                       precision);
+  return true;
+}
+
+}  // namespace sc

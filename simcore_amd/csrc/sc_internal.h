@@ -79,6 +79,13 @@ struct Dev {
   int32_t *scan_tmp;
 };
 
+// Launch shapes the autotuner chooses among; none changes a result.
+struct Tune {
+  int pair_block = 256;  // threads per block of the pair kernel
+  int pair_xcd = 0;      // 1: consecutive bond ranges stay on one XCD
+  int int_waves = 1;     // waves per block of the register integrator
+};
+
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
@@ -89,16 +96,20 @@ enum Guard { kGuardStep = 0, kGuardRebuild = 1 };
 void launch_decide(const Dev &d, const Geo &g, hipStream_t st);
 void launch_commit(const Dev &d, hipStream_t st);
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st);
-void launch_pair(const Dev &d, const Geo &g, int precision, int grid,
+void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t,
                  hipStream_t st);
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
-                      int bin_count, int bin_maxn, int precision,
+                      int bin_count, int bin_maxn, int precision, const Tune &t,
                       hipStream_t st);
 // register-resident fast-path integrator; false when the bin is not one it
 // serves (FP64 precision or more than 64 sites), nothing launched then
 bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,
-                          int bin_count, int bin_maxn, int precision,
+                          int bin_count, int bin_maxn, int precision, int waves,
                           hipStream_t st);
+// the same for 65..256 sites (sc_integrate_long.hip)
+bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list,
+                           int bin_count, int bin_maxn, int precision, int waves,
+                           hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
 // analyses (sc_analysis.hip)

@@ -113,11 +113,23 @@ __device__ __forceinline__ bool sphero32(const float *dr, const float *u1, float
 
 }  // namespace
 
+// Block index -> bond range.  Workgroups are dispatched to the 8 XCDs
+// round-robin; with `xcd` set, the blocks one XCD receives cover one
+// contiguous range of bonds, so with spatially ordered storage a bond's
+// partners tend to be in the same XCD's L2.
+__device__ __forceinline__ int64_t logical_block(int xcd) {
+  const int64_t b = blockIdx.x;
+  if (!xcd) return b;
+  const int64_t n = gridDim.x, q = n / 8, r = n % 8;
+  const int64_t x = b % 8, k = b / 8;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_pair_fused(Dev d, Geo g) {
+__global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   if (d.ctl->abort) return;
   const int64_t nb = g.nb;
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = logical_block(xcd) * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const BondRec own = load_rec(d.br, b);
   double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
@@ -235,14 +247,14 @@ __global__ __launch_bounds__(256) void k_pair_fused(Dev d, Geo g) {
   }
 }
 
-void launch_pair(const Dev &d, const Geo &g, int precision, int /*grid*/,
-                 hipStream_t st) {
-  const int64_t blocks = (g.nb + 255) / 256;
-  const dim3 grid((unsigned)(blocks > 0 ? blocks : 1)), block(256);
+void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t, hipStream_t st) {
+  const int bs = t.pair_block;
+  const int64_t blocks = (g.nb + bs - 1) / bs;
+  const dim3 grid((unsigned)(blocks > 0 ? blocks : 1)), block(bs);
   if (precision == SC_PREC_FP64)
-    hipLaunchKernelGGL(k_pair_fused<false>, grid, block, 0, st, d, g);
+    hipLaunchKernelGGL(k_pair_fused<false>, grid, block, 0, st, d, g, t.pair_xcd);
   else
-    hipLaunchKernelGGL(k_pair_fused<true>, grid, block, 0, st, d, g);
+    hipLaunchKernelGGL(k_pair_fused<true>, grid, block, 0, st, d, g, t.pair_xcd);
 }
 
 }  // namespace sc

@@ -14,8 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import (SC_ERR_UNPHYSICAL, SC_OK, SC_PREC_FP64, SC_PREC_MIXED,
-                   ScFilamentParams, ScParams, SimcoreError, UnphysicalResult,
-                   lib)
+                   ScFilamentParams, ScParams, ScTuneReport, SimcoreError,
+                   UnphysicalResult, lib)
 
 
 def _dp(a):
@@ -226,6 +226,12 @@ class FilamentSystem:
                self.h, "sc_get_counters")
         return {"i_step": i_step.value, "n_rebuilds": nreb.value,
                 "n_candidates": nc.value, "n_cells_1d": n1d.value}
+
+    def autotune(self, steps=20):
+        """Pick the fastest result-invariant launch shapes (sc_autotune)."""
+        rep = ScTuneReport()
+        _check(lib.sc_autotune(self.h, int(steps), byref(rep)), self.h, "sc_autotune")
+        return {k: getattr(rep, k) for k, _ in ScTuneReport._fields_}
 
     def checkpoint(self):
         """Complete dynamic state as bytes (sc_checkpoint_save)."""

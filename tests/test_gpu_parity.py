@@ -299,3 +299,22 @@ def test_checkpoint_resume_is_bit_identical(precision, spatial):
         ora = OracleSystem(p, fil)
         assert ora.step(71) == 0
         np.testing.assert_array_equal(b.sites()[0], ora.sites()[0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_leaves_trajectory_unchanged(precision):
+    print(f"case: C0 {precision} autotune after 31 steps, then 40 steps", flush=True)
+    from tests.helpers import uneven_system
+    p, fil = uneven_system()
+    a = sc.FilamentSystem(p, fil, precision=precision, use_graph=True)
+    b = sc.FilamentSystem(p, fil, precision=precision, use_graph=True)
+    a.step(31)
+    b.step(31)
+    rep = b.autotune(steps=6)
+    print(rep)
+    assert rep["n_trials"] == 18 and rep["step_ms_best"] <= rep["step_ms_start"]
+    assert b.counters()["i_step"] == 31
+    a.step(40)
+    b.step(40)
+    for x, y in zip(a.sites(), b.sites()):
+        np.testing.assert_array_equal(x, y)
