@@ -209,6 +209,50 @@ int sc_copy_probe(size_t bytes, int repeats, double *avg_ms);
 const char *sc_last_error(sc_handle h);
 int sc_destroy(sc_handle h);
 
+/* ---- crosslinks -------------------------------------------------------------- */
+/* Harmonic springs between points on two bonds.  Force law, anchor geometry
+ * and application: Crosslink::CalculateTetherForces (src/crosslink.cpp:
+ * 246-273), MinimumDistance::PointPoint (src/minimum_distance.cpp:71-101),
+ * Anchor::UpdateAnchorPositionToBond / ApplyAnchorForces (src/anchor.cpp:
+ * 132-146, 236-248).  Schedule of CrosslinkSpecies::UpdatePositions
+ * (src/crosslink_species.cpp:218-238): even steps recompute the forces from
+ * the current bonds, odd steps re-apply the stored ones; both are added to
+ * the bonds before the pair forces, in crosslink order (anchor 0 takes +F,
+ * anchor 1 -F).  Anchors do not move along their bonds.  Binding kinetics
+ * are this library's model, not the reference's KMC (absent): on even
+ * step k each crosslink releases, with probability 1 - exp(-k_off dt), the
+ * anchor a draw picks and rebinds it, at a uniform fraction of the nominal
+ * bond length, to the bond within r_bind (point-segment distance) of the
+ * other anchor that has the smallest Philox score; its stored force is
+ * zeroed.  Counter-based draws keep this independent of storage order. */
+typedef struct sc_crosslink_params {
+  double k_spring;     /* spring constant (kT / sigma^2)                  */
+  double rest_length;  /* sigma                                           */
+  double k_compress;   /* constant when compressed, < 0: k_spring         */
+  double k_off;        /* unbinding rate per unit time (0: static)        */
+  double r_bind;       /* rebinding range (sigma)                         */
+  uint64_t seed;       /* key of the binding draws                        */
+} sc_crosslink_params;
+
+/* bonds: 2*n bond indices (anchor 0, anchor 1 per crosslink, caller bond
+ * numbering), lambda: 2*n distances from each bond's first site.  Call
+ * after sc_load_filaments and before the first step. */
+int sc_load_crosslinks(sc_handle h, int64_t n, const int32_t *bonds,
+                       const double *lambda, const sc_crosslink_params *p);
+/* Current bonds / lambdas (2*n each) and stored forces (9*n: F, torque on
+ * anchor 0's bond, torque on anchor 1's bond); any pointer may be NULL.
+ * *events receives the number of rebinding events so far. */
+int sc_get_crosslinks(sc_handle h, int32_t *bonds, double *lambda, double *force,
+                      int64_t *events);
+/* Host generator: n crosslinks between a uniformly chosen bond (uniform
+ * fraction along it) and a uniformly chosen other bond within r_bind of
+ * that anchor point (uniform fraction). */
+int sc_generate_crosslinks(const sc_params *params, int32_t n_filaments,
+                           const int32_t *site_off, const double *r,
+                           const sc_filament_params *fil, int64_t n,
+                           double r_bind, uint64_t seed, int32_t *bonds,
+                           double *lambda);
+
 /* ---- ensemble runner ------------------------------------------------------ */
 /* Serves independent systems of different sizes and seeds (the reference's
  * n_runs use case, src/simulation_manager.cpp:461-545, which runs them one

@@ -73,6 +73,14 @@ struct orc_sys {
   /* per-thread scratch */
   int n_thr;
   double **scratch;
+  /* crosslinks */
+  int64_t nx, xl_events;
+  int32_t *xb;     /* 2 per crosslink */
+  double *xlam;    /* 2 per crosslink */
+  double *xf;      /* 9 per crosslink: F, T0, T1 */
+  orc_xlink_params xp;
+  double xpoff;
+  int xreach;
 };
 
 /* ---------------------------------------------------------------------------
@@ -604,6 +612,9 @@ void orc_destroy(orc_sys *s) {
   free(s->c1);
   free(s->c2);
   free(s->no_ix);
+  free(s->xb);
+  free(s->xlam);
+  free(s->xf);
   for (int t = 0; t < s->n_thr; ++t) free(s->scratch[t]);
   free(s->scratch);
   free(s);
@@ -911,6 +922,191 @@ static void update_dr_tot(orc_sys *s, int f) {
   }
 }
 
+/* ---------------------------------------------------------------------------
+ * Crosslinks
+ * ------------------------------------------------------------------------- */
+/* Anchor::UpdateAnchorPositionToBond (src/anchor.cpp:236-248) with the bond
+ * state of Bond::ReInit: position, orientation and current length. */
+static void anchor_point(const orc_sys *s, int32_t b, double lam, double *P, double *u,
+                         double *len) {
+  for (int i = 0; i < 3; ++i) u[i] = s->u[i][b];
+  *len = s->len[b];
+  for (int i = 0; i < 3; ++i) P[i] = s->pos[i][b] - (0.5 * *len - lam) * u[i];
+}
+
+static void scaled(const double *h_inv, const double *P, double *sp) {
+  for (int i = 0; i < 3; ++i) {
+    double si = 0.0;
+    for (int j = 0; j < 3; ++j) si += h_inv[3 * i + j] * P[j];
+    si -= O_NINT(si);
+    sp[i] = si;
+  }
+}
+
+double orc_xlink_force(const double *h, const double *h_inv, const double *p0,
+                       const double *p1, double k, double r0, double kc, double *out) {
+  double s0[3], s1[3], ds[3], dr[3], dr2 = 0.0;
+  scaled(h_inv, p0, s0);
+  scaled(h_inv, p1, s1);
+  /* MinimumDistance::PointPoint (src/minimum_distance.cpp:71-101) */
+  for (int i = 0; i < 3; ++i) {
+    ds[i] = s1[i] - s0[i];
+    ds[i] -= O_NINT(ds[i]);
+  }
+  for (int i = 0; i < 3; ++i) {
+    dr[i] = 0.0;
+    for (int j = 0; j < 3; ++j) dr[i] += h[3 * i + j] * ds[j];
+  }
+  for (int i = 0; i < 3; ++i) dr2 += dr[i] * dr[i];
+  /* Crosslink::CalculateTetherForces (src/crosslink.cpp:246-273) */
+  const double length = sqrt(dr2);
+  const double stretch = length - r0;
+  double tf;
+  if (kc >= 0 && stretch < 0)
+    tf = kc * stretch;
+  else
+    tf = k * stretch;
+  for (int i = 0; i < 3; ++i) out[i] = tf * (dr[i] / length);
+  return length;
+}
+
+/* even steps: forces from the current bonds, stored per crosslink */
+static void xlink_forces(orc_sys *s) {
+  for (int64_t x = 0; x < s->nx; ++x) {
+    double P[2][3], u[2][3], len[2];
+    for (int a = 0; a < 2; ++a)
+      anchor_point(s, s->xb[2 * x + a], s->xlam[2 * x + a], P[a], u[a], &len[a]);
+    double F[3];
+    orc_xlink_force(s->p.h, s->p.h_inv, P[0], P[1], s->xp.k_spring, s->xp.rest_length,
+                    s->xp.k_compress, F);
+    double *o = s->xf + 9 * x;
+    for (int i = 0; i < 3; ++i) o[i] = F[i];
+    /* Anchor::ApplyAnchorForces (src/anchor.cpp:132-146): lever
+     * (lambda - l/2) u; anchor 1 carries -F */
+    for (int a = 0; a < 2; ++a) {
+      double dl[3], Fa[3], T[3];
+      for (int i = 0; i < 3; ++i) {
+        dl[i] = (s->xlam[2 * x + a] - 0.5 * len[a]) * u[a][i];
+        Fa[i] = a == 0 ? F[i] : -F[i];
+      }
+      cross3(dl, Fa, T);
+      for (int i = 0; i < 3; ++i) o[3 + 3 * a + i] = T[i];
+    }
+  }
+}
+
+/* CrosslinkSpecies::ApplyCrosslinkTetherForces: serial, crosslink order */
+static void xlink_apply(orc_sys *s) {
+  for (int64_t x = 0; x < s->nx; ++x) {
+    const double *o = s->xf + 9 * x;
+    for (int a = 0; a < 2; ++a) {
+      const int32_t b = s->xb[2 * x + a];
+      for (int i = 0; i < 3; ++i) s->bf[i][b] += a == 0 ? o[i] : -o[i];
+      for (int i = 0; i < 3; ++i) s->bt[i][b] += o[3 + 3 * a + i];
+    }
+  }
+}
+
+static double unit_draw(uint32_t v) { return ((double)v + 0.5) * 2.3283064365386962890625e-10; }
+
+/* even steps after the forces: unbinding / rebinding */
+static void xlink_events(orc_sys *s) {
+  const int64_t m = s->i_step / 2;
+  const uint32_t key[2] = {(uint32_t)s->xp.seed, (uint32_t)(s->xp.seed >> 32)};
+  const int n = s->n1d, R = s->xreach;
+  const double rb2 = s->xp.r_bind * s->xp.r_bind;
+  for (int64_t x = 0; x < s->nx; ++x) {
+    uint32_t ctr[4] = {(uint32_t)x, (uint32_t)((uint64_t)x >> 32), (uint32_t)m, 1u}, o[4];
+    orc_philox4x32(ctr, key, o);
+    if (!(unit_draw(o[0]) < s->xpoff)) continue;
+    const int am = unit_draw(o[1]) < 0.5 ? 0 : 1, as = 1 - am;
+    const int32_t bs = s->xb[2 * x + as];
+    double P[3], u[3], len, sP[3];
+    anchor_point(s, bs, s->xlam[2 * x + as], P, u, &len);
+    scaled(s->p.h_inv, P, sP);
+    const int32_t key0 = orc_cell_key(n, sP);
+    const int cx = key0 / (n * n), cy = (key0 / n) % n, cz = key0 % n;
+    const int span = 2 * R + 1 >= n ? n : 2 * R + 1;
+    const int off0 = 2 * R + 1 >= n ? 0 : -R;
+    int32_t best = -1;
+    uint32_t best_score = 0;
+    for (int dx = 0; dx < span; ++dx)
+      for (int dy = 0; dy < span; ++dy)
+        for (int dz = 0; dz < span; ++dz) {
+          const int X = ((cx + off0 + dx) % n + n) % n, Y = ((cy + off0 + dy) % n + n) % n;
+          const int Z = ((cz + off0 + dz) % n + n) % n;
+          const int32_t c = ((int32_t)X * n + Y) * n + Z;
+          for (int32_t j = s->cell_start[c]; j < s->cell_start[c + 1]; ++j) {
+            const int32_t b = s->cell_obj[j];
+            if (b == bs) continue;
+            double ds[3], dr[3], t = 0.0, d2 = 0.0;
+            for (int i = 0; i < 3; ++i) {
+              ds[i] = s->sp[i][b] - sP[i];
+              ds[i] -= O_NINT(ds[i]);
+            }
+            for (int i = 0; i < 3; ++i) {
+              dr[i] = 0.0;
+              for (int k = 0; k < 3; ++k) dr[i] += s->p.h[3 * i + k] * ds[k];
+            }
+            for (int i = 0; i < 3; ++i) t += dr[i] * s->u[i][b];
+            const double hl = 0.5 * s->len[b];
+            if (t > hl) t = hl;
+            if (t < -hl) t = -hl;
+            for (int i = 0; i < 3; ++i) {
+              const double e = dr[i] - t * s->u[i][b];
+              d2 += e * e;
+            }
+            if (!(d2 <= rb2)) continue;
+            uint32_t c2[4] = {(uint32_t)x, (uint32_t)m, (uint32_t)b, 2u}, q[4];
+            orc_philox4x32(c2, key, q);
+            if (best < 0 || q[0] < best_score || (q[0] == best_score && b < best)) {
+              best = b;
+              best_score = q[0];
+            }
+          }
+        }
+    if (best < 0) continue;
+    s->xb[2 * x + am] = best;
+    s->xlam[2 * x + am] = unit_draw(o[2]) * s->bl[s->bond_fil[best]];
+    for (int i = 0; i < 9; ++i) s->xf[9 * x + i] = 0.0;
+    s->xl_events++;
+  }
+}
+
+int orc_load_crosslinks(orc_sys *s, int64_t nx, const int32_t *bonds, const double *lambda,
+                        const orc_xlink_params *p) {
+  if (!s || nx < 0 || (nx > 0 && (!bonds || !lambda)) || !p) return -1;
+  for (int64_t i = 0; i < 2 * nx; ++i)
+    if (bonds[i] < 0 || bonds[i] >= s->nb) return -1;
+  free(s->xb);
+  free(s->xlam);
+  free(s->xf);
+  s->nx = nx;
+  s->xb = xcalloc((size_t)(2 * nx), sizeof(int32_t));
+  s->xlam = xcalloc((size_t)(2 * nx), sizeof(double));
+  s->xf = xcalloc((size_t)(9 * nx), sizeof(double));
+  memcpy(s->xb, bonds, sizeof(int32_t) * (size_t)(2 * nx));
+  memcpy(s->xlam, lambda, sizeof(double) * (size_t)(2 * nx));
+  s->xp = *p;
+  s->xpoff = 1.0 - exp(-p->k_off * s->p.delta);
+  double lmax = 0.0;
+  for (int f = 0; f < s->nf; ++f)
+    if (s->bl[f] > lmax) lmax = s->bl[f];
+  /* cells to search: the range plus half a bond, plus one cell for
+   * assignments made at the last rebuild */
+  s->xreach = (int)ceil((p->r_bind + 0.5 * lmax) / s->cell_len) + 1;
+  s->xl_events = 0;
+  return 0;
+}
+
+void orc_get_crosslinks(const orc_sys *s, int32_t *bonds, double *lambda, double *force) {
+  if (bonds) memcpy(bonds, s->xb, sizeof(int32_t) * (size_t)(2 * s->nx));
+  if (lambda) memcpy(lambda, s->xlam, sizeof(double) * (size_t)(2 * s->nx));
+  if (force) memcpy(force, s->xf, sizeof(double) * (size_t)(9 * s->nx));
+}
+
+int64_t orc_xlink_events(const orc_sys *s) { return s->xl_events; }
+
 int orc_step(orc_sys *s, int nsteps) {
   for (int step = 0; step < nsteps; ++step) {
     s->i_step++;
@@ -919,6 +1115,13 @@ int orc_step(orc_sys *s, int nsteps) {
     for (int d = 0; d < 3; ++d) {
       memset(s->bf[d], 0, sizeof(double) * (size_t)s->nb);
       memset(s->bt[d], 0, sizeof(double) * (size_t)s->nb);
+    }
+    /* InteractionManager::Interact -> CrosslinkManager::UpdateCrosslinks
+     * (src/interaction_manager.cpp:70, src/crosslink_species.cpp:218-238) */
+    if (s->nx > 0) {
+      if (s->i_step % 2 == 0) xlink_forces(s);
+      xlink_apply(s);
+      if (s->i_step % 2 == 0 && s->xpoff > 0) xlink_events(s);
     }
     /* CheckUpdateInteractions -> GetDrMax (src/interaction_manager.cpp:
      * 284-312, include/simcore/species.hpp:105-120) */

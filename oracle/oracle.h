@@ -94,6 +94,44 @@ void orc_get_tensions(const orc_sys *s, double *tension);   /* per bond  */
 void orc_get_dr(const orc_sys *s, double *acc, double *dr_max_per_fil);
 void orc_set_sites(orc_sys *s, const double *r);  /* for forced-error tests */
 
+/* ---- crosslinks (harmonic springs between two bonds) ----------------------- *
+ * Force law and application: Crosslink::CalculateTetherForces
+ * (src/crosslink.cpp:246-273), MinimumDistance::PointPoint
+ * (src/minimum_distance.cpp:71-101), Anchor::UpdateAnchorPositionToBond and
+ * ApplyAnchorForces (src/anchor.cpp:113-146, 236-248); schedule of
+ * CrosslinkSpecies::UpdatePositions (src/crosslink_species.cpp:218-238):
+ * even steps recompute the forces from the current bonds and apply them,
+ * odd steps re-apply the stored forces, both before the pair pass and in
+ * crosslink order.  Anchors do not move along their bonds.  Binding
+ * kinetics are this library's model (the reference's KMC is not
+ * available): on even step k each crosslink, with probability
+ * p_off = 1 - exp(-k_off dt), releases one anchor (chosen by a draw) and
+ * rebinds it to the in-range bond (point-segment distance <= r_bind from
+ * the staying anchor, staying bond excluded) with the smallest Philox score
+ * at a uniform fraction of its nominal length; its stored force is zeroed.
+ * Draws: Philox4x32-10, key = seed, counter = (crosslink, k/2, 0, 1) for the
+ * event and (crosslink, k/2, bond, 2) for the scores. */
+typedef struct orc_xlink_params {
+  double k_spring, rest_length, k_compress; /* k_compress < 0: none      */
+  double k_off;                             /* unbinding rate per time   */
+  double r_bind;                            /* rebinding range           */
+  uint64_t seed;
+} orc_xlink_params;
+/* bonds: 2*nx (anchor 0, anchor 1 per crosslink), lambda: 2*nx distances
+ * from the bond's first site.  Stored forces start at zero. */
+int orc_load_crosslinks(orc_sys *s, int64_t nx, const int32_t *bonds,
+                        const double *lambda, const orc_xlink_params *p);
+/* bonds/lambda as above; force: 9*nx (F, torque on anchor 0's bond, torque
+ * on anchor 1's bond) stored from the last even step; any may be NULL. */
+void orc_get_crosslinks(const orc_sys *s, int32_t *bonds, double *lambda,
+                        double *force);
+int64_t orc_xlink_events(const orc_sys *s);
+/* force of one spring between two anchor points (for pinning tests):
+ * out = F[3] on anchor 0; returns the spring length. */
+double orc_xlink_force(const double *h, const double *h_inv, const double *p0,
+                       const double *p1, double k, double r0, double kc,
+                       double *out);
+
 /* ---- unit-level restatements (for pinning tests) --------------------------- */
 /* MinimumDistance::Sphero.  out: r_min[3], r_min_mag2, contact1[3],
  * contact2[3] (10 doubles). */

@@ -22,6 +22,11 @@ class OrcParams(ctypes.Structure):
     ]
 
 
+class OrcXlinkParams(ctypes.Structure):
+    _fields_ = [("k_spring", c_double), ("rest_length", c_double), ("k_compress", c_double),
+                ("k_off", c_double), ("r_bind", c_double), ("seed", c_uint64)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
@@ -54,6 +59,10 @@ def _load():
         "orc_philox4x32": ([POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)], None),
         "orc_noise": ([c_uint64, c_int64, c_int64, dp], None),
         "orc_bending_forces": ([c_int, dp, c_double, c_double, dp, dp], None),
+        "orc_load_crosslinks": ([V, c_int64, ip, dp, POINTER(OrcXlinkParams)], c_int),
+        "orc_get_crosslinks": ([V, ip, dp, dp], None),
+        "orc_xlink_events": ([V], c_int64),
+        "orc_xlink_force": ([dp, dp, dp, dp, c_double, c_double, c_double, dp], c_double),
     }
     for n, (a, r) in sig.items():
         f = getattr(L, n)
@@ -170,6 +179,27 @@ class OracleSystem:
         r = np.ascontiguousarray(r, dtype=np.float64)
         lib.orc_set_sites(self.s, _dp(r))
 
+    def load_crosslinks(self, bonds, lam, k_spring=10.0, rest_length=1.0, k_compress=-1.0,
+                        k_off=0.1, r_bind=2.0, seed=1):
+        bonds = np.ascontiguousarray(bonds, dtype=np.int32).reshape(-1)
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)
+        xp = OrcXlinkParams(k_spring, rest_length, k_compress, k_off, r_bind, seed)
+        rc = lib.orc_load_crosslinks(self.s, len(bonds) // 2, _ip(bonds), _dp(lam), byref(xp))
+        if rc != 0:
+            raise ValueError("orc_load_crosslinks rejected the input")
+        self.nx = len(bonds) // 2
+
+    def crosslinks(self):
+        """(bonds (nx,2), lambda (nx,2), stored forces (nx,9))."""
+        b = np.empty(2 * self.nx, dtype=np.int32)
+        lam = np.empty(2 * self.nx)
+        f = np.empty(9 * self.nx)
+        lib.orc_get_crosslinks(self.s, _ip(b), _dp(lam), _dp(f))
+        return b.reshape(-1, 2), lam.reshape(-1, 2), f.reshape(-1, 9)
+
+    def xlink_events(self):
+        return lib.orc_xlink_events(self.s)
+
     def counters(self):
         return {"i_step": lib.orc_step_index(self.s),
                 "n_rebuilds": lib.orc_rebuild_count(self.s),
@@ -231,3 +261,14 @@ def bending_forces(u_soa, kappa, bl):
     u = np.ascontiguousarray(u_soa, dtype=np.float64)
     lib.orc_bending_forces(n, _dp(u), kappa, bl, _dp(f), _dp(ke))
     return f, ke[:n - 2]
+
+
+def xlink_force(h, h_inv, p0, p1, k, r0, kc=-1.0):
+    """(force on anchor 0, spring length) of one crosslink spring."""
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    hi = np.ascontiguousarray(h_inv, dtype=np.float64)
+    p0 = np.ascontiguousarray(p0, dtype=np.float64)
+    p1 = np.ascontiguousarray(p1, dtype=np.float64)
+    out = np.empty(3)
+    length = lib.orc_xlink_force(_dp(h), _dp(hi), _dp(p0), _dp(p1), k, r0, kc, _dp(out))
+    return out, length

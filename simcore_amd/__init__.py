@@ -6,12 +6,12 @@ ImportError if it has not been built; there is no CPU fallback.
 """
 from ._lib import (SC_PREC_FP64, SC_PREC_MIXED, EXPORTS, LIB_PATH, ScFilamentParams,
                    ScParams, SimcoreError, UnphysicalResult, lib)
-from .system import (FilamentSystem, Filaments, filament_layout, generate_filaments,
-                     params_from_yaml, params_init)
+from .system import (FilamentSystem, Filaments, crosslink_params, filament_layout,
+                     generate_crosslinks, generate_filaments, params_from_yaml, params_init)
 from .ensemble import Ensemble
 
 __all__ = [
-    "FilamentSystem", "Filaments", "Ensemble", "generate_filaments", "params_init",
+    "FilamentSystem", "Filaments", "Ensemble", "generate_crosslinks", "crosslink_params", "generate_filaments", "params_init",
     "params_from_yaml", "filament_layout", "ScParams", "ScFilamentParams",
     "SimcoreError", "UnphysicalResult", "SC_PREC_FP64", "SC_PREC_MIXED",
     "EXPORTS", "LIB_PATH", "lib",

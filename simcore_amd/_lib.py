@@ -32,6 +32,7 @@ EXPORTS = [
     "sc_generate_filaments", "sc_copy_probe", "sc_step_async", "sc_step_poll",
     "sc_filament_stats", "sc_theta_histogram", "sc_output_header", "sc_spec_frame",
     "sc_posit_frame", "sc_checkpoint_save", "sc_checkpoint_load", "sc_autotune",
+    "sc_load_crosslinks", "sc_get_crosslinks", "sc_generate_crosslinks",
     "sc_ensemble_create", "sc_ensemble_submit", "sc_ensemble_progress",
     "sc_ensemble_drain", "sc_ensemble_request", "sc_ensemble_result",
     "sc_ensemble_stats_get", "sc_ensemble_clock", "sc_ensemble_last_error",
@@ -85,6 +86,11 @@ class ScEnsembleStats(ctypes.Structure):
                 ("arena_bytes", c_size_t)]
 
 
+class ScCrosslinkParams(ctypes.Structure):
+    _fields_ = [("k_spring", c_double), ("rest_length", c_double), ("k_compress", c_double),
+                ("k_off", c_double), ("r_bind", c_double), ("seed", c_uint64)]
+
+
 class ScTuneReport(ctypes.Structure):
     _fields_ = [("pair_block", c_int32), ("pair_xcd", c_int32),
                 ("integrate_waves", c_int32), ("n_trials", c_int32),
@@ -132,6 +138,10 @@ def _load():
         "sc_checkpoint_save": ([H, c_void_p, c_size_t, POINTER(c_size_t)], c_int),
         "sc_checkpoint_load": ([H, c_void_p, c_size_t], c_int),
         "sc_autotune": ([H, c_int32, POINTER(ScTuneReport)], c_int),
+        "sc_load_crosslinks": ([H, c_int64, ip, dp, POINTER(ScCrosslinkParams)], c_int),
+        "sc_get_crosslinks": ([H, ip, dp, dp, lp], c_int),
+        "sc_generate_crosslinks": ([POINTER(ScParams), c_int32, ip, dp, POINTER(ScFilamentParams),
+                                    c_int64, c_double, c_uint64, ip, dp], c_int),
         "sc_theta_histogram": ([H, c_int32, c_int32, lp], c_int),
         "sc_device_bytes": ([H, POINTER(c_size_t)], c_int),
         "sc_profile_enable": ([H, c_int], c_int),

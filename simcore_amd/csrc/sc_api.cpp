@@ -195,7 +195,12 @@ void enqueue_step(sc_system *h, bool odd) {
     (void)hipEventRecord(h->ev[7], st);
     (void)hipEventRecord(h->ev[0], st);
   }
+  if (!odd) sc::launch_xlink_forces(d, g, st);
   sc::launch_pair(d, g, h->p.precision, h->tune, st);
+  if (!odd) {
+    sc::launch_xlink_events(d, g, st);
+    if (g.xpoff > 0) sc::launch_xlink_lists(d, g, st);
+  }
   if (pf) {
     (void)hipEventRecord(h->ev[1], st);
     (void)hipEventRecord(h->ev[2], st);
@@ -605,6 +610,96 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   return SC_OK;
 }
 
+int sc_load_crosslinks(sc_handle h, int64_t n, const int32_t *bonds, const double *lambda,
+                       const sc_crosslink_params *p) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!p || n < 0 || (n > 0 && (!bonds || !lambda))) return fail(h, SC_ERR_ARG, "null input");
+  if (h->g.nx > 0) return fail(h, SC_ERR_STATE, "crosslinks already loaded");
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
+  if (n == 0) return SC_OK;
+  const int64_t nb = h->g.nb;
+  if (n > ((int64_t)1 << 30)) return fail(h, SC_ERR_ARG, "too many crosslinks");
+  // caller bond numbering -> storage
+  std::vector<int32_t> to_storage(nb), caller(nb);
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t c = h->reordered ? h->bond_map[b] : b;
+    caller[b] = (int32_t)c;
+    to_storage[c] = (int32_t)b;
+  }
+  std::vector<int32_t> xb(2 * n);
+  for (int64_t i = 0; i < 2 * n; ++i) {
+    if (bonds[i] < 0 || bonds[i] >= nb) return fail(h, SC_ERR_ARG, "crosslink bond out of range");
+    if (!(lambda[i] >= 0)) return fail(h, SC_ERR_ARG, "negative anchor position");
+    xb[i] = to_storage[bonds[i]];
+  }
+  for (int64_t x = 0; x < n; ++x)
+    if (xb[2 * x] == xb[2 * x + 1]) return fail(h, SC_ERR_ARG, "crosslink joins a bond to itself");
+  Dev &d = h->d;
+  Geo &g = h->g;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if ((rc = dalloc(h, &d.xb, 2 * n, false))) return rc;
+  if ((rc = dalloc(h, &d.xlam, 2 * n, false))) return rc;
+  if ((rc = dalloc(h, &d.xf, 9 * n, false))) return rc;
+  if ((rc = dalloc(h, &d.xinc, 2 * n, false))) return rc;
+  if ((rc = dalloc(h, &d.xinc_start, nb + 1, false))) return rc;
+  if ((rc = dalloc(h, &d.xdeg, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.bond_caller, nb, false))) return rc;
+  HIPCHK(h, hipMemcpyAsync(d.xb, xb.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.xlam, lambda, sizeof(double) * 2 * n, hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.bond_caller, caller.data(), sizeof(int32_t) * nb,
+                           hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.xf, 0, sizeof(double) * 9 * n, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.xdeg, 0, sizeof(int32_t) * nb, h->stream));
+  g.nx = n;
+  g.xk = p->k_spring;
+  g.xr0 = p->rest_length;
+  g.xkc = p->k_compress;
+  g.xpoff = p->k_off > 0 ? 1.0 - std::exp(-p->k_off * h->p.delta) : 0.0;
+  g.xrb2 = p->r_bind * p->r_bind;
+  g.xseed = p->seed;
+  double lmax = 0.0;
+  for (int f = 0; f < g.nf; ++f) {
+    double bl = 0;
+    HIPCHK(h, hipMemcpy(&bl, d.fpar + 8 * f, sizeof(double), hipMemcpyDeviceToHost));
+    lmax = std::max(lmax, bl);
+  }
+  const double cell_len = 2.0 * h->p.system_radius / g.n1d;
+  g.xreach = (int)std::ceil((p->r_bind + 0.5 * lmax) / cell_len) + 1;
+  // build the per-bond lists once
+  Ctl c = read_ctl(h);
+  c.xl_dirty = 1;
+  if ((rc = write_ctl(h, c))) return rc;
+  sc::launch_xlink_lists(d, g, h->stream);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipGetLastError());
+  *h->ctl_host = read_ctl(h);
+  drop_graph(h);  // kernels take the system constants by value
+  return SC_OK;
+}
+
+int sc_get_crosslinks(sc_handle h, int32_t *bonds, double *lambda, double *force,
+                      int64_t *events) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const int64_t n = h->g.nx;
+  if (events) *events = read_ctl(h).xl_events;
+  if (n == 0) return SC_OK;
+  if (bonds) {
+    std::vector<int32_t> xb(2 * n);
+    HIPCHK(h, hipMemcpy(xb.data(), h->d.xb, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < 2 * n; ++i)
+      bonds[i] = (int32_t)(h->reordered ? h->bond_map[xb[i]] : xb[i]);
+  }
+  if (lambda)
+    HIPCHK(h, hipMemcpy(lambda, h->d.xlam, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+  if (force) HIPCHK(h, hipMemcpy(force, h->d.xf, sizeof(double) * 9 * n, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
 int sc_step(sc_handle h, int32_t n_steps) {
   int rc = sc_step_async(h, n_steps);
   if (rc) return rc;
@@ -904,7 +999,7 @@ namespace {
 struct CkHeader {
   char magic[8];
   int32_t version, precision, spatial_order, nf;
-  int64_t ns, nb, nc, n_cells;
+  int64_t ns, nb, nc, n_cells, nx;
   Ctl ctl;
 };
 
@@ -928,7 +1023,10 @@ std::vector<CkField> ck_fields(sc_system *h, int64_t nc) {
           {d.bt, 24 * (size_t)nb},       {d.tension, 8 * (size_t)nb},
           {d.keys, 4 * (size_t)nb},      {d.norm, 8 * (size_t)nf},
           {d.pairs, 8 * (size_t)nc},     {slots, slot * (size_t)nc},
-          {d.inc_start, 4 * (size_t)(nb + 1)}, {d.inc, 16 * (size_t)nc}};
+          {d.inc_start, 4 * (size_t)(nb + 1)}, {d.inc, 16 * (size_t)nc},
+          {d.xb, 8 * (size_t)h->g.nx}, {d.xlam, 16 * (size_t)h->g.nx},
+          {d.xf, 72 * (size_t)h->g.nx}, {d.xinc, 8 * (size_t)h->g.nx},
+          {d.xinc_start, h->g.nx ? 4 * (size_t)(nb + 1) : 0}};
 }
 
 size_t ck_size(sc_system *h, int64_t nc) {
@@ -962,6 +1060,7 @@ int sc_checkpoint_save(sc_handle h, void *buf, size_t cap, size_t *size) {
   hd.nb = h->g.nb;
   hd.nc = c.nc;
   hd.n_cells = h->g.n_cells;
+  hd.nx = h->g.nx;
   hd.ctl = c;
   std::memcpy(p, &hd, sizeof(hd));
   p += sizeof(hd);
@@ -989,7 +1088,8 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size) {
   if (std::memcmp(hd.magic, kCkMagic, 8) != 0 || hd.version != 1)
     return fail(h, SC_ERR_ARG, "not a checkpoint of this library");
   if (hd.precision != h->p.precision || hd.spatial_order != (h->reordered ? 1 : 0) ||
-      hd.nf != h->g.nf || hd.ns != h->g.ns || hd.nb != h->g.nb || hd.n_cells != h->g.n_cells)
+      hd.nf != h->g.nf || hd.ns != h->g.ns || hd.nb != h->g.nb || hd.n_cells != h->g.n_cells ||
+      hd.nx != h->g.nx)
     return fail(h, SC_ERR_ARG, "checkpoint of a different system layout");
   if (size != ck_size(h, hd.nc)) return fail(h, SC_ERR_ARG, "checkpoint size mismatch");
   const char *p = static_cast<const char *>(buf) + sizeof(hd);

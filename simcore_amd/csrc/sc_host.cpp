@@ -715,6 +715,107 @@ int sc_filament_layout(double length, double min_bond_length, double min_length,
   return SC_OK;
 }
 
+int sc_generate_crosslinks(const sc_params *params, int32_t nf, const int32_t *site_off,
+                           const double *r, const sc_filament_params *fil, int64_t nx,
+                           double r_bind, uint64_t seed, int32_t *bonds, double *lambda) {
+  if (!params || nf <= 0 || !site_off || !r || !fil || nx < 0 || !(r_bind > 0))
+    return SC_ERR_ARG;
+  if (nx > 0 && (!bonds || !lambda)) return SC_ERR_ARG;
+  const double L = 2.0 * params->system_radius;
+  const int64_t ns = site_off[nf], nb = ns - nf;
+  if (nb <= 1) return SC_ERR_ARG;
+  // bond centres (wrapped into the box), orientations, half lengths
+  std::vector<double> c(3 * nb), u(3 * nb), hl(nb);
+  std::vector<int32_t> bf(nb);
+  double lmax = 0;
+  for (int f = 0; f < nf; ++f)
+    for (int64_t i = site_off[f]; i < site_off[f + 1] - 1; ++i) {
+      const int64_t b = i - f;
+      double o[3], l = 0;
+      for (int k = 0; k < 3; ++k) {
+        o[k] = r[k * ns + i + 1] - r[k * ns + i];
+        l += o[k] * o[k];
+      }
+      l = std::sqrt(l);
+      for (int k = 0; k < 3; ++k) {
+        double x = r[k * ns + i] + 0.5 * o[k];
+        x -= L * std::floor(x / L + 0.5);
+        c[3 * b + k] = x;
+        u[3 * b + k] = o[k] / l;
+      }
+      hl[b] = 0.5 * l;
+      bf[b] = f;
+      lmax = std::max(lmax, l);
+    }
+  // grid of bond centres, cells no smaller than the reach
+  const double reach = r_bind + 0.5 * lmax;
+  int n = (int)std::floor(L / reach);
+  if (n < 1) n = 1;
+  if ((int64_t)n * n * n > ((int64_t)1 << 24)) n = 256;
+  std::vector<int32_t> head((size_t)n * n * n, -1), next(nb, -1);
+  auto idx = [&](double x) {
+    int i = (int)std::floor((x / L + 0.5) * n);
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+  };
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t k = ((int64_t)idx(c[3 * b]) * n + idx(c[3 * b + 1])) * n + idx(c[3 * b + 2]);
+    next[b] = head[k];
+    head[k] = (int32_t)b;
+  }
+  sc_host::Rng rng(seed);
+  std::vector<int32_t> cand;
+  for (int64_t x = 0; x < nx; ++x) {
+    bool placed = false;
+    for (int attempt = 0; attempt < 1000 && !placed; ++attempt) {
+      const int32_t b1 = (int32_t)std::min<int64_t>(nb - 1, (int64_t)(rng.uniform_pos() * nb));
+      const double lam1 = rng.uniform_pos() * fil[bf[b1]].bond_length;
+      double P[3];
+      for (int k = 0; k < 3; ++k) P[k] = c[3 * b1 + k] - (hl[b1] - lam1) * u[3 * b1 + k];
+      cand.clear();
+      const int span = n < 3 ? n : 3, off = n < 3 ? 0 : -1;
+      const int cx = idx(P[0] - L * std::floor(P[0] / L + 0.5));
+      const int cy = idx(P[1] - L * std::floor(P[1] / L + 0.5));
+      const int cz = idx(P[2] - L * std::floor(P[2] / L + 0.5));
+      for (int dx = 0; dx < span; ++dx)
+        for (int dy = 0; dy < span; ++dy)
+          for (int dz = 0; dz < span; ++dz) {
+            const int X = ((cx + off + dx) % n + n) % n, Y = ((cy + off + dy) % n + n) % n;
+            const int Z = ((cz + off + dz) % n + n) % n;
+            for (int32_t b = head[((int64_t)X * n + Y) * n + Z]; b >= 0; b = next[b]) {
+              if (b == b1) continue;
+              double d[3], t = 0, d2 = 0;
+              for (int k = 0; k < 3; ++k) {
+                d[k] = c[3 * b + k] - P[k];
+                d[k] -= L * std::round(d[k] / L);
+                t += d[k] * u[3 * b + k];
+              }
+              t = std::max(-hl[b], std::min(hl[b], t));
+              for (int k = 0; k < 3; ++k) {
+                const double e = d[k] - t * u[3 * b + k];
+                d2 += e * e;
+              }
+              if (d2 <= r_bind * r_bind) cand.push_back(b);
+            }
+          }
+      if (cand.empty()) continue;
+      std::sort(cand.begin(), cand.end());
+      const int32_t b2 = cand[std::min<size_t>(cand.size() - 1,
+                                               (size_t)(rng.uniform_pos() * cand.size()))];
+      bonds[2 * x] = b1;
+      bonds[2 * x + 1] = b2;
+      lambda[2 * x] = lam1;
+      lambda[2 * x + 1] = rng.uniform_pos() * fil[bf[b2]].bond_length;
+      placed = true;
+    }
+    if (!placed) {
+      std::fprintf(stderr, "simcore_amd: crosslink %lld found no bond within range\n",
+                   (long long)x);
+      return SC_ERR_ARG;
+    }
+  }
+  return SC_OK;
+}
+
 int sc_generate_filaments(const sc_params *params, int32_t nf, const int32_t *n_sites,
                           const double *length, const double *persistence_length,
                           const double *diameter, double friction_ratio, uint64_t seed,

@@ -36,6 +36,9 @@ struct Ctl {
   int32_t midstep;         // the current step is a midstep
   int32_t force_rebuild;   // host request (load time)
   int32_t overflow;        // candidate capacity exceeded
+  int32_t xl_dirty;        // crosslink bonds changed: rebuild their lists
+  int32_t pad_;
+  int64_t xl_events;       // crosslink rebinding events so far
 };
 
 // Per-system constants passed by value to kernels.
@@ -50,6 +53,12 @@ struct Geo {
   uint64_t seed;
   int32_t nf;
   int64_t ns, nb, n_cells, cap;
+  // crosslinks
+  int64_t nx;
+  double xk, xr0, xkc;     // spring, rest length, compression spring
+  double xpoff, xrb2;      // unbinding probability per full step, range^2
+  uint64_t xseed;
+  int xreach;              // cells searched on each side when rebinding
 };
 
 // Device pointers (a plain struct so it can be passed by value).
@@ -77,6 +86,13 @@ struct Dev {
   int2 *inc;               // per incidence {c << 1 | role, partner bond}
   // scan scratch
   int32_t *scan_tmp;
+  // crosslinks: anchors (2 per crosslink), stored forces (9 per
+  // crosslink: F, torque on anchor 0's bond, on anchor 1's bond), per-bond
+  // lists of {crosslink << 1 | anchor} in crosslink order
+  int32_t *xb;
+  double *xlam, *xf;
+  int32_t *xinc_start, *xinc, *xdeg;
+  int32_t *bond_caller;    // storage bond -> caller bond index
 };
 
 // Launch shapes the autotuner chooses among; none changes a result.
@@ -90,7 +106,7 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
-enum Guard { kGuardStep = 0, kGuardRebuild = 1 };
+enum Guard { kGuardStep = 0, kGuardRebuild = 1, kGuardXlink = 2 };
 
 // Launchers (sc_kernels.hip).  All are asynchronous on `st`.
 void launch_decide(const Dev &d, const Geo &g, hipStream_t st);
@@ -112,6 +128,10 @@ bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list,
                            hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
+// crosslinks (sc_crosslink.hip)
+void launch_xlink_forces(const Dev &d, const Geo &g, hipStream_t st);
+void launch_xlink_events(const Dev &d, const Geo &g, hipStream_t st);
+void launch_xlink_lists(const Dev &d, const Geo &g, hipStream_t st);
 // analyses (sc_analysis.hip)
 void launch_filament_stats(const Dev &d, const Geo &g, double *e2e, double *com,
                            hipStream_t st);

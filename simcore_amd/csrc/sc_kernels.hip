@@ -61,6 +61,7 @@ __global__ void k_zero_i32(int32_t *p, int64_t n) {
 __device__ __forceinline__ bool guard_off(const Ctl *c, int guard) {
   if (!c) return false;
   if (c->abort) return true;
+  if (guard == kGuardXlink) return !c->xl_dirty;
   return guard == kGuardRebuild && !c->rebuild;
 }
 

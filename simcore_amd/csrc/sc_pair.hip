@@ -133,6 +133,19 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   if (b >= nb) return;
   const BondRec own = load_rec(d.br, b);
   double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+  if (g.nx > 0) {
+    // crosslink forces first, in crosslink order (the reference applies
+    // them inside Interact before the pair forces,
+    // src/interaction_manager.cpp:70, src/crosslink_species.cpp:262-266)
+    const int32_t x0 = d.xinc_start[b], x1 = d.xinc_start[b + 1];
+    for (int32_t q = x0; q < x1; ++q) {
+      const int32_t e = d.xinc[q];
+      const double *o = d.xf + 9 * (int64_t)(e >> 1);
+      const int a = e & 1;
+      for (int k = 0; k < 3; ++k) F[k] += a == 0 ? o[k] : -o[k];
+      for (int k = 0; k < 3; ++k) T[k] += o[3 + 3 * a + k];
+    }
+  }
   const int32_t q0 = d.inc_start[b], q1 = d.inc_start[b + 1];
   // software pipeline: the next incidence's entry and partner record are in
   // flight while the current pair is computed

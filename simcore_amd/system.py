@@ -14,8 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import (SC_ERR_UNPHYSICAL, SC_OK, SC_PREC_FP64, SC_PREC_MIXED,
-                   ScFilamentParams, ScParams, ScTuneReport, SimcoreError,
-                   UnphysicalResult, lib)
+                   ScCrosslinkParams, ScFilamentParams, ScParams, ScTuneReport,
+                   SimcoreError, UnphysicalResult, lib)
 
 
 def _dp(a):
@@ -112,6 +112,24 @@ def generate_filaments(params, n_sites, length, persistence_length, diameter=Non
                                      _dp(d), friction_ratio, seed, _ip(site_off), _dp(r), fil,
                                      byref(tot)), what="sc_generate_filaments")
     return Filaments(site_off, r, fil)
+
+
+def generate_crosslinks(params, filaments, n, r_bind=2.0, seed=1):
+    """(bonds (n,2), lambda (n,2)): n crosslinks between a random bond and a
+    random other bond within r_bind of the first anchor."""
+    so = np.ascontiguousarray(filaments.site_off, dtype=np.int32)
+    r = np.ascontiguousarray(filaments.r, dtype=np.float64)
+    b = np.empty(2 * n, dtype=np.int32)
+    lam = np.empty(2 * n)
+    _check(lib.sc_generate_crosslinks(byref(params), filaments.nf, _ip(so), _dp(r), filaments.fil,
+                                      int(n), float(r_bind), int(seed), _ip(b), _dp(lam)),
+           what="sc_generate_crosslinks")
+    return b.reshape(-1, 2), lam.reshape(-1, 2)
+
+
+def crosslink_params(k_spring=10.0, rest_length=1.0, k_compress=-1.0, k_off=0.1, r_bind=2.0,
+                     seed=1):
+    return ScCrosslinkParams(k_spring, rest_length, k_compress, k_off, r_bind, seed)
 
 
 class FilamentSystem:
@@ -226,6 +244,27 @@ class FilamentSystem:
                self.h, "sc_get_counters")
         return {"i_step": i_step.value, "n_rebuilds": nreb.value,
                 "n_candidates": nc.value, "n_cells_1d": n1d.value}
+
+    def load_crosslinks(self, bonds, lam, **kw):
+        """Attach crosslinks (caller bond numbering); keyword arguments as
+        crosslink_params."""
+        b = np.ascontiguousarray(bonds, dtype=np.int32).reshape(-1)
+        la = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)
+        xp = crosslink_params(**kw)
+        _check(lib.sc_load_crosslinks(self.h, len(b) // 2, _ip(b), _dp(la), byref(xp)), self.h,
+               "sc_load_crosslinks")
+        self.nx = len(b) // 2
+
+    def crosslinks(self):
+        """(bonds (nx,2), lambda (nx,2), stored forces (nx,9), events)."""
+        nx = getattr(self, "nx", 0)
+        b = np.empty(2 * nx, dtype=np.int32)
+        la = np.empty(2 * nx)
+        f = np.empty(9 * nx)
+        ev = c_int64()
+        _check(lib.sc_get_crosslinks(self.h, _ip(b), _dp(la), _dp(f), byref(ev)), self.h,
+               "sc_get_crosslinks")
+        return b.reshape(-1, 2), la.reshape(-1, 2), f.reshape(-1, 9), ev.value
 
     def autotune(self, steps=20):
         """Pick the fastest result-invariant launch shapes (sc_autotune)."""
