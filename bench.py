@@ -66,6 +66,17 @@ def workload(name, sc):
                             "sigma_ln 0.8) clipped to [8,256], lp U[10,1000] sigma, "
                             "volume fraction 0.08, periodic cube, WCA, dt 1e-5",
                 "n_filaments": nf, "volume_fraction": 0.08, "box_side": 2 * R}
+    elif name == "c3":
+        p, fil, desc = workload("c2", sc)
+        b, lam = sc.generate_crosslinks(p, fil, 200000, r_bind=2.0, seed=13)
+        desc = dict(desc)
+        desc["workload"] = ("crosslinked network: the uneven-length system plus 200000 harmonic "
+                            "crosslinks (k 10 kT/sigma^2, rest length 1 sigma) between random "
+                            "bond pairs within 2 sigma at uniform positions, unbinding at rate "
+                            "0.1/tau with rebinding to a random in-range bond, dt 1e-5")
+        desc["crosslinks"] = 200000
+        desc["_xlinks"] = (b, lam)
+        return p, fil, desc
     elif name == "c0":
         p, spec = sc.params_from_yaml(os.path.join(ROOT, "simcore_amd", "configs",
                                                    "c0_plain_fp64.yaml"))
@@ -249,7 +260,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--fp64-steps", type=int, default=200)
@@ -292,8 +303,12 @@ def main():
     p, fil, desc = workload(args.workload, sc)
     log(f"[bench] {fil.nf} filaments, {fil.nb} bonds")
     spatial = args.precision == "mixed" and not args.caller_order
+    xl = desc.pop("_xlinks", None)
+    xl_params = dict(k_spring=10.0, rest_length=1.0, k_off=0.1, r_bind=2.0, seed=14)
     dev = sc.FilamentSystem(p, fil, precision=args.precision, use_graph=True,
                             spatial_order=spatial)
+    if xl is not None:
+        dev.load_crosslinks(xl[0], xl[1], **xl_params)
     dev.step(args.warmup)
     log(f"[bench] warmup {args.warmup} steps done, counters {dev.counters()}")
     c0 = dev.counters()
@@ -328,6 +343,8 @@ def main():
     extra = {}
     if not args.no_fp64_baseline and args.precision != "fp64":
         d64 = sc.FilamentSystem(p, fil, precision="fp64", use_graph=False)
+        if xl is not None:
+            d64.load_crosslinks(xl[0], xl[1], **xl_params)
         d64.step(10)
         w64 = timed_steps(d64, args.fp64_steps)
         d64.profile(True)
@@ -349,6 +366,8 @@ def main():
         if threads <= 0:
             threads = len(os.sched_getaffinity(0))
         ora = OracleSystem(p, fil, n_threads=threads)
+        if xl is not None:
+            ora.load_crosslinks(xl[0], xl[1], **xl_params)
         ora.step(10)
         t0 = time.perf_counter()
         rc = ora.step(args.cpu_steps)
