@@ -76,11 +76,14 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
   // back substitution
 #pragma unroll
   for (int st = 1; st < S; st <<= 1) {
+    // shuffles on every lane (a lane-dependent condition around them would
+    // switch off the lanes they read from), then the boundary select
     const bool first = lane == 0;
-    const double pa = first ? 0.0 : __shfl(a[S - st], first ? lane : lane - 1, 64);
-    const double pb = first ? 1.0 : __shfl(b[S - st], first ? lane : lane - 1, 64);
-    const double pc = first ? 0.0 : __shfl(c[S - st], first ? lane : lane - 1, 64);
-    const double pd = first ? 0.0 : __shfl(d[S - st], first ? lane : lane - 1, 64);
+    const int src = first ? lane : lane - 1;
+    const double ya = __shfl(a[S - st], src, 64), yb = __shfl(b[S - st], src, 64);
+    const double yc = __shfl(c[S - st], src, 64), yd = __shfl(d[S - st], src, 64);
+    const double pa = first ? 0.0 : ya, pb = first ? 1.0 : yb;
+    const double pc = first ? 0.0 : yc, pd = first ? 0.0 : yd;
 #pragma unroll
     for (int k = 0; k < S; k += 2 * st) {
       const double la = k == 0 ? pa : a[k - st], lb = k == 0 ? pb : b[k - st];

@@ -204,8 +204,11 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
     const double diag = 2.0 * igperp + dg * (a1 * a1 + a2 * a2);
     // coupling of rows i-1 and i, computed at row i
     const double hu = -igperp * dot(u, um) - dg * (dot(t, um) * dot(t, u));
+    // the shuffle runs on every lane: under a lane-dependent condition the
+    // lanes it reads from could be switched off
+    const double hun = d_next(hu);
     const double lower = (i >= 1) ? hu : 0.0;
-    const double upper = (i < m - 1) ? d_next(hu) : 0.0;
+    const double upper = (i < m - 1) ? hun : 0.0;
     T = pcr<G>(lower, diag, upper, rhs, m, lane, gl);
   }
   if (i < m) d.tension[b0 + i] = T;
