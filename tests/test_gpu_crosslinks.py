@@ -10,6 +10,12 @@ from tests.helpers import box_for_fraction
 
 pytestmark = pytest.mark.gpu
 
+# Mixed-path position tolerance over the first steps: FP32 pair forces
+# round to ~6e-8 relative, and WCA forces reach the 2000 kT/sigma cap at
+# contact, so one step (dt/2 = 5e-5, gamma_perp ~ 0.4) can move a site by a
+# few 1e-8 sigma against the FP64 path; 1e-7 sigma keeps a 3x margin.
+POS_TOL_SHORT = 1e-7
+
 XL = dict(k_spring=10.0, rest_length=1.0, k_compress=-1.0, k_off=500.0, r_bind=2.0, seed=99)
 
 
@@ -62,7 +68,7 @@ def test_crosslinks_mixed_spatial_order_within_tolerance():
     assert ora.step(2) == 0
     d2 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
     print("max |dr| after 2 steps", d2)
-    assert d2 <= 1e-8
+    assert d2 <= POS_TOL_SHORT
     dev.step(98)
     assert ora.step(98) == 0
     d100 = np.abs(dev.sites()[0] - ora.sites()[0]).max()

@@ -9,6 +9,12 @@ from tests.helpers import c0_system, culled
 
 pytestmark = pytest.mark.gpu
 
+# Mixed-path position tolerance over the first steps: FP32 pair forces
+# round to ~6e-8 relative, and WCA forces reach the 2000 kT/sigma cap at
+# contact, so one step (dt/2 = 5e-5, gamma_perp ~ 0.4) can move a site by a
+# few 1e-8 sigma against the FP64 path; 1e-7 sigma keeps a 3x margin.
+POS_TOL_SHORT = 1e-7
+
 
 def _pair(num=None, seed=7, precision="fp64", use_graph=False, capacity_factor=0.0):
     p, fil = c0_system(num=num, seed=seed)
@@ -144,7 +150,7 @@ def test_set_sites_keeps_trajectories_identical():
 def test_mixed_trajectory_within_tolerance():
     """Mixed path (FP32 pair math, FP64 integrator with the algebraically
     reduced forms and parallel cyclic reduction) against the FP64 CPU path
-    from the same state and noise: positions within 1e-8 sigma after one
+    from the same state and noise: positions within POS_TOL_SHORT after one
     step and 1e-5 sigma after 100 steps; candidate sets identical."""
     print("case: C0 mixed trajectory 1 and 100 steps", flush=True)
     p, fil, dev, ora = _pair(precision="mixed")
@@ -152,7 +158,7 @@ def test_mixed_trajectory_within_tolerance():
     assert ora.step(1) == 0
     d1 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
     print("max |dr| after 1 step", d1)
-    assert d1 <= 1e-8
+    assert d1 <= POS_TOL_SHORT
     np.testing.assert_allclose(dev.tensions(), ora.tensions(), rtol=1e-5,
                                atol=1e-5 * np.abs(ora.tensions()).max())
     dev.step(99)
@@ -196,7 +202,7 @@ def test_uneven_lengths_fp64_bit_exact(kind):
 @pytest.mark.parametrize("kind", ["short", "long"])
 def test_uneven_lengths_mixed_trajectory(kind):
     """Mixed path on every length bin against the FP64 CPU path: positions
-    within 1e-8 sigma after one step and 1e-5 sigma after 100 steps,
+    within POS_TOL_SHORT after one step and 1e-5 sigma after 100 steps,
     tensions within 1e-5 relative after one step."""
     print(f"case: uneven {kind} mixed 1 and 100 steps", flush=True)
     p, fil, dev = _uneven(kind, "mixed")
@@ -205,14 +211,14 @@ def test_uneven_lengths_mixed_trajectory(kind):
     assert ora.step(1) == 0
     d1 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
     print("max |dr| after 1 step", d1)
-    assert d1 <= 1e-8
+    assert d1 <= POS_TOL_SHORT
     np.testing.assert_allclose(dev.tensions(), ora.tensions(), rtol=1e-5,
                                atol=1e-5 * np.abs(ora.tensions()).max())
     dev.step(1)
     assert ora.step(1) == 0
     d2 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
     print("max |dr| after 2 steps", d2)
-    assert d2 <= 1e-8
+    assert d2 <= POS_TOL_SHORT
     dev.step(98)
     assert ora.step(98) == 0
     d100 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
@@ -262,7 +268,7 @@ def test_spatial_order_mixed_trajectory(kind):
     assert ora.step(1) == 0
     d1 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
     print("max |dr| after 1 step", d1)
-    assert d1 <= 1e-8
+    assert d1 <= POS_TOL_SHORT
     fd, _ = dev.bond_forces()
     fo, _ = ora.bond_forces()
     assert np.linalg.norm(fd - fo) / np.linalg.norm(fo) <= 1e-5
