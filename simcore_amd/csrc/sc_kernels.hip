@@ -240,31 +240,89 @@ __device__ __forceinline__ bool excluded(const int32_t *bond_fil, int32_t a,
   return (df == 1 || df == -1) && bond_fil[a] == bond_fil[b];
 }
 
+// q-th cell of the half-shell stencil of (x, y, z) in the order of
+// CellList::AssignCellNeighbors (src/cell_list.cpp:206-262): (x+1, y, z);
+// (x-1..x+1, y+1, z); then (x-1..x+1, y-1..y+1, z+1), x fastest.
+__device__ __forceinline__ int32_t shell_key(int n, int x, int y, int z, int q) {
+  int dx, dy, dz;
+  if (q == 0) {
+    dx = 1;
+    dy = 0;
+    dz = 0;
+  } else if (q < 4) {
+    dx = q - 2;
+    dy = 1;
+    dz = 0;
+  } else {
+    dx = (q - 4) % 3 - 1;
+    dy = (q - 4) / 3 - 1;
+    dz = 1;
+  }
+  int xp = x + dx, yp = y + dy, zp = z + dz;
+  xp = xp < 0 ? n - 1 : (xp == n ? 0 : xp);
+  yp = yp < 0 ? n - 1 : (yp == n ? 0 : yp);
+  zp = zp == n ? 0 : zp;
+  return ((int32_t)xp * n + yp) * n + zp;
+}
+
+// Sixteen lanes per sorted position p: for the first position of an occupied
+// cell (its "home" run [p, e)), lane 13 covers the self pairs and lanes
+// 0..12 the 13 neighbour cells; the other positions contribute nothing.
+struct HomeRun {
+  bool leader;
+  int32_t k, e;
+};
+
+__device__ __forceinline__ HomeRun home_run(const Dev &d, const Geo &g, int64_t p) {
+  HomeRun h{false, 0, 0};
+  if (p < g.nb) {
+    h.k = d.skey[p];
+    h.leader = d.cell_start[h.k] == p;
+    h.e = d.cell_start[h.k + 1];
+  }
+  return h;
+}
+
+__device__ __forceinline__ void seg_range(const Dev &d, const Geo &g, const HomeRun &h, int q,
+                                          int32_t *o0, int32_t *o1) {
+  const int n = g.n1d;
+  const int x = h.k / (n * n), y = (h.k / n) % n, z = h.k % n;
+  const int32_t nb = shell_key(n, x, y, z, q);
+  *o0 = d.cell_start[nb];
+  *o1 = d.cell_start[nb + 1];
+}
+
+__device__ __forceinline__ int32_t seg_count(const Dev &d, const Geo &g, const HomeRun &h,
+                                             int64_t p, int q) {
+  if (!h.leader || q > 13) return 0;
+  int32_t cnt = 0;
+  if (q == 13) {
+    for (int32_t i = (int32_t)p; i < h.e; ++i)
+      for (int32_t j = i + 1; j < h.e; ++j) cnt += !excluded(d.bond_fil, d.sorted[i], d.sorted[j]);
+    return cnt;
+  }
+  int32_t o0, o1;
+  seg_range(d, g, h, q, &o0, &o1);
+  for (int32_t i = (int32_t)p; i < h.e; ++i) {
+    const int32_t bi = d.sorted[i];
+    for (int32_t j = o0; j < o1; ++j) cnt += !excluded(d.bond_fil, bi, d.sorted[j]);
+  }
+  return cnt;
+}
+
 // Candidate count of each occupied home cell (Cell::MakePairs,
 // src/cell.cpp:14-70), excluded pairs already removed (the reference culls
 // them on the rebuild step, src/interaction_manager.cpp:485-497).
-__global__ void k_count(Dev d, Geo g) {
+__global__ __launch_bounds__(256) void k_count(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.nb) return;
-  int32_t k = d.skey[p];
-  int32_t cnt = 0;
-  if (d.cell_start[k] == p) {
-    int32_t e = d.cell_start[k + 1];
-    for (int32_t i = (int32_t)p; i < e; ++i)
-      for (int32_t j = i + 1; j < e; ++j)
-        cnt += !excluded(d.bond_fil, d.sorted[i], d.sorted[j]);
-    int32_t nbr[13];
-    half_shell(g.n1d, k, nbr);
-    for (int q = 0; q < 13; ++q) {
-      int32_t o0 = d.cell_start[nbr[q]], o1 = d.cell_start[nbr[q] + 1];
-      for (int32_t i = (int32_t)p; i < e; ++i) {
-        int32_t bi = d.sorted[i];
-        for (int32_t j = o0; j < o1; ++j) cnt += !excluded(d.bond_fil, bi, d.sorted[j]);
-      }
-    }
-  }
-  d.ccount[p] = cnt;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = t >> 4;
+  const int q = (int)(t & 15);
+  const HomeRun h = home_run(d, g, p);
+  int32_t c = seg_count(d, g, h, p, q);
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) c += __shfl_xor(c, o, 16);
+  if (q == 0 && p < g.nb) d.ccount[p] = c;
 }
 
 __global__ void k_check_cap(Dev d, Geo g) {
@@ -282,15 +340,27 @@ __global__ void k_check_cap(Dev d, Geo g) {
 
 // Writes the candidate list in MakePairs order: home cells ascending; in
 // each, the self pairs (i<j) then, for each of the 13 half-shell
-// neighbours in AssignCellNeighbors order, all home x neighbour pairs.
-__global__ void k_emit(Dev d, Geo g, int prec) {
+// neighbours in AssignCellNeighbors order, all home x neighbour pairs.  The
+// sixteen lanes of a home run write their segments concurrently at offsets
+// from a scan of the segment counts.
+__global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.nb) return;
-  int32_t k = d.skey[p];
-  if (d.cell_start[k] != p) return;
-  int32_t e = d.cell_start[k + 1];
-  int32_t w = d.coff[p];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = t >> 4;
+  const int q = (int)(t & 15);
+  const HomeRun h = home_run(d, g, p);
+  const int32_t c = seg_count(d, g, h, p, q);
+  // segment order: self (lane 13) first, then lanes 0..12
+  const int32_t cq = q == 13 ? 0 : c;
+  int32_t incl = cq;
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) {
+    const int32_t y = __shfl_up(incl, o, 16);
+    if (q >= o) incl += y;
+  }
+  const int32_t c13 = __shfl(c, 13, 16);
+  if (!h.leader || q > 13 || c == 0) return;
+  int32_t w = d.coff[p] + (q == 13 ? 0 : c13 + incl - cq);
   auto put = [&](int32_t b1, int32_t b2) {
     d.pairs[w] = make_int2(b1, b2);
     if (prec == SC_PREC_FP64) {
@@ -302,21 +372,21 @@ __global__ void k_emit(Dev d, Geo g, int prec) {
     atomicAdd(&d.deg[b2], 1);
     ++w;
   };
-  for (int32_t i = (int32_t)p; i < e; ++i)
-    for (int32_t j = i + 1; j < e; ++j) {
-      int32_t bi = d.sorted[i], bj = d.sorted[j];
-      if (!excluded(d.bond_fil, bi, bj)) put(bi, bj);
-    }
-  int32_t nbr[13];
-  half_shell(g.n1d, k, nbr);
-  for (int q = 0; q < 13; ++q) {
-    int32_t o0 = d.cell_start[nbr[q]], o1 = d.cell_start[nbr[q] + 1];
-    for (int32_t i = (int32_t)p; i < e; ++i) {
-      int32_t bi = d.sorted[i];
-      for (int32_t j = o0; j < o1; ++j) {
-        int32_t bj = d.sorted[j];
+  if (q == 13) {
+    for (int32_t i = (int32_t)p; i < h.e; ++i)
+      for (int32_t j = i + 1; j < h.e; ++j) {
+        const int32_t bi = d.sorted[i], bj = d.sorted[j];
         if (!excluded(d.bond_fil, bi, bj)) put(bi, bj);
       }
+    return;
+  }
+  int32_t o0, o1;
+  seg_range(d, g, h, q, &o0, &o1);
+  for (int32_t i = (int32_t)p; i < h.e; ++i) {
+    const int32_t bi = d.sorted[i];
+    for (int32_t j = o0; j < o1; ++j) {
+      const int32_t bj = d.sorted[j];
+      if (!excluded(d.bond_fil, bi, bj)) put(bi, bj);
     }
   }
 }
@@ -378,10 +448,11 @@ void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
               kGuardRebuild, st);
   hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(bs), 0, st, d, g);
   hipLaunchKernelGGL(k_run_sort, dim3(gb), dim3(bs), 0, st, d, g);
-  hipLaunchKernelGGL(k_count, dim3(gb), dim3(bs), 0, st, d, g);
+  const unsigned gb16 = nblocks(16 * g.nb, bs);
+  hipLaunchKernelGGL(k_count, dim3(gb16), dim3(bs), 0, st, d, g);
   launch_scan(d.ccount, d.coff, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
   hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, st, d, g);
-  hipLaunchKernelGGL(k_emit, dim3(gb), dim3(bs), 0, st, d, g,
+  hipLaunchKernelGGL(k_emit, dim3(gb16), dim3(bs), 0, st, d, g,
                      d.slot64 ? (int)SC_PREC_FP64 : (int)SC_PREC_MIXED);
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
   hipLaunchKernelGGL(k_inc_fill, dim3(2048), dim3(bs), 0, st, d, g);
