@@ -267,6 +267,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--caller-order", action="store_true",
                     help="keep filaments in generation order (no spatial storage order)")
     # serving workload (--workload c4)
@@ -311,6 +312,12 @@ def main():
         dev.load_crosslinks(xl[0], xl[1], **xl_params)
     dev.step(args.warmup)
     log(f"[bench] warmup {args.warmup} steps done, counters {dev.counters()}")
+    tune = None
+    if not args.no_autotune:
+        # launch shapes that leave results unchanged, chosen from a
+        # checkpoint of the warmed-up state (the state is restored)
+        tune = dev.autotune(steps=20)
+        log(f"[bench] autotune: {tune}")
     c0 = dev.counters()
     wall = timed_steps(dev, args.steps)
     c1 = dev.counters()
@@ -389,7 +396,8 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if args.precision == "mixed" else "f64",
         "data": "synthetic",
-        "config": dict(desc, precision=args.precision, graph=True, spatial_order=spatial),
+        "config": dict(desc, precision=args.precision, graph=True, spatial_order=spatial,
+                       launch_shape=tune),
         "roofline": roofline, "cpu_baseline": cpu,
     }
     out.update(extra)

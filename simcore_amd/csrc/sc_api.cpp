@@ -188,7 +188,6 @@ void enqueue_step(sc_system *h, bool odd) {
   const Geo &g = h->g;
   hipStream_t st = h->stream;
   const bool pf = h->prof;
-  sc::launch_decide(d, g, st);
   if (pf) (void)hipEventRecord(h->ev[6], st);
   if (odd) sc::launch_rebuild(d, g, st);
   if (pf) {
@@ -212,7 +211,7 @@ void enqueue_step(sc_system *h, bool odd) {
   for (const Bin &b : h->bins)
     sc::launch_integrate(d, g, b.list, b.count, b.maxn, h->p.precision, h->tune, st);
   if (pf) (void)hipEventRecord(h->ev[5], st);
-  sc::launch_commit(d, st);
+  sc::launch_commit(d, g, st);
 }
 
 int capture_graphs(sc_system *h) {
@@ -604,6 +603,8 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipGetLastError());
   if ((rc = initial_rebuild(h))) return rc;
+  sc::launch_decide(d, g, h->stream);  // the decision for step 1
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   *h->ctl_host = read_ctl(h);
   h->i_step_host = h->ctl_host->i_step;
   h->loaded = true;
@@ -940,6 +941,8 @@ int sc_set_sites(sc_handle h, const double *r) {
     if ((rc = write_ctl(h, c))) return rc;
     sc::launch_redo_dr(h->d, h->g, h->stream);
   }
+  // the decision for the next step was taken from the old positions
+  sc::launch_decide(h->d, h->g, h->stream);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipGetLastError());
   return SC_OK;
@@ -1164,12 +1167,14 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   const int waves[] = {1, 2, 4};
   for (int bs : blocks)
     for (int x = 0; x < 2 && rc == SC_OK; ++x)
-      for (int w : waves) {
+      for (int w : waves)
+        for (int lay = 0; lay < 2; ++lay) {
         if (rc) break;
         sc::Tune t;
         t.pair_block = bs;
         t.pair_xcd = x;
         t.int_waves = w;
+        t.int_layout = lay;
         double ms = 0;
         rc = trial(t, &ms);
         ++trials;
@@ -1191,10 +1196,29 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
     report->pair_block = best.pair_block;
     report->pair_xcd = best.pair_xcd;
     report->integrate_waves = best.int_waves;
+    report->integrate_layout = best.int_layout;
     report->n_trials = trials;
     report->step_ms_start = start_ms;
     report->step_ms_best = best_ms;
   }
+  return SC_OK;
+}
+
+int sc_set_launch_shape(sc_handle h, int32_t pair_block, int32_t pair_xcd,
+                        int32_t integrate_waves, int32_t integrate_layout) {
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  if ((pair_block != 64 && pair_block != 128 && pair_block != 256 && pair_block != 512) ||
+      (pair_xcd != 0 && pair_xcd != 1) ||
+      (integrate_waves != 1 && integrate_waves != 2 && integrate_waves != 4) ||
+      (integrate_layout != 0 && integrate_layout != 1))
+    return fail(h, SC_ERR_ARG, "unsupported launch shape");
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
+  h->tune.pair_block = pair_block;
+  h->tune.pair_xcd = pair_xcd;
+  h->tune.int_waves = integrate_waves;
+  h->tune.int_layout = integrate_layout;
+  drop_graph(h);
   return SC_OK;
 }
 

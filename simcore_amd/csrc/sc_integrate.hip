@@ -661,10 +661,11 @@ void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
                       int bin_count, int bin_maxn, int precision, const Tune &t,
                       hipStream_t st) {
   if (bin_count <= 0) return;
-  // fast path, filaments of up to 64 sites: register-resident variant
-  if (launch_integrate_reg(d, g, bin_list, bin_count, bin_maxn, precision, t.int_waves, st))
+  // fast path: register-resident variants
+  if (launch_integrate_long(d, g, bin_list, bin_count, bin_maxn, precision, t.int_waves,
+                            t.int_layout, st))
     return;
-  if (launch_integrate_long(d, g, bin_list, bin_count, bin_maxn, precision, t.int_waves, st))
+  if (launch_integrate_reg(d, g, bin_list, bin_count, bin_maxn, precision, t.int_waves, st))
     return;
   const bool exact = precision == SC_PREC_FP64;
   const int G = integrate_bin_lanes(bin_maxn);

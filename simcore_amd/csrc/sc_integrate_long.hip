@@ -1,16 +1,17 @@
-// sc_integrate_long.hip -- register-resident fast-path integrator for
-// filaments of 65 to 256 sites (Filament::UpdatePosition,
-// src/filament.cpp:394-428).
+// sc_integrate_long.hip -- register-resident fast-path integrator with
+// several sites per lane (Filament::UpdatePosition,
+// src/filament.cpp:394-428): filaments of up to G*S sites, G lanes each,
+// 64/G filaments per wavefront.
 //
-// One wavefront per filament; lane l owns the S consecutive sites
-// lS .. lS+S-1 (S = 2 or 4), so a neighbour is either in the lane's own
-// registers or one lane-shuffle away.  Tridiagonal systems: log2(S) levels
-// of cyclic reduction inside each lane leave one row per lane, solved by
-// parallel cyclic reduction across the 64 lanes, then back-substituted
-// level by level.  Metric determinants: 2x2 matrix products, sequential
-// inside a lane, scanned across lanes.  Renormalisation: prefix sums the
-// same way.  No LDS, no barriers.  Same reduced algebra as the short-filament
-// kernel (sc_integrate_reg.hip).
+// Group lane l owns the S consecutive sites lS .. lS+S-1, so a neighbour is
+// either in the lane's own registers or one lane-shuffle away.
+// Tridiagonal systems: log2(S) levels of cyclic reduction inside each lane
+// leave one row per lane, solved by parallel cyclic reduction across the
+// group's lanes, then back-substituted level by level.  Metric
+// determinants: 2x2 matrix products, sequential inside a lane, scanned
+// across lanes.  Renormalisation: prefix sums the same way.  No LDS, no
+// barriers.  Against one site per lane (sc_integrate_reg.hip) this halves
+// the lanes, hence the shuffles, per filament.  Same reduced algebra.
 #include <hip/hip_runtime.h>
 
 #include "sc_internal.h"
@@ -23,9 +24,10 @@ using namespace lanes;
 
 namespace {
 
-// o[k] = value at site lS + k + DELTA (fb outside the wave's 64*S sites)
-template <int S, int DELTA>
-__device__ __forceinline__ void site_at(const double (&v)[S], double (&o)[S], int lane,
+// o[k] = value at site gl*S + k + DELTA of the group (fb outside its G*S
+// sites); the shuffles run on every lane
+template <int G, int S, int DELTA>
+__device__ __forceinline__ void site_at(const double (&v)[S], double (&o)[S], int lane, int gl,
                                         double fb) {
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -35,22 +37,22 @@ __device__ __forceinline__ void site_at(const double (&v)[S], double (&o)[S], in
     if (q == 0) {
       o[k] = v[ks];
     } else {
-      const int src = lane + q;
-      const bool ok = src >= 0 && src < 64;
-      const double y = __shfl(v[ks], ok ? src : lane, 64);
+      const bool ok = gl + q >= 0 && gl + q < G;
+      const double y = __shfl(v[ks], ok ? lane + q : lane, 64);
       o[k] = ok ? y : fb;
     }
   }
 }
 
-template <int S, int DELTA>
-__device__ __forceinline__ void site_at3(const double (&v)[S][3], double (&o)[S][3], int lane) {
+template <int G, int S, int DELTA>
+__device__ __forceinline__ void site_at3(const double (&v)[S][3], double (&o)[S][3], int lane,
+                                         int gl) {
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     double a[S], b[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) a[k] = v[k][q];
-    site_at<S, DELTA>(a, b, lane, 0.0);
+    site_at<G, S, DELTA>(a, b, lane, gl, 0.0);
 #pragma unroll
     for (int k = 0; k < S; ++k) o[k][q] = b[k];
   }
@@ -58,13 +60,13 @@ __device__ __forceinline__ void site_at3(const double (&v)[S][3], double (&o)[S]
 
 // Solves rows i = lS + k < m (lower a: x[i-1], diagonal b, upper c:
 // x[i+1], right-hand side d); rows >= m are inert.  Result in x.
-template <int S>
+template <int G, int S>
 __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], double (&c)[S],
-                                              double (&d)[S], int m, int lane,
+                                              double (&d)[S], int m, int lane, int gl,
                                               double (&x)[S]) {
 #pragma unroll
   for (int k = 0; k < S; ++k)
-    if (lane * S + k >= m) {
+    if (gl * S + k >= m) {
       a[k] = 0.0;
       b[k] = 1.0;
       c[k] = 0.0;
@@ -78,7 +80,7 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
   for (int st = 1; st < S; st <<= 1) {
     // shuffles on every lane (a lane-dependent condition around them would
     // switch off the lanes they read from), then the boundary select
-    const bool first = lane == 0;
+    const bool first = gl == 0;
     const int src = first ? lane : lane - 1;
     const double ya = __shfl(a[S - st], src, 64), yb = __shfl(b[S - st], src, 64);
     const double yc = __shfl(c[S - st], src, 64), yd = __shfl(d[S - st], src, 64);
@@ -98,14 +100,15 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
   }
   // one row per lane left: parallel cyclic reduction across the lanes
   const int m_red = (m + S - 1) / S;
-  x[0] = pcr<64>(a[0], b[0], c[0], d[0], m_red, lane, lane);
+  x[0] = pcr<G>(a[0], b[0], c[0], d[0], m_red, lane, gl);
 #pragma unroll
   for (int st = S / 2; st >= 1; st >>= 1) {
-    const double nx = __shfl(x[0], lane < 63 ? lane + 1 : lane, 64);
+    const bool last = gl == G - 1;
+    const double nx = __shfl(x[0], last ? lane : lane + 1, 64);
 #pragma unroll
     for (int k = st; k < S; k += 2 * st) {
       const double xl = x[k - st];
-      const double xu = k + st < S ? x[k + st] : (lane < 63 ? nx : 0.0);
+      const double xu = k + st < S ? x[k + st] : (last ? 0.0 : nx);
       x[k] = (d[k] - a[k] * xl - c[k] * xu) / b[k];
     }
   }
@@ -120,22 +123,22 @@ __device__ __forceinline__ void mat_mul(const double *P, const double *Q, double
 
 // Prefix (forward) products of per-site 2x2 matrices M (newest on the
 // left): out[k] = M_{lS+k} ... M_0 applied to [2, 1], first component.
-template <int S>
+template <int G, int S>
 __device__ __forceinline__ void det_forward(const double (&M)[S][4], double (&out)[S],
-                                            int lane) {
+                                            int lane, int gl) {
   double L[S][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) L[0][q] = M[0][q];
 #pragma unroll
   for (int k = 1; k < S; ++k) mat_mul(M[k], L[k - 1], L[k]);
   double A[4] = {L[S - 1][0], L[S - 1][1], L[S - 1][2], L[S - 1][3]};
-  mat_scan<64>(A, lane, lane, -1);  // A_l ... A_0
+  mat_scan<G>(A, lane, gl, -1);  // A_l ... A_0
   double E[4];
   const double id[4] = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const double y = __shfl(A[q], lane > 0 ? lane - 1 : lane, 64);
-    E[q] = lane > 0 ? y : id[q];
+    const double y = __shfl(A[q], gl > 0 ? lane - 1 : lane, 64);
+    E[q] = gl > 0 ? y : id[q];
   }
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -146,22 +149,22 @@ __device__ __forceinline__ void det_forward(const double (&M)[S][4], double (&ou
 }
 
 // Suffix products: out[k] = N_{lS+k} ... N_last applied to [2, 1].
-template <int S>
+template <int G, int S>
 __device__ __forceinline__ void det_backward(const double (&N)[S][4], double (&out)[S],
-                                             int lane) {
+                                             int lane, int gl) {
   double R[S][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) R[S - 1][q] = N[S - 1][q];
 #pragma unroll
   for (int k = S - 2; k >= 0; --k) mat_mul(N[k], R[k + 1], R[k]);
   double B[4] = {R[0][0], R[0][1], R[0][2], R[0][3]};
-  mat_scan<64>(B, lane, lane, +1);  // B_l ... B_63
+  mat_scan<G>(B, lane, gl, +1);  // B_l ... B_last
   double F[4];
   const double id[4] = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const double y = __shfl(B[q], lane < 63 ? lane + 1 : lane, 64);
-    F[q] = lane < 63 ? y : id[q];
+    const double y = __shfl(B[q], gl < G - 1 ? lane + 1 : lane, 64);
+    F[q] = gl < G - 1 ? y : id[q];
   }
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -173,13 +176,16 @@ __device__ __forceinline__ void det_backward(const double (&N)[S][4], double (&o
 
 }  // namespace
 
-template <int S>
+template <int G, int S>
 __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int32_t *list,
                                                         int count, int prec) {
+  constexpr int NG = 64 / G;
   const Ctl *ctl = d.ctl;
   if (ctl->abort) return;
   const int lane = threadIdx.x & 63;
-  const int fidx = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int grp = lane / G, gl = lane % G;
+  const int fidx = wave * NG + grp;
   const bool factive = fidx < count;
   const int f = factive ? list[fidx] : 0;
   const int64_t s0 = factive ? d.site_off[f] : 0;
@@ -201,7 +207,7 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
     sgperp = fp[5];
     noise0 = d.noise_off[f];
   }
-  const int base = lane * S;
+  const int base = gl * S;
 
   // ---- loads -------------------------------------------------------------
   double rp[S][3], u[S][3], Fb[S][3], Tb[S][3], fr[S][3], dr0[S][3], acc[S];
@@ -251,8 +257,8 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
       sf[k][q] = 0.5 * Fb[k][q];
     }
   }
-  site_at3<S, -1>(sf, sfm, lane);
-  site_at3<S, -1>(pt, ptm, lane);
+  site_at3<G, S, -1>(sf, sfm, lane, gl);
+  site_at3<G, S, -1>(pt, ptm, lane, gl);
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     const int i = base + k;
@@ -265,8 +271,8 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
 
   // ---- angles and tangents ----------------------------------------------
   double um[S][3], up[S][3], t[S][3], cs[S];
-  site_at3<S, -1>(u, um, lane);
-  site_at3<S, 1>(u, up, lane);
+  site_at3<G, S, -1>(u, um, lane, gl);
+  site_at3<G, S, 1>(u, up, lane, gl);
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     const int i = base + k;
@@ -280,7 +286,7 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
     }
   }
   double csm[S];
-  site_at<S, -1>(cs, csm, lane, 0.0);
+  site_at<G, S, -1>(cs, csm, lane, gl, 0.0);
 
   // ---- random forces (midstep) ------------------------------------------
   if (mid) {
@@ -295,7 +301,7 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
           fr[k][q] = sgperp * xi[q] + (sgpar - sgperp) * t[k][q] * txi;
       }
       double frn[S][3], a[S], b[S], c[S], rh[S], Tp[S], Tm[S];
-      site_at3<S, 1>(fr, frn, lane);
+      site_at3<G, S, 1>(fr, frn, lane, gl);
 #pragma unroll
       for (int k = 0; k < S; ++k) {
         const int i = base + k;
@@ -306,8 +312,8 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
         b[k] = 2.0;
         c[k] = i < m - 1 ? -cs[k] : 0.0;
       }
-      solve_blocked<S>(a, b, c, rh, m, lane, Tp);
-      site_at<S, -1>(Tp, Tm, lane, 0.0);
+      solve_blocked<G, S>(a, b, c, rh, m, lane, gl, Tp);
+      site_at<G, S, -1>(Tp, Tm, lane, gl, 0.0);
 #pragma unroll
       for (int k = 0; k < S; ++k) {
         const int i = base + k;
@@ -335,7 +341,7 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
   // ---- bending (src/filament.cpp:608-776), reduced form ------------------
   {
     double csmm[S], M[S][4], N[S][4], det_t[S], det_b[S], db3[S], ke[S];
-    site_at<S, -1>(csm, csmm, lane, 0.0);
+    site_at<G, S, -1>(csm, csmm, lane, gl, 0.0);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
@@ -350,16 +356,16 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
       N[k][2] = sn ? 1.0 : 0.0;
       N[k][3] = sn ? 0.0 : 1.0;
     }
-    det_forward<S>(M, det_t, lane);
-    det_backward<S>(N, det_b, lane);
+    det_forward<G, S>(M, det_t, lane, gl);
+    det_backward<G, S>(N, det_b, lane, gl);
     // det_t at the last site, det_b three sites ahead (det_b[n] = 1)
     const int last = n > 0 ? n - 1 : 0;
     double pick = 0.0;
 #pragma unroll
     for (int k = 0; k < S; ++k)
       if (k == last % S) pick = det_t[k];
-    const double det_g = __shfl(pick, last / S, 64);
-    site_at<S, 3>(det_b, db3, lane, 1.0);
+    const double det_g = __shfl(pick, lane - gl + last / S, 64);
+    site_at<G, S, 3>(det_b, db3, lane, gl, 1.0);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
@@ -372,9 +378,9 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
       }
     }
     double kem[S], kemm[S], umm[S][3];
-    site_at<S, -1>(ke, kem, lane, 0.0);
-    site_at<S, -2>(ke, kemm, lane, 0.0);
-    site_at3<S, -2>(u, umm, lane);
+    site_at<G, S, -1>(ke, kem, lane, gl, 0.0);
+    site_at<G, S, -2>(ke, kemm, lane, gl, 0.0);
+    site_at3<G, S, -2>(u, umm, lane, gl);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
@@ -401,8 +407,8 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
   double T[S], Tm[S];
   {
     double tn[S][3], fn[S][3], hu[S], hun[S], a[S], b[S], c[S], rh[S];
-    site_at3<S, 1>(t, tn, lane);
-    site_at3<S, 1>(fs, fn, lane);
+    site_at3<G, S, 1>(t, tn, lane, gl);
+    site_at3<G, S, 1>(fs, fn, lane, gl);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       double m1[3], m2[3], fd[3];
@@ -414,16 +420,16 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
       b[k] = 2.0 * igperp + dg * (a1 * a1 + a2 * a2);
       hu[k] = -igperp * dot(u[k], um[k]) - dg * (dot(t[k], um[k]) * dot(t[k], u[k]));
     }
-    site_at<S, 1>(hu, hun, lane, 0.0);
+    site_at<G, S, 1>(hu, hun, lane, gl, 0.0);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
       a[k] = i >= 1 ? hu[k] : 0.0;
       c[k] = i < m - 1 ? hun[k] : 0.0;
     }
-    solve_blocked<S>(a, b, c, rh, m, lane, T);
+    solve_blocked<G, S>(a, b, c, rh, m, lane, gl, T);
   }
-  site_at<S, -1>(T, Tm, lane, 0.0);
+  site_at<G, S, -1>(T, Tm, lane, gl, 0.0);
 #pragma unroll
   for (int k = 0; k < S; ++k)
     if (base + k < m) d.tension[b0 + base + k] = T[k];
@@ -445,7 +451,7 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
     mob(igperp, dg, t[k], fs[k], mv);
     for (int q = 0; q < 3; ++q) rn[k][q] = rp[k][q] + delta * mv[q];
   }
-  site_at3<S, 1>(rn, rnn, lane);
+  site_at3<G, S, 1>(rn, rnn, lane, gl);
   int bad = 0;
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -460,7 +466,9 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
       if ((diff < 0 ? -diff : diff) / bl > 1e-3) bad = 1;
     }
   }
-  if (__ballot(bad) != 0) {
+  const unsigned long long bal = __ballot(bad);
+  const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << (grp * G));
+  if ((bal & gmask) != 0) {
     // bond renormalisation (src/filament.cpp:904-924): cumulative sums of
     // l u along the chain, tail to head or head to tail by the switch
     int32_t sw = factive ? d.norm[2 * f] : 1;
@@ -482,10 +490,10 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
           run += st3[k][q];
         }
         tot = run;
-        const double incl = sum_scan<64>(tot, lane, lane, -1);
-        const double y = __shfl(incl, lane > 0 ? lane - 1 : lane, 64);
-        const double before = lane > 0 ? y : 0.0;
-        const double r0 = __shfl(rn[0][q], 0, 64);
+        const double incl = sum_scan<G>(tot, lane, gl, -1);
+        const double y = __shfl(incl, gl > 0 ? lane - 1 : lane, 64);
+        const double before = gl > 0 ? y : 0.0;
+        const double r0 = __shfl(rn[0][q], lane - gl, 64);
 #pragma unroll
         for (int k = 0; k < S; ++k)
           if (base + k >= 1) rn[k][q] = r0 + (before + loc[k]);
@@ -498,25 +506,25 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
           loc[k] = run;  // inclusive suffix inside the lane
         }
         tot = run;
-        const double incl = sum_scan<64>(tot, lane, lane, +1);
-        const double y = __shfl(incl, lane < 63 ? lane + 1 : lane, 64);
-        const double after = lane < 63 ? y : 0.0;
+        const double incl = sum_scan<G>(tot, lane, gl, +1);
+        const double y = __shfl(incl, gl < G - 1 ? lane + 1 : lane, 64);
+        const double after = gl < G - 1 ? y : 0.0;
         double pick = 0.0;
 #pragma unroll
         for (int k = 0; k < S; ++k)
           if (k == last % S) pick = rn[k][q];
-        const double rl = __shfl(pick, last / S, 64);
+        const double rl = __shfl(pick, lane - gl + last / S, 64);
 #pragma unroll
         for (int k = 0; k < S; ++k)
           if (base + k < n - 1) rn[k][q] = rl - (after + loc[k]);
       }
     }
-    site_at3<S, 1>(rn, rnn, lane);
-    if (lane == 0 && factive) {
+    site_at3<G, S, 1>(rn, rnn, lane, gl);
+    if (gl == 0 && factive) {
       d.norm[2 * f] = sw;
       d.norm[2 * f + 1] = 1;
     }
-  } else if (lane == 0 && factive) {
+  } else if (gl == 0 && factive) {
     d.norm[2 * f + 1] += 1;
   }
 
@@ -553,16 +561,26 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
 }
 
 bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list, int bin_count,
-                           int bin_maxn, int precision, int waves, hipStream_t st) {
-  if (precision == SC_PREC_FP64 || bin_maxn <= 64 || bin_maxn > 256) return false;
+                           int bin_maxn, int precision, int waves, int layout, hipStream_t st) {
+  if (precision == SC_PREC_FP64 || bin_maxn > 256) return false;
+  // up to 64 sites: only in the two-sites-per-lane layout
+  if (bin_maxn <= 64 && layout != 1) return false;
   if (bin_count <= 0) return true;
-  dim3 grid((bin_count + waves - 1) / waves), block(64 * waves);
-  if (bin_maxn <= 128)
-    hipLaunchKernelGGL(k_integrate_long<2>, grid, block, 0, st, d, g, bin_list, bin_count,
-                       precision);
-  else
-    hipLaunchKernelGGL(k_integrate_long<4>, grid, block, 0, st, d, g, bin_list, bin_count,
-                       precision);
+  const int G = bin_maxn <= 128 ? bin_maxn / 2 : 64;
+  const int per_block = (64 / G) * waves;
+  dim3 grid((bin_count + per_block - 1) / per_block), block(64 * waves);
+#define SC_LONG(GG, SS)                                                                   \
+  hipLaunchKernelGGL((k_integrate_long<GG, SS>), grid, block, 0, st, d, g, bin_list,     \
+                     bin_count, precision)
+  switch (bin_maxn) {
+    case 8: SC_LONG(4, 2); break;
+    case 16: SC_LONG(8, 2); break;
+    case 32: SC_LONG(16, 2); break;
+    case 64: SC_LONG(32, 2); break;
+    case 128: SC_LONG(64, 2); break;
+    default: SC_LONG(64, 4); break;
+  }
+#undef SC_LONG
   return true;
 }
 

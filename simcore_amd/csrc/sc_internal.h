@@ -100,6 +100,7 @@ struct Tune {
   int pair_block = 256;  // threads per block of the pair kernel
   int pair_xcd = 0;      // 1: consecutive bond ranges stay on one XCD
   int int_waves = 1;     // waves per block of the register integrator
+  int int_layout = 0;    // up to 64 sites: 0 one site per lane, 1 two
 };
 
 constexpr int kScanBlock = 256;
@@ -110,7 +111,7 @@ enum Guard { kGuardStep = 0, kGuardRebuild = 1, kGuardXlink = 2 };
 
 // Launchers (sc_kernels.hip).  All are asynchronous on `st`.
 void launch_decide(const Dev &d, const Geo &g, hipStream_t st);
-void launch_commit(const Dev &d, hipStream_t st);
+void launch_commit(const Dev &d, const Geo &g, hipStream_t st);
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st);
 void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t,
                  hipStream_t st);
@@ -122,10 +123,11 @@ void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
 bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,
                           int bin_count, int bin_maxn, int precision, int waves,
                           hipStream_t st);
-// the same for 65..256 sites (sc_integrate_long.hip)
+// several sites per lane (sc_integrate_long.hip): 65..256 sites always,
+// up to 64 sites when layout == 1
 bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list,
                            int bin_count, int bin_maxn, int precision, int waves,
-                           hipStream_t st);
+                           int layout, hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
 // crosslinks (sc_crosslink.hip)

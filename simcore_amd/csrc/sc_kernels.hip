@@ -6,11 +6,12 @@
 // IEEE correctly rounded on gfx950 (v_div_scale/fmas/fixup sequence and
 // the refined v_sqrt_f64 expansion).
 //
-// One step, in stream order (all guarded by the device control block):
-//   decide -> [rebuild: keys, scan, scatter, run sort, count, scan, cap,
-//              emit, scan, incidence fill, incidence sort, zero dr]
-//          -> pair+scatter -> integrate (one launch per length bin)
-//          -> commit
+// One step, in stream order (all guarded by the device control block,
+// which holds the decision for the step about to run):
+//   [rebuild: keys, scan, scatter, run sort, count, scan, cap, emit, scan,
+//    incidence fill, incidence sort, zero dr]
+//   -> pair+scatter -> integrate (one launch per length bin)
+//   -> commit (and decide the next step)
 #include <hip/hip_runtime.h>
 
 #include "sc_internal.h"
@@ -23,18 +24,18 @@ namespace sc {
 // ---------------------------------------------------------------------------
 
 // Replaces InteractionManager::CheckUpdateInteractions / GetDrMax
-// (src/interaction_manager.cpp:284-312): the accumulator of a full step is
-// compared with 0.25*cell_len^2 at the next (odd) step; the mesh skips the
-// update after a midstep (src/mesh.cpp:533-536).
-__global__ void k_decide(Ctl *c, double thr) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (c->abort) return;
-  int64_t k = c->i_step + 1;
+// (src/interaction_manager.cpp:284-312) for the step after i_step: the
+// accumulator of a full step is compared with 0.25*cell_len^2 at the next
+// (odd) step; the mesh skips the update after a midstep
+// (src/mesh.cpp:533-536).  A full step clears the accumulator it is about
+// to fill.
+__device__ __forceinline__ void decide_next(Ctl *c, double thr) {
+  const int64_t k = c->i_step + 1;
   c->cur_step = k;
   c->midstep = (k & 1) ? 1 : 0;
   int rb = 0;
   if ((k & 1) && k >= 3) {
-    double drm = __longlong_as_double((long long)c->dr_bits);
+    const double drm = __longlong_as_double((long long)c->dr_bits);
     if (drm > 10000) {  // include/simcore/species.hpp:114-118
       c->abort = 1;
       c->err = SC_ERR_UNPHYSICAL;
@@ -42,14 +43,23 @@ __global__ void k_decide(Ctl *c, double thr) {
     }
     if (drm > thr) rb = 1;
   }
-  if (!(k & 1)) c->dr_bits = 0ull;  // the full step about to run re-accumulates
+  if (!(k & 1)) c->dr_bits = 0ull;
   c->rebuild = rb;
 }
 
-__global__ void k_commit(Ctl *c) {
+// standalone decision: after loading or after the host changed the state
+__global__ void k_decide(Ctl *c, double thr) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->abort) return;
+  decide_next(c, thr);
+}
+
+// end of a step: commit it and decide the next one (one launch for both)
+__global__ void k_commit(Ctl *c, double thr) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (c->abort) return;
   c->i_step = c->cur_step;
+  decide_next(c, thr);
 }
 
 __global__ void k_zero_i32(int32_t *p, int64_t n) {
@@ -265,64 +275,47 @@ __device__ __forceinline__ int32_t shell_key(int n, int x, int y, int z, int q) 
   return ((int32_t)xp * n + yp) * n + zp;
 }
 
-// Sixteen lanes per sorted position p: for the first position of an occupied
-// cell (its "home" run [p, e)), lane 13 covers the self pairs and lanes
-// 0..12 the 13 neighbour cells; the other positions contribute nothing.
-struct HomeRun {
-  bool leader;
-  int32_t k, e;
+// The 13 neighbour runs of home cell k, loaded up front so that the 26
+// lookups are in flight together.
+struct Shell {
+  int32_t o0[13], o1[13];
 };
 
-__device__ __forceinline__ HomeRun home_run(const Dev &d, const Geo &g, int64_t p) {
-  HomeRun h{false, 0, 0};
-  if (p < g.nb) {
-    h.k = d.skey[p];
-    h.leader = d.cell_start[h.k] == p;
-    h.e = d.cell_start[h.k + 1];
-  }
-  return h;
-}
-
-__device__ __forceinline__ void seg_range(const Dev &d, const Geo &g, const HomeRun &h, int q,
-                                          int32_t *o0, int32_t *o1) {
+__device__ __forceinline__ void load_shell(const Dev &d, const Geo &g, int32_t k, Shell *sh) {
   const int n = g.n1d;
-  const int x = h.k / (n * n), y = (h.k / n) % n, z = h.k % n;
-  const int32_t nb = shell_key(n, x, y, z, q);
-  *o0 = d.cell_start[nb];
-  *o1 = d.cell_start[nb + 1];
-}
-
-__device__ __forceinline__ int32_t seg_count(const Dev &d, const Geo &g, const HomeRun &h,
-                                             int64_t p, int q) {
-  if (!h.leader || q > 13) return 0;
-  int32_t cnt = 0;
-  if (q == 13) {
-    for (int32_t i = (int32_t)p; i < h.e; ++i)
-      for (int32_t j = i + 1; j < h.e; ++j) cnt += !excluded(d.bond_fil, d.sorted[i], d.sorted[j]);
-    return cnt;
+  const int x = k / (n * n), y = (k / n) % n, z = k % n;
+#pragma unroll
+  for (int q = 0; q < 13; ++q) {
+    const int32_t nb = shell_key(n, x, y, z, q);
+    sh->o0[q] = d.cell_start[nb];
+    sh->o1[q] = d.cell_start[nb + 1];
   }
-  int32_t o0, o1;
-  seg_range(d, g, h, q, &o0, &o1);
-  for (int32_t i = (int32_t)p; i < h.e; ++i) {
-    const int32_t bi = d.sorted[i];
-    for (int32_t j = o0; j < o1; ++j) cnt += !excluded(d.bond_fil, bi, d.sorted[j]);
-  }
-  return cnt;
 }
 
 // Candidate count of each occupied home cell (Cell::MakePairs,
 // src/cell.cpp:14-70), excluded pairs already removed (the reference culls
-// them on the rebuild step, src/interaction_manager.cpp:485-497).
+// them on the rebuild step, src/interaction_manager.cpp:485-497).  One
+// thread per sorted position; the first position of a cell's run counts.
 __global__ __launch_bounds__(256) void k_count(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t p = t >> 4;
-  const int q = (int)(t & 15);
-  const HomeRun h = home_run(d, g, p);
-  int32_t c = seg_count(d, g, h, p, q);
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.nb) return;
+  const int32_t k = d.skey[p];
+  int32_t cnt = 0;
+  if (d.cell_start[k] == p) {
+    const int32_t e = d.cell_start[k + 1];
+    Shell sh;
+    load_shell(d, g, k, &sh);
+    for (int32_t i = (int32_t)p; i < e; ++i)
+      for (int32_t j = i + 1; j < e; ++j) cnt += !excluded(d.bond_fil, d.sorted[i], d.sorted[j]);
+    for (int32_t i = (int32_t)p; i < e; ++i) {
+      const int32_t bi = d.sorted[i];
 #pragma unroll
-  for (int o = 8; o > 0; o >>= 1) c += __shfl_xor(c, o, 16);
-  if (q == 0 && p < g.nb) d.ccount[p] = c;
+      for (int q = 0; q < 13; ++q)
+        for (int32_t j = sh.o0[q]; j < sh.o1[q]; ++j) cnt += !excluded(d.bond_fil, bi, d.sorted[j]);
+    }
+  }
+  d.ccount[p] = cnt;
 }
 
 __global__ void k_check_cap(Dev d, Geo g) {
@@ -340,27 +333,17 @@ __global__ void k_check_cap(Dev d, Geo g) {
 
 // Writes the candidate list in MakePairs order: home cells ascending; in
 // each, the self pairs (i<j) then, for each of the 13 half-shell
-// neighbours in AssignCellNeighbors order, all home x neighbour pairs.  The
-// sixteen lanes of a home run write their segments concurrently at offsets
-// from a scan of the segment counts.
+// neighbours in AssignCellNeighbors order, all home x neighbour pairs.
 __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t p = t >> 4;
-  const int q = (int)(t & 15);
-  const HomeRun h = home_run(d, g, p);
-  const int32_t c = seg_count(d, g, h, p, q);
-  // segment order: self (lane 13) first, then lanes 0..12
-  const int32_t cq = q == 13 ? 0 : c;
-  int32_t incl = cq;
-#pragma unroll
-  for (int o = 1; o < 16; o <<= 1) {
-    const int32_t y = __shfl_up(incl, o, 16);
-    if (q >= o) incl += y;
-  }
-  const int32_t c13 = __shfl(c, 13, 16);
-  if (!h.leader || q > 13 || c == 0) return;
-  int32_t w = d.coff[p] + (q == 13 ? 0 : c13 + incl - cq);
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.nb) return;
+  const int32_t k = d.skey[p];
+  if (d.cell_start[k] != p) return;
+  const int32_t e = d.cell_start[k + 1];
+  Shell sh;
+  load_shell(d, g, k, &sh);
+  int32_t w = d.coff[p];
   auto put = [&](int32_t b1, int32_t b2) {
     d.pairs[w] = make_int2(b1, b2);
     if (prec == SC_PREC_FP64) {
@@ -372,23 +355,20 @@ __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
     atomicAdd(&d.deg[b2], 1);
     ++w;
   };
-  if (q == 13) {
-    for (int32_t i = (int32_t)p; i < h.e; ++i)
-      for (int32_t j = i + 1; j < h.e; ++j) {
-        const int32_t bi = d.sorted[i], bj = d.sorted[j];
-        if (!excluded(d.bond_fil, bi, bj)) put(bi, bj);
-      }
-    return;
-  }
-  int32_t o0, o1;
-  seg_range(d, g, h, q, &o0, &o1);
-  for (int32_t i = (int32_t)p; i < h.e; ++i) {
-    const int32_t bi = d.sorted[i];
-    for (int32_t j = o0; j < o1; ++j) {
-      const int32_t bj = d.sorted[j];
+  for (int32_t i = (int32_t)p; i < e; ++i)
+    for (int32_t j = i + 1; j < e; ++j) {
+      const int32_t bi = d.sorted[i], bj = d.sorted[j];
       if (!excluded(d.bond_fil, bi, bj)) put(bi, bj);
     }
-  }
+#pragma unroll
+  for (int q = 0; q < 13; ++q)
+    for (int32_t i = (int32_t)p; i < e; ++i) {
+      const int32_t bi = d.sorted[i];
+      for (int32_t j = sh.o0[q]; j < sh.o1[q]; ++j) {
+        const int32_t bj = d.sorted[j];
+        if (!excluded(d.bond_fil, bi, bj)) put(bi, bj);
+      }
+    }
 }
 
 // Incidence CSR: every candidate appears under obj1 (role 0) and obj2
@@ -448,11 +428,10 @@ void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
               kGuardRebuild, st);
   hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(bs), 0, st, d, g);
   hipLaunchKernelGGL(k_run_sort, dim3(gb), dim3(bs), 0, st, d, g);
-  const unsigned gb16 = nblocks(16 * g.nb, bs);
-  hipLaunchKernelGGL(k_count, dim3(gb16), dim3(bs), 0, st, d, g);
+  hipLaunchKernelGGL(k_count, dim3(gb), dim3(bs), 0, st, d, g);
   launch_scan(d.ccount, d.coff, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
   hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, st, d, g);
-  hipLaunchKernelGGL(k_emit, dim3(gb16), dim3(bs), 0, st, d, g,
+  hipLaunchKernelGGL(k_emit, dim3(gb), dim3(bs), 0, st, d, g,
                      d.slot64 ? (int)SC_PREC_FP64 : (int)SC_PREC_MIXED);
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
   hipLaunchKernelGGL(k_inc_fill, dim3(2048), dim3(bs), 0, st, d, g);
@@ -505,8 +484,8 @@ void launch_decide(const Dev &d, const Geo &g, hipStream_t st) {
   hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, st, d.ctl, g.thr);
 }
 
-void launch_commit(const Dev &d, hipStream_t st) {
-  hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, d.ctl);
+void launch_commit(const Dev &d, const Geo &g, hipStream_t st) {
+  hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, d.ctl, g.thr);
 }
 
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t st) {

@@ -266,6 +266,11 @@ class FilamentSystem:
                "sc_get_crosslinks")
         return b.reshape(-1, 2), la.reshape(-1, 2), f.reshape(-1, 9), ev.value
 
+    def set_launch_shape(self, pair_block=256, pair_xcd=0, integrate_waves=1,
+                         integrate_layout=0):
+        _check(lib.sc_set_launch_shape(self.h, pair_block, pair_xcd, integrate_waves,
+                                       integrate_layout), self.h, "sc_set_launch_shape")
+
     def autotune(self, steps=20):
         """Pick the fastest result-invariant launch shapes (sc_autotune)."""
         rep = ScTuneReport()

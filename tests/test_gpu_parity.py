@@ -321,9 +321,35 @@ def test_autotune_leaves_trajectory_unchanged(precision):
     b.step(31)
     rep = b.autotune(steps=6)
     print(rep)
-    assert rep["n_trials"] == 18 and rep["step_ms_best"] <= rep["step_ms_start"]
+    assert rep["n_trials"] == 36 and rep["step_ms_best"] <= rep["step_ms_start"]
     assert b.counters()["i_step"] == 31
     a.step(40)
     b.step(40)
     for x, y in zip(a.sites(), b.sites()):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("kind", ["short", "long"])
+def test_two_sites_per_lane_layout_within_tolerance(kind):
+    """The integrator layout with two sites per lane for filaments of up to
+    64 sites, against the CPU path at the mixed tolerances, and with shapes
+    that leave results unchanged against the default layout's run
+    (launch shapes: block sizes and waves per block)."""
+    print(f"case: {kind} mixed, two sites per lane, 1 and 100 steps", flush=True)
+    from tests.helpers import long_system, uneven_system
+    p, fil = uneven_system() if kind == "short" else long_system()
+    dev = sc.FilamentSystem(p, fil, precision="mixed", use_graph=True)
+    dev.set_launch_shape(pair_block=128, pair_xcd=1, integrate_waves=2, integrate_layout=1)
+    ora = OracleSystem(p, fil)
+    dev.step(1)
+    assert ora.step(1) == 0
+    d1 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 1 step", d1)
+    assert d1 <= POS_TOL_SHORT
+    np.testing.assert_allclose(dev.tensions(), ora.tensions(), rtol=1e-5,
+                               atol=1e-5 * np.abs(ora.tensions()).max())
+    dev.step(99)
+    assert ora.step(99) == 0
+    d100 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
+    print("max |dr| after 100 steps", d100)
+    assert d100 <= 1e-5
