@@ -180,7 +180,8 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
  * kernel (128/256/512), its XCD-contiguous block order on or off,
  * waves per block of the integrator (1/2/4) and its lane layout for
  * filaments of up to 64 sites (one or two sites per lane) -- over `steps`
- * steps each from a checkpoint of the current state, keeps the fastest and
+ * steps each from a checkpoint of the current state (pair_block 0 in the
+ * report: the load-balanced pair kernel), keeps the fastest and
  * restores the state, so the trajectory is the one without tuning.  The
  * cell size is not tuned: in the reference it sets the rebuild threshold
  * and the candidate set (src/cell_list.cpp:9-64), i.e. it is part of the
@@ -190,7 +191,8 @@ typedef struct sc_tune_report {
   double step_ms_start, step_ms_best;
 } sc_tune_report;
 int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report);
-/* Sets the launch shapes directly: pair threads per block (64/128/256/512),
+/* Sets the launch shapes directly: pair threads per block (64/128/256/512;
+ * 0 selects the load-balanced pair kernel, one wave per 64 bonds),
  * XCD-contiguous pair blocks (0/1), integrator waves per block (1/2/4) and,
  * for filaments of up to 64 sites, one (0) or two (1) sites per lane. */
 int sc_set_launch_shape(sc_handle h, int32_t pair_block, int32_t pair_xcd,

@@ -1163,7 +1163,9 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   int trials = 0;
   rc = trial(start, &start_ms);
   best_ms = start_ms;
-  const int blocks[] = {128, 256, 512};
+  // pair kernel: per-bond with 128/256/512 threads per block, or the
+  // balanced one-wave-per-64-bonds variant (block size not used)
+  const int blocks[] = {128, 256, 512, 0};
   const int waves[] = {1, 2, 4};
   for (int bs : blocks)
     for (int x = 0; x < 2 && rc == SC_OK; ++x)
@@ -1171,7 +1173,8 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
         for (int lay = 0; lay < 2; ++lay) {
         if (rc) break;
         sc::Tune t;
-        t.pair_block = bs;
+        t.pair_bal = bs == 0 ? 1 : 0;
+        t.pair_block = bs == 0 ? 256 : bs;
         t.pair_xcd = x;
         t.int_waves = w;
         t.int_layout = lay;
@@ -1193,7 +1196,7 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   if (rc) return rc;
   if (rc2) return rc2;
   if (report) {
-    report->pair_block = best.pair_block;
+    report->pair_block = best.pair_bal ? 0 : best.pair_block;
     report->pair_xcd = best.pair_xcd;
     report->integrate_waves = best.int_waves;
     report->integrate_layout = best.int_layout;
@@ -1208,13 +1211,15 @@ int sc_set_launch_shape(sc_handle h, int32_t pair_block, int32_t pair_xcd,
                         int32_t integrate_waves, int32_t integrate_layout) {
   int rc = check_handle(h, false);
   if (rc) return rc;
-  if ((pair_block != 64 && pair_block != 128 && pair_block != 256 && pair_block != 512) ||
+  if ((pair_block != 0 && pair_block != 64 && pair_block != 128 && pair_block != 256 &&
+       pair_block != 512) ||
       (pair_xcd != 0 && pair_xcd != 1) ||
       (integrate_waves != 1 && integrate_waves != 2 && integrate_waves != 4) ||
       (integrate_layout != 0 && integrate_layout != 1))
     return fail(h, SC_ERR_ARG, "unsupported launch shape");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
-  h->tune.pair_block = pair_block;
+  h->tune.pair_bal = pair_block == 0 ? 1 : 0;
+  h->tune.pair_block = pair_block == 0 ? 256 : pair_block;
   h->tune.pair_xcd = pair_xcd;
   h->tune.int_waves = integrate_waves;
   h->tune.int_layout = integrate_layout;

@@ -99,6 +99,7 @@ struct Dev {
 struct Tune {
   int pair_block = 256;  // threads per block of the pair kernel
   int pair_xcd = 0;      // 1: consecutive bond ranges stay on one XCD
+  int pair_bal = 0;      // 1: incidences dealt evenly over a wave's lanes
   int int_waves = 1;     // waves per block of the register integrator
   int int_layout = 0;    // up to 64 sites: 0 one site per lane, 1 two
 };

@@ -125,6 +125,109 @@ __device__ __forceinline__ int64_t logical_block(int xcd) {
   return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// Contribution of one incidence {candidate << 1 | role, partner} to its
+// bond ("own"): the pair computed with the candidate's own (obj1, obj2)
+// orientation, the sticky slot updated when in range, then +F and
+// contact1 x F for obj1, -F and -(contact2 x F) for obj2
+// (src/interaction_manager.cpp:448-517).  Both owners of a candidate run
+// the same instructions on the same inputs and obtain the same bits.
+template <bool MIXED>
+__device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const BondRec &own,
+                                             int2 ent, const BondRec &oth, double *cf,
+                                             double *ct) {
+  const int32_t c = ent.x >> 1;
+  const bool second = (ent.x & 1) != 0;  // this bond is obj2 of candidate c
+  const BondRec &r1 = second ? oth : own;
+  const BondRec &r2 = second ? own : oth;
+  double fc[3], lam, mu;
+  if (!MIXED) {
+    double dr[3], dr2;
+    sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
+    bool wrote = false;
+    if (!(dr2 > g.rcut2)) {
+      double f[3];
+      if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
+        fc[0] = f[0];
+        fc[1] = f[1];
+        fc[2] = f[2];
+        d.slot64[c] = make_double4(f[0], f[1], f[2], 0.0);
+        wrote = true;
+      }
+    }
+    if (!wrote) {
+      const double4 o = d.slot64[c];
+      fc[0] = o.x;
+      fc[1] = o.y;
+      fc[2] = o.z;
+    }
+  } else {
+    float dr32[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double ds = r2.s[k] - r1.s[k];
+      ds -= (double)nint(ds);
+      dr32[k] = (float)(g.h[4 * k] * ds);
+    }
+    const float u1[3] = {(float)r1.u[0], (float)r1.u[1], (float)r1.u[2]};
+    const float u2[3] = {(float)r2.u[0], (float)r2.u[1], (float)r2.u[2]};
+    float rmin[3], rm2, lf, mf;
+    bool wrote = false;
+    if (sphero32(dr32, u1, (float)(0.5 * r1.len), u2, (float)(0.5 * r2.len), rmin, &rm2, &lf,
+                 &mf)) {
+      lam = lf;
+      mu = mf;
+      if (!(rm2 > (float)g.rcut2)) {
+        const float rinv = 1.0f / sqrtf(rm2);
+        const float rinv2 = rinv * rinv;
+        const float r6 = rinv2 * rinv2 * rinv2;
+        float ffac = -(12.0f * (float)g.c12 * r6 - 6.0f * (float)g.c6) * r6 * rinv;
+        const float fcut = (float)g.fcut;
+        if (fabs_(ffac) > fcut) ffac = (ffac >= 0.f ? 1.f : -1.f) * fcut;
+        const float s = ffac * rinv;
+        const float4 f = make_float4(s * rmin[0], s * rmin[1], s * rmin[2], 0.f);
+        d.slot32[c] = f;
+        fc[0] = f.x;
+        fc[1] = f.y;
+        fc[2] = f.z;
+        wrote = true;
+      }
+    } else {
+      double dr[3], dr2;
+      sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
+      if (!(dr2 > g.rcut2)) {
+        double f[3];
+        if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
+          const float4 f4 = make_float4((float)f[0], (float)f[1], (float)f[2], 0.f);
+          d.slot32[c] = f4;
+          fc[0] = f4.x;
+          fc[1] = f4.y;
+          fc[2] = f4.z;
+          wrote = true;
+        }
+      }
+    }
+    if (!wrote) {
+      const float4 o = d.slot32[c];
+      fc[0] = o.x;
+      fc[1] = o.y;
+      fc[2] = o.z;
+    }
+  }
+  // contact of this bond and the torque cross product
+  // (src/interaction_manager.cpp:469-470, src/minimum_distance.cpp:339-342)
+  double con[3], t[3];
+  const double sc = second ? mu : lam;
+  for (int k = 0; k < 3; ++k) con[k] = sc * own.u[k];
+  cross3(con, fc, t);
+  if (!second) {
+    for (int k = 0; k < 3; ++k) cf[k] = fc[k];
+    for (int k = 0; k < 3; ++k) ct[k] = t[k];
+  } else {
+    for (int k = 0; k < 3; ++k) cf[k] = -fc[k];
+    for (int k = 0; k < 3; ++k) ct[k] = -t[k];
+  }
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   if (d.ctl->abort) return;
@@ -162,97 +265,10 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
       ent_n = d.inc[q + 1];
       oth_n = load_rec(d.br, ent_n.y);
     }
-    const int32_t c = ent.x >> 1;
-    const bool second = (ent.x & 1) != 0;  // this bond is obj2 of candidate c
-    const BondRec &r1 = second ? oth : own;
-    const BondRec &r2 = second ? own : oth;
-    double fc[3], lam, mu;
-    if (!MIXED) {
-      double dr[3], dr2;
-      sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
-      bool wrote = false;
-      if (!(dr2 > g.rcut2)) {
-        double f[3];
-        if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
-          fc[0] = f[0];
-          fc[1] = f[1];
-          fc[2] = f[2];
-          d.slot64[c] = make_double4(f[0], f[1], f[2], 0.0);
-          wrote = true;
-        }
-      }
-      if (!wrote) {
-        const double4 o = d.slot64[c];
-        fc[0] = o.x;
-        fc[1] = o.y;
-        fc[2] = o.z;
-      }
-    } else {
-      float dr32[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double ds = r2.s[k] - r1.s[k];
-        ds -= (double)nint(ds);
-        dr32[k] = (float)(g.h[4 * k] * ds);
-      }
-      const float u1[3] = {(float)r1.u[0], (float)r1.u[1], (float)r1.u[2]};
-      const float u2[3] = {(float)r2.u[0], (float)r2.u[1], (float)r2.u[2]};
-      float rmin[3], rm2, lf, mf;
-      bool wrote = false;
-      if (sphero32(dr32, u1, (float)(0.5 * r1.len), u2, (float)(0.5 * r2.len), rmin, &rm2, &lf,
-                   &mf)) {
-        lam = lf;
-        mu = mf;
-        if (!(rm2 > (float)g.rcut2)) {
-          const float rinv = 1.0f / sqrtf(rm2);
-          const float rinv2 = rinv * rinv;
-          const float r6 = rinv2 * rinv2 * rinv2;
-          float ffac = -(12.0f * (float)g.c12 * r6 - 6.0f * (float)g.c6) * r6 * rinv;
-          const float fcut = (float)g.fcut;
-          if (fabs_(ffac) > fcut) ffac = (ffac >= 0.f ? 1.f : -1.f) * fcut;
-          const float s = ffac * rinv;
-          const float4 f = make_float4(s * rmin[0], s * rmin[1], s * rmin[2], 0.f);
-          d.slot32[c] = f;
-          fc[0] = f.x;
-          fc[1] = f.y;
-          fc[2] = f.z;
-          wrote = true;
-        }
-      } else {
-        double dr[3], dr2;
-        sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
-        if (!(dr2 > g.rcut2)) {
-          double f[3];
-          if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
-            const float4 f4 = make_float4((float)f[0], (float)f[1], (float)f[2], 0.f);
-            d.slot32[c] = f4;
-            fc[0] = f4.x;
-            fc[1] = f4.y;
-            fc[2] = f4.z;
-            wrote = true;
-          }
-        }
-      }
-      if (!wrote) {
-        const float4 o = d.slot32[c];
-        fc[0] = o.x;
-        fc[1] = o.y;
-        fc[2] = o.z;
-      }
-    }
-    // contact of this bond and the torque cross product
-    // (src/interaction_manager.cpp:469-470, src/minimum_distance.cpp:339-342)
-    double con[3], t[3];
-    const double sc = second ? mu : lam;
-    for (int k = 0; k < 3; ++k) con[k] = sc * own.u[k];
-    cross3(con, fc, t);
-    if (!second) {
-      for (int k = 0; k < 3; ++k) F[k] += fc[k];
-      for (int k = 0; k < 3; ++k) T[k] += t[k];
-    } else {
-      for (int k = 0; k < 3; ++k) F[k] -= fc[k];
-      for (int k = 0; k < 3; ++k) T[k] -= t[k];
-    }
+    double cf[3], ct[3];
+    pair_contrib<MIXED>(d, g, own, ent, oth, cf, ct);
+    for (int k = 0; k < 3; ++k) F[k] += cf[k];
+    for (int k = 0; k < 3; ++k) T[k] += ct[k];
   }
   for (int k = 0; k < 3; ++k) {
     d.bf[k * nb + b] = F[k];
@@ -260,7 +276,91 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   }
 }
 
+// Load-balanced variant: one wavefront per 64 bonds.  A bond's candidate
+// count varies (0 to ~10), and in the per-bond kernel a wave runs as long
+// as its busiest lane.  Here the wave's incidences (contiguous in the CSR)
+// are dealt out to the lanes in chunks of kChunk, the contributions staged
+// in LDS, and every lane then adds its own bond's contributions in
+// incidence order -- the same values summed in the same order as the
+// per-bond kernel, so the results are identical bit for bit.
+constexpr int kChunk = 128;
+
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
+  __shared__ double cF[kChunk][3], cT[kChunk][3];
+  __shared__ int32_t qs[65];
+  if (d.ctl->abort) return;
+  const int64_t nb = g.nb;
+  const int lane = threadIdx.x;
+  const int64_t b0 = logical_block(xcd) * 64;
+  const int64_t b = b0 + lane;
+  const bool valid = b < nb;
+  qs[lane] = d.inc_start[valid ? b : nb];
+  if (lane == 0) qs[64] = d.inc_start[b0 + 64 <= nb ? b0 + 64 : nb];
+  __syncthreads();
+  const int32_t Q0 = qs[0], Q1 = qs[64];
+  double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+  if (valid && g.nx > 0) {
+    const int32_t x0 = d.xinc_start[b], x1 = d.xinc_start[b + 1];
+    for (int32_t q = x0; q < x1; ++q) {
+      const int32_t e = d.xinc[q];
+      const double *o = d.xf + 9 * (int64_t)(e >> 1);
+      const int a = e & 1;
+      for (int k = 0; k < 3; ++k) F[k] += a == 0 ? o[k] : -o[k];
+      for (int k = 0; k < 3; ++k) T[k] += o[3 + 3 * a + k];
+    }
+  }
+  const int32_t my0 = qs[lane], my1 = qs[lane + 1];
+  for (int32_t cs = Q0; cs < Q1; cs += kChunk) {
+#pragma unroll
+    for (int kk = 0; kk < kChunk / 64; ++kk) {
+      const int32_t q = cs + kk * 64 + lane;
+      if (q < Q1) {
+        // owning bond: the last l with qs[l] <= q
+        int lo = 0, hi = 64;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (qs[mid] <= q)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        const BondRec own = load_rec(d.br, b0 + lo);
+        const int2 ent = d.inc[q];
+        const BondRec oth = load_rec(d.br, ent.y);
+        double cf[3], ct[3];
+        pair_contrib<MIXED>(d, g, own, ent, oth, cf, ct);
+        for (int k = 0; k < 3; ++k) {
+          cF[q - cs][k] = cf[k];
+          cT[q - cs][k] = ct[k];
+        }
+      }
+    }
+    __syncthreads();
+    const int32_t e = my1 < cs + kChunk ? my1 : cs + kChunk;
+    for (int32_t q = my0 > cs ? my0 : cs; q < e; ++q) {
+      for (int k = 0; k < 3; ++k) F[k] += cF[q - cs][k];
+      for (int k = 0; k < 3; ++k) T[k] += cT[q - cs][k];
+    }
+    __syncthreads();
+  }
+  if (valid)
+    for (int k = 0; k < 3; ++k) {
+      d.bf[k * nb + b] = F[k];
+      d.bt[k * nb + b] = T[k];
+    }
+}
+
 void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t, hipStream_t st) {
+  if (t.pair_bal) {
+    const int64_t blocks = (g.nb + 63) / 64;
+    const dim3 grid((unsigned)(blocks > 0 ? blocks : 1)), block(64);
+    if (precision == SC_PREC_FP64)
+      hipLaunchKernelGGL(k_pair_bal<false>, grid, block, 0, st, d, g, t.pair_xcd);
+    else
+      hipLaunchKernelGGL(k_pair_bal<true>, grid, block, 0, st, d, g, t.pair_xcd);
+    return;
+  }
   const int bs = t.pair_block;
   const int64_t blocks = (g.nb + bs - 1) / bs;
   const dim3 grid((unsigned)(blocks > 0 ? blocks : 1)), block(bs);
