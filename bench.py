@@ -268,6 +268,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch-shape", default="",
+                    help="pair_block,pair_xcd,integrate_waves,integrate_layout instead of "
+                         "autotuning (pair_block 0: load-balanced pair kernel)")
     ap.add_argument("--caller-order", action="store_true",
                     help="keep filaments in generation order (no spatial storage order)")
     # serving workload (--workload c4)
@@ -313,7 +316,12 @@ def main():
     dev.step(args.warmup)
     log(f"[bench] warmup {args.warmup} steps done, counters {dev.counters()}")
     tune = None
-    if not args.no_autotune:
+    if args.launch_shape:
+        pb, px, iw, il = (int(v) for v in args.launch_shape.split(","))
+        dev.set_launch_shape(pb, px, iw, il)
+        tune = {"pair_block": pb, "pair_xcd": px, "integrate_waves": iw,
+                "integrate_layout": il, "set": "fixed"}
+    elif not args.no_autotune:
         # launch shapes that leave results unchanged, chosen from a
         # checkpoint of the warmed-up state (the state is restored)
         tune = dev.autotune(steps=20)

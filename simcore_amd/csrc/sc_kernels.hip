@@ -138,30 +138,9 @@ __device__ void tile_scan(int *tile, int *wsum, int carry_in, int *total) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_mid(int32_t *bsum,
-                                                         int64_t nblk,
-                                                         int32_t *out,
-                                                         int64_t n,
-                                                         const Ctl *ctl,
-                                                         int guard) {
-  if (guard_off(ctl, guard)) return;
-  __shared__ int tile[kScanTile];
-  __shared__ int wsum[kScanBlock / 64];
-  int carry = 0;
-  for (int64_t b0 = 0; b0 < nblk; b0 += kScanTile) {
-    for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
-      tile[j] = (b0 + j < nblk) ? bsum[b0 + j] : 0;
-    __syncthreads();
-    int tot;
-    tile_scan(tile, wsum, carry, &tot);
-    for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
-      if (b0 + j < nblk) bsum[b0 + j] = tile[j];
-    carry += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[n] = carry;
-}
-
+// Second pass: each tile adds up the sums of the tiles before it (at most a
+// few thousand, read from L2) and scans itself; the last tile also writes
+// the total.  Two launches per scan.
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
                                                           int64_t n,
                                                           const int32_t *bsum,
@@ -171,14 +150,19 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
   if (guard_off(ctl, guard)) return;
   __shared__ int tile[kScanTile];
   __shared__ int wsum[kScanBlock / 64];
+  int part = 0;
+  for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += kScanBlock) part += bsum[i];
+  int carry;
+  block_excl_scan256(part, &carry, wsum);
   int64_t base = (int64_t)blockIdx.x * kScanTile;
   for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
     tile[j] = (base + j < n) ? in[base + j] : 0;
   __syncthreads();
   int tot;
-  tile_scan(tile, wsum, bsum[blockIdx.x], &tot);
+  tile_scan(tile, wsum, carry, &tot);
   for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
     if (base + j < n) out[base + j] = tile[j];
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry + tot;
 }
 
 int64_t scan_tmp_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
@@ -189,8 +173,6 @@ void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
   if (nblk == 0) nblk = 1;
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblk), dim3(kScanBlock), 0,
                      st, in, n, tmp, ctl, guard);
-  hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kScanBlock), 0, st, tmp, nblk,
-                     out, n, ctl, guard);
   hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nblk), dim3(kScanBlock), 0,
                      st, in, n, tmp, out, ctl, guard);
 }
@@ -318,24 +300,26 @@ __global__ __launch_bounds__(256) void k_count(Dev d, Geo g) {
   d.ccount[p] = cnt;
 }
 
-__global__ void k_check_cap(Dev d, Geo g) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t need = d.coff[g.nb];
-  d.ctl->nc_need = need;
-  if (need > g.cap) {
-    d.ctl->abort = 1;
-    d.ctl->overflow = 1;
-  } else {
-    d.ctl->nc = need;
-  }
-}
 
 // Writes the candidate list in MakePairs order: home cells ascending; in
 // each, the self pairs (i<j) then, for each of the 13 half-shell
 // neighbours in AssignCellNeighbors order, all home x neighbour pairs.
 __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
+  // capacity check (every thread reads the total; one records it): an
+  // overflowing list aborts the step, the host grows the buffers and
+  // re-runs it
+  const int64_t need = d.coff[g.nb];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    d.ctl->nc_need = need;
+    if (need > g.cap) {
+      d.ctl->abort = 1;
+      d.ctl->overflow = 1;
+    } else {
+      d.ctl->nc = need;
+    }
+  }
+  if (need > g.cap) return;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.nb) return;
   const int32_t k = d.skey[p];
@@ -387,32 +371,29 @@ __global__ void k_inc_fill(Dev d, Geo g) {
 }
 
 // Per-bond incidence lists in candidate order, so the reduction adds the
-// contributions in the order ApplyPairInteractions does.
+// contributions in the order ApplyPairInteractions does; the same launch
+// restarts the displacement bookkeeping, InteractionManager::ZeroDrTot ->
+// Mesh/Object::ZeroDrTot (src/interaction_manager.cpp:316-321,
+// src/mesh.cpp:546-551).
 __global__ void k_inc_sort(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= g.nb) return;
-  int32_t s = d.inc_start[b], e = d.inc_start[b + 1];
-  for (int32_t i = s + 1; i < e; ++i) {
-    const int2 v = d.inc[i];
-    int32_t j = i - 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) d.ctl->n_rebuilds += 1;
+  if (i < g.ns) {
+    for (int q = 0; q < 3; ++q) d.dr0[q * g.ns + i] = d.r[q * g.ns + i];
+    d.acc[i] = 0.0;
+  }
+  if (i >= g.nb) return;
+  const int32_t s = d.inc_start[i], e = d.inc_start[i + 1];
+  for (int32_t a = s + 1; a < e; ++a) {
+    const int2 v = d.inc[a];
+    int32_t j = a - 1;
     while (j >= s && d.inc[j].x > v.x) {
       d.inc[j + 1] = d.inc[j];
       --j;
     }
     d.inc[j + 1] = v;
   }
-}
-
-// InteractionManager::ZeroDrTot -> Mesh/Object::ZeroDrTot
-// (src/interaction_manager.cpp:316-321, src/mesh.cpp:546-551).
-__global__ void k_zero_dr(Dev d, Geo g) {
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) d.ctl->n_rebuilds += 1;
-  if (i >= g.ns) return;
-  for (int q = 0; q < 3; ++q) d.dr0[q * g.ns + i] = d.r[q * g.ns + i];
-  d.acc[i] = 0.0;
 }
 
 static inline unsigned nblocks(int64_t n, int bs) {
@@ -430,13 +411,12 @@ void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
   hipLaunchKernelGGL(k_run_sort, dim3(gb), dim3(bs), 0, st, d, g);
   hipLaunchKernelGGL(k_count, dim3(gb), dim3(bs), 0, st, d, g);
   launch_scan(d.ccount, d.coff, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
-  hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, st, d, g);
   hipLaunchKernelGGL(k_emit, dim3(gb), dim3(bs), 0, st, d, g,
                      d.slot64 ? (int)SC_PREC_FP64 : (int)SC_PREC_MIXED);
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
   hipLaunchKernelGGL(k_inc_fill, dim3(2048), dim3(bs), 0, st, d, g);
-  hipLaunchKernelGGL(k_inc_sort, dim3(gb), dim3(bs), 0, st, d, g);
-  hipLaunchKernelGGL(k_zero_dr, dim3(nblocks(g.ns, bs)), dim3(bs), 0, st, d, g);
+  hipLaunchKernelGGL(k_inc_sort, dim3(nblocks(g.ns > g.nb ? g.ns : g.nb, bs)), dim3(bs), 0, st,
+                     d, g);
 }
 
 // Replays the displacement bookkeeping of the last full step after the host
