@@ -239,7 +239,8 @@ def measured_traffic(tag, kernel):
 
 
 def pair_bytes(precision, nb, nc):
-    """Algorithmic HBM bytes of one launch of the fused pair/scatter kernel:
+    """Algorithmic HBM bytes of one launch of the pair/scatter kernel (either
+    launch shape):
     per bond its 64-byte state record read, its force and torque (6 FP64)
     written and its incidence offset read; per candidate the sticky force
     slot and its two incidence entries {candidate, partner} read.  Re-reads of
@@ -343,7 +344,9 @@ def main():
     bytes_pair = pair_bytes(args.precision, fil.nb, nc)
     achieved = bytes_pair / (pair_ms * 1e-3) / 1e9 if pair_ms > 0 else 0.0
     log(f"[bench] profile: {json.dumps(prof)} nc={nc}")
-    kname = "k_pair_fused<true>" if args.precision == "mixed" else "k_pair_fused<false>"
+    balanced = tune is not None and int(tune["pair_block"]) == 0
+    kname = ("k_pair_bal" if balanced else "k_pair_fused") + (
+        "<true>" if args.precision == "mixed" else "<false>")
     traffic = None
     if args.workload == "c1":
         traffic = measured_traffic(f"dense620k_{args.precision}", kname)

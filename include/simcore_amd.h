@@ -202,7 +202,8 @@ int sc_set_launch_shape(sc_handle h, int32_t pair_block, int32_t pair_xcd,
 int sc_device_bytes(sc_handle h, size_t *bytes);
 /* Average per-launch duration (ms) of each kernel family recorded with HIP
  * events on the handle's stream while profiling was enabled; families:
- * 0 pair pass with the fused per-bond force scatter, 1 unused (0),
+ * 0 pair pass with the fused per-bond force scatter (and crosslinks),
+ * 1 commit (step counter, next-step decision),
  * 2 integrate (all length bins), 3 rebuild stage (odd steps only). */
 int sc_profile_enable(sc_handle h, int on);
 int sc_profile_read(sc_handle h, double *avg_ms, int64_t *launches,

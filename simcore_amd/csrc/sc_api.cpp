@@ -24,7 +24,7 @@ struct Bin {
   int32_t *list = nullptr;
 };
 
-constexpr int kFamilies = 4;  // pair+scatter, (unused), integrate, rebuild
+constexpr int kFamilies = 4;  // pair+scatter, commit, integrate, rebuild
 
 }  // namespace
 
@@ -202,16 +202,16 @@ void enqueue_step(sc_system *h, bool odd) {
   }
   if (pf) {
     (void)hipEventRecord(h->ev[1], st);
-    (void)hipEventRecord(h->ev[2], st);
-  }
-  if (pf) {
-    (void)hipEventRecord(h->ev[3], st);
     (void)hipEventRecord(h->ev[4], st);
   }
   for (const Bin &b : h->bins)
     sc::launch_integrate(d, g, b.list, b.count, b.maxn, h->p.precision, h->tune, st);
-  if (pf) (void)hipEventRecord(h->ev[5], st);
+  if (pf) {
+    (void)hipEventRecord(h->ev[5], st);
+    (void)hipEventRecord(h->ev[2], st);
+  }
   sc::launch_commit(d, g, st);
+  if (pf) (void)hipEventRecord(h->ev[3], st);
 }
 
 int capture_graphs(sc_system *h) {
@@ -333,7 +333,7 @@ int enqueue_range(sc_system *h, int64_t from, int64_t count) {
     } else {
       enqueue_step(h, odd);
       if (h->prof) {
-        HIPCHK(h, hipEventSynchronize(h->ev[5]));
+        HIPCHK(h, hipEventSynchronize(h->ev[3]));
         float ms = 0.f;
         const int pairs[kFamilies][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}};
         for (int k = 0; k < kFamilies; ++k) {

@@ -324,5 +324,5 @@ class FilamentSystem:
         ms = (c_double * 4)()
         n = (c_int64 * 4)()
         _check(lib.sc_profile_read(self.h, ms, n, 4), self.h, "sc_profile_read")
-        names = ["pair", "unused", "integrate", "rebuild"]
+        names = ["pair", "commit", "integrate", "rebuild"]
         return {k: (ms[i], n[i]) for i, k in enumerate(names)}
