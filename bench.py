@@ -271,7 +271,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch-shape", default="",
                     help="pair_block,pair_xcd,integrate_waves,integrate_layout instead of "
-                         "autotuning (pair_block 0: load-balanced pair kernel)")
+                         "autotuning "
+                         "(pair_block 0: load-balanced pair kernel)")
     ap.add_argument("--caller-order", action="store_true",
                     help="keep filaments in generation order (no spatial storage order)")
     # serving workload (--workload c4)
@@ -318,10 +319,10 @@ def main():
     log(f"[bench] warmup {args.warmup} steps done, counters {dev.counters()}")
     tune = None
     if args.launch_shape:
-        pb, px, iw, il = (int(v) for v in args.launch_shape.split(","))
-        dev.set_launch_shape(pb, px, iw, il)
-        tune = {"pair_block": pb, "pair_xcd": px, "integrate_waves": iw,
-                "integrate_layout": il, "set": "fixed"}
+        keys = ["pair_block", "pair_xcd", "integrate_waves", "integrate_layout"]
+        shape = dict(zip(keys, (int(v) for v in args.launch_shape.split(","))))
+        dev.set_launch_shape(**shape)
+        tune = dict(dev.launch_shape(), set="fixed")
     elif not args.no_autotune:
         # launch shapes that leave results unchanged, chosen from a
         # checkpoint of the warmed-up state (the state is restored)
@@ -344,7 +345,7 @@ def main():
     bytes_pair = pair_bytes(args.precision, fil.nb, nc)
     achieved = bytes_pair / (pair_ms * 1e-3) / 1e9 if pair_ms > 0 else 0.0
     log(f"[bench] profile: {json.dumps(prof)} nc={nc}")
-    balanced = tune is not None and int(tune["pair_block"]) == 0
+    balanced = int(dev.launch_shape()["pair_block"]) == 0
     kname = ("k_pair_bal" if balanced else "k_pair_fused") + (
         "<true>" if args.precision == "mixed" else "<false>")
     traffic = None

@@ -175,28 +175,36 @@ int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows,
 int sc_checkpoint_save(sc_handle h, void *buf, size_t cap, size_t *size);
 int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
 
+/* Launch shapes (no counterpart in the reference).  None of them changes
+ * any result except integrate_layout, which changes the rounding of the
+ * mixed-precision integrator.
+ *   pair_block          pair kernel threads per block, 64/128/256/512; 0
+ *                       selects the load-balanced variant (one wave per 64
+ *                       bonds, incidences dealt evenly over its lanes)
+ *   pair_xcd            1: consecutive bond ranges on one XCD
+ *   integrate_waves     integrator waves per block, 1/2/4
+ *   integrate_layout    filaments of up to 64 sites: one (0) or two (1,
+ *                       default) sites per lane */
+typedef struct sc_launch_shape {
+  int32_t pair_block, pair_xcd, integrate_waves, integrate_layout;
+} sc_launch_shape;
+int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape);
+int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
+
 /* Autotuner (no counterpart in the reference): times the launch shapes
- * that leave every result unchanged -- threads per block of the pair
- * kernel (128/256/512), its XCD-contiguous block order on or off,
- * waves per block of the integrator (1/2/4) and its lane layout for
- * filaments of up to 64 sites (one or two sites per lane) -- over `steps`
- * steps each from a checkpoint of the current state (pair_block 0 in the
- * report: the load-balanced pair kernel), keeps the fastest and
- * restores the state, so the trajectory is the one without tuning.  The
- * cell size is not tuned: in the reference it sets the rebuild threshold
- * and the candidate set (src/cell_list.cpp:9-64), i.e. it is part of the
- * model. */
+ * that leave every result unchanged -- first the pair kernel's (block size
+ * or balanced variant, XCD order), then the integrator's (waves per block;
+ * the layout is kept) -- over `steps` steps each
+ * from a checkpoint of the current state, keeps the fastest and restores
+ * the state, so the trajectory is the one without tuning.  The cell size
+ * is not tuned: in the reference it sets the rebuild threshold and the
+ * candidate set (src/cell_list.cpp:9-64), i.e. it is part of the model. */
 typedef struct sc_tune_report {
-  int32_t pair_block, pair_xcd, integrate_waves, integrate_layout, n_trials;
+  sc_launch_shape shape;  /* the one kept */
+  int32_t n_trials;
   double step_ms_start, step_ms_best;
 } sc_tune_report;
 int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report);
-/* Sets the launch shapes directly: pair threads per block (64/128/256/512;
- * 0 selects the load-balanced pair kernel, one wave per 64 bonds),
- * XCD-contiguous pair blocks (0/1), integrator waves per block (1/2/4) and,
- * for filaments of up to 64 sites, one (0) or two (1) sites per lane. */
-int sc_set_launch_shape(sc_handle h, int32_t pair_block, int32_t pair_xcd,
-                        int32_t integrate_waves, int32_t integrate_layout);
 
 /* Device bytes this handle holds. */
 int sc_device_bytes(sc_handle h, size_t *bytes);

@@ -32,7 +32,7 @@ EXPORTS = [
     "sc_generate_filaments", "sc_copy_probe", "sc_step_async", "sc_step_poll",
     "sc_filament_stats", "sc_theta_histogram", "sc_output_header", "sc_spec_frame",
     "sc_posit_frame", "sc_checkpoint_save", "sc_checkpoint_load", "sc_autotune",
-    "sc_set_launch_shape",
+    "sc_set_launch_shape", "sc_get_launch_shape",
     "sc_load_crosslinks", "sc_get_crosslinks", "sc_generate_crosslinks",
     "sc_ensemble_create", "sc_ensemble_submit", "sc_ensemble_progress",
     "sc_ensemble_drain", "sc_ensemble_request", "sc_ensemble_result",
@@ -92,10 +92,13 @@ class ScCrosslinkParams(ctypes.Structure):
                 ("k_off", c_double), ("r_bind", c_double), ("seed", c_uint64)]
 
 
-class ScTuneReport(ctypes.Structure):
+class ScLaunchShape(ctypes.Structure):
     _fields_ = [("pair_block", c_int32), ("pair_xcd", c_int32),
-                ("integrate_waves", c_int32), ("integrate_layout", c_int32),
-                ("n_trials", c_int32),
+                ("integrate_waves", c_int32), ("integrate_layout", c_int32)]
+
+
+class ScTuneReport(ctypes.Structure):
+    _fields_ = [("shape", ScLaunchShape), ("n_trials", c_int32),
                 ("step_ms_start", c_double), ("step_ms_best", c_double)]
 
 
@@ -140,7 +143,8 @@ def _load():
         "sc_checkpoint_save": ([H, c_void_p, c_size_t, POINTER(c_size_t)], c_int),
         "sc_checkpoint_load": ([H, c_void_p, c_size_t], c_int),
         "sc_autotune": ([H, c_int32, POINTER(ScTuneReport)], c_int),
-        "sc_set_launch_shape": ([H, c_int32, c_int32, c_int32, c_int32], c_int),
+        "sc_set_launch_shape": ([H, POINTER(ScLaunchShape)], c_int),
+        "sc_get_launch_shape": ([H, POINTER(ScLaunchShape)], c_int),
         "sc_load_crosslinks": ([H, c_int64, ip, dp, POINTER(ScCrosslinkParams)], c_int),
         "sc_get_crosslinks": ([H, ip, dp, dp, lp], c_int),
         "sc_generate_crosslinks": ([POINTER(ScParams), c_int32, ip, dp, POINTER(ScFilamentParams),

@@ -204,8 +204,10 @@ void enqueue_step(sc_system *h, bool odd) {
     (void)hipEventRecord(h->ev[1], st);
     (void)hipEventRecord(h->ev[4], st);
   }
-  for (const Bin &b : h->bins)
-    sc::launch_integrate(d, g, b.list, b.count, b.maxn, h->p.precision, h->tune, st);
+  sc::BinDesc bins[sc::kMaxBins];
+  int nbins = 0;
+  for (const Bin &b : h->bins) bins[nbins++] = {b.list, b.count, b.maxn};
+  sc::launch_integrate_all(d, g, bins, nbins, h->p.precision, h->tune, st);
   if (pf) {
     (void)hipEventRecord(h->ev[5], st);
     (void)hipEventRecord(h->ev[2], st);
@@ -1163,29 +1165,39 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   int trials = 0;
   rc = trial(start, &start_ms);
   best_ms = start_ms;
+  // Two stages: the pair kernel's shape with the integrator's fixed, then
+  // the integrator's with the best pair shape (the two kernels are
+  // separate launches, so their times add).
+  auto consider = [&](const sc::Tune &t) {
+    if (rc) return;
+    double ms = 0;
+    rc = trial(t, &ms);
+    ++trials;
+    if (rc == SC_OK && ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  };
   // pair kernel: per-bond with 128/256/512 threads per block, or the
   // balanced one-wave-per-64-bonds variant (block size not used)
   const int blocks[] = {128, 256, 512, 0};
-  const int waves[] = {1, 2, 4};
   for (int bs : blocks)
-    for (int x = 0; x < 2 && rc == SC_OK; ++x)
-      for (int w : waves)
-        for (int lay = 0; lay < 2; ++lay) {
-        if (rc) break;
-        sc::Tune t;
-        t.pair_bal = bs == 0 ? 1 : 0;
-        t.pair_block = bs == 0 ? 256 : bs;
-        t.pair_xcd = x;
-        t.int_waves = w;
-        t.int_layout = lay;
-        double ms = 0;
-        rc = trial(t, &ms);
-        ++trials;
-        if (rc == SC_OK && ms < best_ms) {
-          best_ms = ms;
-          best = t;
-        }
-      }
+    for (int x = 0; x < 2; ++x) {
+      sc::Tune t = best;
+      t.pair_bal = bs == 0 ? 1 : 0;
+      t.pair_block = bs == 0 ? 256 : bs;
+      t.pair_xcd = x;
+      consider(t);
+    }
+  // integrator: waves per block (the layout, sites per lane, changes the
+  // rounding of the mixed path and is not tuned)
+  const sc::Tune pair_best = best;
+  const int waves[] = {1, 2, 4};
+  for (int w : waves) {
+    sc::Tune t = pair_best;
+    t.int_waves = w;
+    consider(t);
+  }
   // leave the system where it was, with the chosen shapes
   h->tune = rc ? start : best;
   drop_graph(h);
@@ -1196,10 +1208,7 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   if (rc) return rc;
   if (rc2) return rc2;
   if (report) {
-    report->pair_block = best.pair_bal ? 0 : best.pair_block;
-    report->pair_xcd = best.pair_xcd;
-    report->integrate_waves = best.int_waves;
-    report->integrate_layout = best.int_layout;
+    (void)sc_get_launch_shape(h, &report->shape);
     report->n_trials = trials;
     report->step_ms_start = start_ms;
     report->step_ms_best = best_ms;
@@ -1207,23 +1216,36 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   return SC_OK;
 }
 
-int sc_set_launch_shape(sc_handle h, int32_t pair_block, int32_t pair_xcd,
-                        int32_t integrate_waves, int32_t integrate_layout) {
+int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
   int rc = check_handle(h, false);
   if (rc) return rc;
-  if ((pair_block != 0 && pair_block != 64 && pair_block != 128 && pair_block != 256 &&
-       pair_block != 512) ||
-      (pair_xcd != 0 && pair_xcd != 1) ||
-      (integrate_waves != 1 && integrate_waves != 2 && integrate_waves != 4) ||
-      (integrate_layout != 0 && integrate_layout != 1))
+  if (!shape) return fail(h, SC_ERR_ARG, "null launch shape");
+  const sc_launch_shape &v = *shape;
+  if ((v.pair_block != 0 && v.pair_block != 64 && v.pair_block != 128 && v.pair_block != 256 &&
+       v.pair_block != 512) ||
+      (v.pair_xcd != 0 && v.pair_xcd != 1) ||
+      (v.integrate_waves != 1 && v.integrate_waves != 2 && v.integrate_waves != 4) ||
+      (v.integrate_layout != 0 && v.integrate_layout != 1))
     return fail(h, SC_ERR_ARG, "unsupported launch shape");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
-  h->tune.pair_bal = pair_block == 0 ? 1 : 0;
-  h->tune.pair_block = pair_block == 0 ? 256 : pair_block;
-  h->tune.pair_xcd = pair_xcd;
-  h->tune.int_waves = integrate_waves;
-  h->tune.int_layout = integrate_layout;
+  h->tune.pair_bal = v.pair_block == 0 ? 1 : 0;
+  h->tune.pair_block = v.pair_block == 0 ? 256 : v.pair_block;
+  h->tune.pair_xcd = v.pair_xcd;
+  h->tune.int_waves = v.integrate_waves;
+  h->tune.int_layout = v.integrate_layout;
   drop_graph(h);
+  return SC_OK;
+}
+
+int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape) {
+  int rc = check_handle(h, false);
+  if (rc) return rc;
+  if (!shape) return fail(h, SC_ERR_ARG, "null launch shape");
+  const sc::Tune &t = h->tune;
+  shape->pair_block = t.pair_bal ? 0 : t.pair_block;
+  shape->pair_xcd = t.pair_xcd;
+  shape->integrate_waves = t.int_waves;
+  shape->integrate_layout = t.int_layout;
   return SC_OK;
 }
 

@@ -691,4 +691,12 @@ void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
 #undef SC_LAUNCH
 }
 
+void launch_integrate_all(const Dev &d, const Geo &g, const BinDesc *bins, int nbins,
+                          int precision, const Tune &t, hipStream_t st) {
+  // one launch per bin: a single launch over all bins (the waves of each
+  // bin running their own variant) measured markedly slower than this
+  for (int k = 0; k < nbins; ++k)
+    launch_integrate(d, g, bins[k].list, bins[k].count, bins[k].maxn, precision, t, st);
+}
+
 }  // namespace sc

@@ -176,14 +176,15 @@ __device__ __forceinline__ void det_backward(const double (&N)[S][4], double (&o
 
 }  // namespace
 
+// One wavefront's filaments: wave `wave` of a bin holds list entries
+// wave*64/G ... (wave+1)*64/G - 1.  Everything stays in registers; there is
+// no LDS and no barrier.
 template <int G, int S>
-__global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int32_t *list,
-                                                        int count, int prec) {
+__device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int32_t *list,
+                                          int count, int wave) {
   constexpr int NG = 64 / G;
   const Ctl *ctl = d.ctl;
-  if (ctl->abort) return;
   const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int grp = lane / G, gl = lane % G;
   const int fidx = wave * NG + grp;
   const bool factive = fidx < count;
@@ -209,23 +210,20 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
   }
   const int base = gl * S;
 
-  // ---- loads -------------------------------------------------------------
-  double rp[S][3], u[S][3], Fb[S][3], Tb[S][3], fr[S][3], dr0[S][3], acc[S];
+  // ---- loads (values needed only at the end are loaded there, which keeps
+  // register pressure, hence occupancy, in check) --------------------------
+  double u[S][3], Fb[S][3], Tb[S][3], fr[S][3];
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     const int i = base + k;
     const int64_t si = s0 + i;
     for (int q = 0; q < 3; ++q) {
-      rp[k][q] = 0.0;
       u[k][q] = q == 0 ? 1.0 : 0.0;
       Fb[k][q] = 0.0;
       Tb[k][q] = 0.0;
       fr[k][q] = 0.0;
-      dr0[k][q] = 0.0;
     }
-    acc[k] = 0.0;
     if (i < n) {
-      for (int q = 0; q < 3; ++q) rp[k][q] = mid ? d.r[q * ns + si] : d.rp[q * ns + si];
       const int64_t jb = b0 + (i < m ? i : m - 1);
       const double4 *rec = reinterpret_cast<const double4 *>(d.br + 8 * jb);
       const double4 r0 = rec[0], r1 = rec[1];
@@ -237,12 +235,6 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
           Fb[k][q] = d.bf[q * g.nb + jb];
           Tb[k][q] = d.bt[q * g.nb + jb];
         }
-      if (!mid && g.stoch)
-        for (int q = 0; q < 3; ++q) fr[k][q] = d.fr[q * ns + si];
-      if (!mid) {
-        acc[k] = d.acc[si];
-        for (int q = 0; q < 3; ++q) dr0[k][q] = d.dr0[q * ns + si];
-      }
     }
   }
 
@@ -328,15 +320,15 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
           for (int q = 0; q < 3; ++q) d.fr[q * ns + s0 + i] = fr[k][q];
       }
     }
-#pragma unroll
-    for (int k = 0; k < S; ++k)
-      if (base + k < n)
-        for (int q = 0; q < 3; ++q) d.rp[q * ns + s0 + base + k] = rp[k][q];
   }
   if (g.stoch)
 #pragma unroll
-    for (int k = 0; k < S; ++k)
+    for (int k = 0; k < S; ++k) {
+      // full step: the projected random force of the midstep
+      if (!mid && base + k < n)
+        for (int q = 0; q < 3; ++q) fr[k][q] = d.fr[q * ns + s0 + base + k];
       for (int q = 0; q < 3; ++q) fs[k][q] += fr[k][q];
+    }
 
   // ---- bending (src/filament.cpp:608-776), reduced form ------------------
   {
@@ -350,13 +342,17 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
       M[k][1] = sm ? -csmm[k] * csmm[k] : 0.0;
       M[k][2] = sm ? 1.0 : 0.0;
       M[k][3] = sm ? 0.0 : 1.0;
+    }
+    det_forward<G, S>(M, det_t, lane, gl);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int i = base + k;
       const bool sn = i >= 1 && i <= n - 2;
       N[k][0] = sn ? 2.0 : 1.0;
       N[k][1] = sn ? -csm[k] * csm[k] : 0.0;
       N[k][2] = sn ? 1.0 : 0.0;
       N[k][3] = sn ? 0.0 : 1.0;
     }
-    det_forward<G, S>(M, det_t, lane, gl);
     det_backward<G, S>(N, det_b, lane, gl);
     // det_t at the last site, det_b three sites ahead (det_b[n] = 1)
     const int last = n > 0 ? n - 1 : 0;
@@ -435,11 +431,20 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
     if (base + k < m) d.tension[b0 + base + k] = T[k];
 
   // ---- positions (src/filament.cpp:840-927) ------------------------------
+  // midstep start r_prev: the current positions on a midstep (stored as the
+  // new r_prev), the stored ones on a full step
   const double delta = mid ? 0.5 * g.delta : g.delta;
   double rn[S][3], rnn[S][3], uo[S][3];
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     const int i = base + k;
+    double rp[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+      const int64_t si = s0 + i;
+      for (int q = 0; q < 3; ++q) rp[q] = mid ? d.r[q * ns + si] : d.rp[q * ns + si];
+      if (mid)
+        for (int q = 0; q < 3; ++q) d.rp[q * ns + si] = rp[q];
+    }
     if (i == 0) {
       for (int q = 0; q < 3; ++q) fs[k][q] += T[k] * u[k][q];
     } else if (i == n - 1) {
@@ -449,7 +454,7 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
     }
     double mv[3];
     mob(igperp, dg, t[k], fs[k], mv);
-    for (int q = 0; q < 3; ++q) rn[k][q] = rp[k][q] + delta * mv[q];
+    for (int q = 0; q < 3; ++q) rn[k][q] = rp[q] + delta * mv[q];
   }
   site_at3<G, S, 1>(rn, rnn, lane, gl);
   int bad = 0;
@@ -536,13 +541,14 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
     if (i >= n) continue;
     const int64_t si = s0 + i;
     for (int q = 0; q < 3; ++q) d.r[q * ns + si] = rn[k][q];
-    if (i < m) bond_reinit(d, g, b0 + i, rn[k], rnn[k], prec);
+    if (i < m) bond_reinit(d, g, b0 + i, rn[k], rnn[k], SC_PREC_MIXED);
     if (!mid) {
-      double a = acc[k];
+      double a = d.acc[si], dr0[3];
+      for (int q = 0; q < 3; ++q) dr0[q] = d.dr0[q * ns + si];
       d.acc_prev[si] = a;
       for (int rep = 0; rep < 2; ++rep)
         for (int q = 0; q < 3; ++q) {
-          const double dd = rn[k][q] - dr0[k][q];
+          const double dd = rn[k][q] - dr0[q];
           a += dd * dd;
         }
       d.acc[si] = a;
@@ -560,8 +566,16 @@ __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int3
   }
 }
 
+template <int G, int S>
+__global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int32_t *list,
+                                                        int count) {
+  if (d.ctl->abort) return;
+  long_wave<G, S>(d, g, list, count, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
+
 bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list, int bin_count,
-                           int bin_maxn, int precision, int waves, int layout, hipStream_t st) {
+                           int bin_maxn, int precision, int waves, int layout,
+                           hipStream_t st) {
   if (precision == SC_PREC_FP64 || bin_maxn > 256) return false;
   // up to 64 sites: only in the two-sites-per-lane layout
   if (bin_maxn <= 64 && layout != 1) return false;
@@ -571,7 +585,7 @@ bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list, 
   dim3 grid((bin_count + per_block - 1) / per_block), block(64 * waves);
 #define SC_LONG(GG, SS)                                                                   \
   hipLaunchKernelGGL((k_integrate_long<GG, SS>), grid, block, 0, st, d, g, bin_list,     \
-                     bin_count, precision)
+                     bin_count)
   switch (bin_maxn) {
     case 8: SC_LONG(4, 2); break;
     case 16: SC_LONG(8, 2); break;

@@ -101,8 +101,18 @@ struct Tune {
   int pair_xcd = 0;      // 1: consecutive bond ranges stay on one XCD
   int pair_bal = 0;      // 1: incidences dealt evenly over a wave's lanes
   int int_waves = 1;     // waves per block of the register integrator
-  int int_layout = 0;    // up to 64 sites: 0 one site per lane, 1 two
+  int int_layout = 1;    // up to 64 sites: 0 one site per lane, 1 two (the
+                         // two round differently; not changed by tuning)
 };
+
+// Length bins of the integrator, as the step loop hands them over.
+constexpr int kMaxBins = 6;
+struct BinDesc {
+  const int32_t *list;  // filament indices of the bin
+  int count;
+  int maxn;  // 8, 16, 32, 64, 128 or 256 sites
+};
+
 
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
@@ -119,6 +129,9 @@ void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t,
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
                       int bin_count, int bin_maxn, int precision, const Tune &t,
                       hipStream_t st);
+// all length bins of a step, one launch per bin
+void launch_integrate_all(const Dev &d, const Geo &g, const BinDesc *bins, int nbins,
+                          int precision, const Tune &t, hipStream_t st);
 // register-resident fast-path integrator; false when the bin is not one it
 // serves (FP64 precision or more than 64 sites), nothing launched then
 bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,

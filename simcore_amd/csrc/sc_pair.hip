@@ -14,11 +14,10 @@
 // Bond state is one 64-byte record per bond (scaled position, orientation,
 // length), so each gather touches one half cache line.
 //
-// FP64 path: bit-for-bit the reference's arithmetic.  Mixed path: minimum
-// image in FP64 from the FP64 scaled positions (box up to ~250 sigma), then
-// FP32 segment distance and WCA; near-parallel orientations
-// (1-(u1.u2)^2 < 2e-2, where FP32 line parameters lose digits) and
-// near-contact pairs (dr^2 < 1e-6) redo the pair with the FP64 algorithm.
+// FP64 path: bit-for-bit the reference's arithmetic.  Mixed path: the same
+// FP64 geometry and branch rules with the divisions and square roots taken
+// from the hardware FP32 reciprocal / reciprocal square root (the latter
+// refined in FP64), and FP32 force slots.
 #include <hip/hip_runtime.h>
 
 #include "sc_internal.h"
@@ -27,13 +26,6 @@
 namespace sc {
 
 namespace {
-
-constexpr float kParallelSwitch = 2.0e-2f;
-
-__device__ __forceinline__ float fabs_(float x) { return x < 0.f ? -x : x; }
-__device__ __forceinline__ float fsign_(float a, float b) {
-  return b >= 0.f ? fabsf(a) : -fabsf(a);
-}
 
 struct BondRec {
   double s[3], u[3], len;
@@ -53,32 +45,48 @@ __device__ __forceinline__ BondRec load_rec(const double *br, int64_t b) {
   return r;
 }
 
-// FP32 MinimumDistance::Sphero (src/minimum_distance.cpp:231-343) on the
-// FP32 minimum-image separation; false when the FP64 path must be used.
-__device__ __forceinline__ bool sphero32(const float *dr, const float *u1, float hl1,
-                                         const float *u2, float hl2, float *rmin,
-                                         float *rm2, float *lam_o, float *mu_o) {
-  const float a = dr[0] * u1[0] + dr[1] * u1[1] + dr[2] * u1[2];
-  const float b = dr[0] * u2[0] + dr[1] * u2[1] + dr[2] * u2[2];
-  const float c = u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2];
-  const float denom = 1.0f - c * c;
-  if (denom < kParallelSwitch) return false;
-  const float idn = 1.0f / denom;
-  float lambda = (a - c * b) * idn;
-  float mu = (-b + c * a) * idn;
-  const float lm = fabs_(lambda), mm = fabs_(mu);
+// Mixed-precision MinimumDistance::Sphero (src/minimum_distance.cpp:231-343):
+// the reference's FP64 arithmetic and branch rules, except that the division
+// by 1-(u1.u2)^2 multiplies by the hardware FP32 reciprocal (relative error
+// ~1e-7).  A clamped line parameter is recomputed exactly from the other
+// one, so the reciprocal's error reaches only unclamped parameters
+// (|lambda| <= l/2).  On CDNA4 an FP64 FMA issues at the rate of an
+// unpacked FP32 one, so keeping the geometry in FP64 costs nothing and
+// needs no separate near-parallel path; only the divisions and square roots
+// (many instructions in IEEE FP64) are replaced.
+__device__ __forceinline__ void sphero_mixed(const double *dr, const double *u1, double hl1,
+                                             const double *u2, double hl2, double *rmin,
+                                             double *rm2, double *lam_o, double *mu_o) {
+  double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    a += dr[i] * u1[i];
+    b += dr[i] * u2[i];
+    c += u1[i] * u2[i];
+  }
+  double lambda, mu;
+  const double denom = 1.0 - c * c;
+  if (denom < 1.0e-12) {
+    lambda = 0.5 * a;
+    mu = -0.5 * b;
+  } else {
+    const double id = (double)__builtin_amdgcn_rcpf((float)denom);
+    lambda = (a - c * b) * id;
+    mu = (-b + c * a) * id;
+  }
+  const double lm = dabs(lambda), mm = dabs(mu);
   if (lm > hl1 && mm > hl2) {
-    float la = fsign_(hl1, lambda);
-    float ma = -b + la * c;
-    if (fabs_(ma) > hl2) ma = fsign_(hl2, ma);
-    float mb = fsign_(hl2, mu);
-    float lb = a + mb * c;
-    if (fabs_(lb) > hl1) lb = fsign_(hl1, lb);
-    float m2a = 0.f, m2b = 0.f;
+    const double la = dsign(hl1, lambda);
+    double ma = -b + la * c;
+    if (dabs(ma) > hl2) ma = dsign(hl2, ma);
+    const double mb = dsign(hl2, mu);
+    double lb = a + mb * c;
+    if (dabs(lb) > hl1) lb = dsign(hl1, lb);
+    double m2a = 0.0, m2b = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const float ra = dr[i] - la * u1[i] + ma * u2[i];
-      const float rb = dr[i] - lb * u1[i] + mb * u2[i];
+      const double ra = dr[i] - la * u1[i] + ma * u2[i];
+      const double rb = dr[i] - lb * u1[i] + mb * u2[i];
       m2a += ra * ra;
       m2b += rb * rb;
     }
@@ -90,25 +98,38 @@ __device__ __forceinline__ bool sphero32(const float *dr, const float *u1, float
       mu = mb;
     }
   } else if (lm > hl1) {
-    lambda = fsign_(hl1, lambda);
+    lambda = dsign(hl1, lambda);
     mu = -b + lambda * c;
-    if (fabs_(mu) > hl2) mu = fsign_(hl2, mu);
+    if (dabs(mu) > hl2) mu = dsign(hl2, mu);
   } else if (mm > hl2) {
-    mu = fsign_(hl2, mu);
+    mu = dsign(hl2, mu);
     lambda = a + mu * c;
-    if (fabs_(lambda) > hl1) lambda = fsign_(hl1, lambda);
+    if (dabs(lambda) > hl1) lambda = dsign(hl1, lambda);
   }
-  float m2 = 0.f;
+  double m2 = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     rmin[i] = dr[i] - lambda * u1[i] + mu * u2[i];
     m2 += rmin[i] * rmin[i];
   }
-  if (m2 < 1.0e-6f) return false;
   *rm2 = m2;
   *lam_o = lambda;
   *mu_o = mu;
-  return true;
+}
+
+// WCA force of WCAPotential::CalcPotential (include/simcore/wca_potential.hpp:
+// 14-32) with 1/r from the hardware FP32 reciprocal square root refined by
+// one FP64 Newton step (relative error ~1e-14), rounded to FP32 for the
+// slot.  dr^2 >= 1e-12 here.
+__device__ __forceinline__ float4 wca_mixed(const Geo &g, const double *rmin, double rm2) {
+  double rinv = (double)__builtin_amdgcn_rsqf((float)rm2);
+  rinv = rinv * (1.5 - 0.5 * rm2 * rinv * rinv);
+  const double rinv2 = rinv * rinv;
+  const double r6 = rinv2 * rinv2 * rinv2;
+  double ffac = -(12.0 * g.c12 * r6 - 6.0 * g.c6) * r6 * rinv;
+  if (dabs(ffac) > g.fcut) ffac = (ffac >= 0.0 ? 1.0 : -1.0) * g.fcut;
+  const double sc = ffac * rinv;
+  return make_float4((float)(sc * rmin[0]), (float)(sc * rmin[1]), (float)(sc * rmin[2]), 0.f);
 }
 
 }  // namespace
@@ -161,49 +182,32 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
       fc[2] = o.z;
     }
   } else {
-    float dr32[3];
+    double sep[3], dr[3], dr2;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       double ds = r2.s[k] - r1.s[k];
       ds -= (double)nint(ds);
-      dr32[k] = (float)(g.h[4 * k] * ds);
+      sep[k] = g.h[4 * k] * ds;
     }
-    const float u1[3] = {(float)r1.u[0], (float)r1.u[1], (float)r1.u[2]};
-    const float u2[3] = {(float)r2.u[0], (float)r2.u[1], (float)r2.u[2]};
-    float rmin[3], rm2, lf, mf;
+    sphero_mixed(sep, r1.u, 0.5 * r1.len, r2.u, 0.5 * r2.len, dr, &dr2, &lam, &mu);
     bool wrote = false;
-    if (sphero32(dr32, u1, (float)(0.5 * r1.len), u2, (float)(0.5 * r2.len), rmin, &rm2, &lf,
-                 &mf)) {
-      lam = lf;
-      mu = mf;
-      if (!(rm2 > (float)g.rcut2)) {
-        const float rinv = 1.0f / sqrtf(rm2);
-        const float rinv2 = rinv * rinv;
-        const float r6 = rinv2 * rinv2 * rinv2;
-        float ffac = -(12.0f * (float)g.c12 * r6 - 6.0f * (float)g.c6) * r6 * rinv;
-        const float fcut = (float)g.fcut;
-        if (fabs_(ffac) > fcut) ffac = (ffac >= 0.f ? 1.f : -1.f) * fcut;
-        const float s = ffac * rinv;
-        const float4 f = make_float4(s * rmin[0], s * rmin[1], s * rmin[2], 0.f);
-        d.slot32[c] = f;
-        fc[0] = f.x;
-        fc[1] = f.y;
-        fc[2] = f.z;
+    if (!(dr2 > g.rcut2)) {
+      float4 f4;
+      if (dr2 < 1.0e-12) {
+        // MaxForcePotential branch (include/simcore/max_force_potential.hpp:
+        // 11-40), FP64 as the reference; practically never taken
+        double f[3];
+        wrote = pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f) != 0;
+        f4 = make_float4((float)f[0], (float)f[1], (float)f[2], 0.f);
+      } else {
+        f4 = wca_mixed(g, dr, dr2);
         wrote = true;
       }
-    } else {
-      double dr[3], dr2;
-      sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
-      if (!(dr2 > g.rcut2)) {
-        double f[3];
-        if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
-          const float4 f4 = make_float4((float)f[0], (float)f[1], (float)f[2], 0.f);
-          d.slot32[c] = f4;
-          fc[0] = f4.x;
-          fc[1] = f4.y;
-          fc[2] = f4.z;
-          wrote = true;
-        }
+      if (wrote) {
+        d.slot32[c] = f4;
+        fc[0] = f4.x;
+        fc[1] = f4.y;
+        fc[2] = f4.z;
       }
     }
     if (!wrote) {

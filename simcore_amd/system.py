@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import (SC_ERR_UNPHYSICAL, SC_OK, SC_PREC_FP64, SC_PREC_MIXED,
-                   ScCrosslinkParams, ScFilamentParams, ScParams, ScTuneReport,
+                   ScCrosslinkParams, ScFilamentParams, ScLaunchShape, ScParams, ScTuneReport,
                    SimcoreError, UnphysicalResult, lib)
 
 
@@ -267,15 +267,26 @@ class FilamentSystem:
         return b.reshape(-1, 2), la.reshape(-1, 2), f.reshape(-1, 9), ev.value
 
     def set_launch_shape(self, pair_block=256, pair_xcd=0, integrate_waves=1,
-                         integrate_layout=0):
-        _check(lib.sc_set_launch_shape(self.h, pair_block, pair_xcd, integrate_waves,
-                                       integrate_layout), self.h, "sc_set_launch_shape")
+                         integrate_layout=1):
+        """Launch shapes (sc_set_launch_shape); only integrate_layout changes
+        results (the rounding of the mixed integrator)."""
+        v = ScLaunchShape(pair_block, pair_xcd, integrate_waves, integrate_layout)
+        _check(lib.sc_set_launch_shape(self.h, byref(v)), self.h, "sc_set_launch_shape")
+
+    def launch_shape(self):
+        v = ScLaunchShape()
+        _check(lib.sc_get_launch_shape(self.h, byref(v)), self.h, "sc_get_launch_shape")
+        return {k: getattr(v, k) for k, _ in ScLaunchShape._fields_}
 
     def autotune(self, steps=20):
-        """Pick the fastest result-invariant launch shapes (sc_autotune)."""
+        """Pick the fastest result-invariant launch shapes (sc_autotune); the
+        kept shape's fields plus n_trials and the step times before/after."""
         rep = ScTuneReport()
         _check(lib.sc_autotune(self.h, int(steps), byref(rep)), self.h, "sc_autotune")
-        return {k: getattr(rep, k) for k, _ in ScTuneReport._fields_}
+        out = {k: getattr(rep.shape, k) for k, _ in ScLaunchShape._fields_}
+        out.update(n_trials=rep.n_trials, step_ms_start=rep.step_ms_start,
+                   step_ms_best=rep.step_ms_best)
+        return out
 
     def checkpoint(self):
         """Complete dynamic state as bytes (sc_checkpoint_save)."""

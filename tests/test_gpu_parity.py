@@ -321,7 +321,8 @@ def test_autotune_leaves_trajectory_unchanged(precision):
     b.step(31)
     rep = b.autotune(steps=6)
     print(rep)
-    assert rep["n_trials"] == 48 and rep["step_ms_best"] <= rep["step_ms_start"]
+    # 8 pair shapes, then 3 integrator shapes (waves per block)
+    assert rep["n_trials"] == 11 and rep["step_ms_best"] <= rep["step_ms_start"]
     assert b.counters()["i_step"] == 31
     a.step(40)
     b.step(40)
@@ -329,17 +330,18 @@ def test_autotune_leaves_trajectory_unchanged(precision):
         np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("kind", ["short", "long"])
-def test_two_sites_per_lane_layout_within_tolerance(kind):
-    """The integrator layout with two sites per lane for filaments of up to
-    64 sites, against the CPU path at the mixed tolerances, and with shapes
-    that leave results unchanged against the default layout's run
-    (launch shapes: block sizes and waves per block)."""
-    print(f"case: {kind} mixed, two sites per lane, 1 and 100 steps", flush=True)
+def test_integrator_layouts_within_tolerance(kind, layout):
+    """Both integrator layouts for filaments of up to 64 sites (one site per
+    lane, two sites per lane -- the default), with non-default block shapes,
+    against the CPU path at the mixed tolerances."""
+    print(f"case: {kind} mixed, layout {layout}, 1 and 100 steps", flush=True)
     from tests.helpers import long_system, uneven_system
     p, fil = uneven_system() if kind == "short" else long_system()
     dev = sc.FilamentSystem(p, fil, precision="mixed", use_graph=True)
-    dev.set_launch_shape(pair_block=128, pair_xcd=1, integrate_waves=2, integrate_layout=1)
+    dev.set_launch_shape(pair_block=128, pair_xcd=1, integrate_waves=2,
+                         integrate_layout=layout)
     ora = OracleSystem(p, fil)
     dev.step(1)
     assert ora.step(1) == 0
@@ -376,3 +378,4 @@ def test_balanced_pair_kernel_is_bit_identical(precision):
         ora = OracleSystem(p, fil)
         assert ora.step(41) == 0
         np.testing.assert_array_equal(b.sites()[0], ora.sites()[0])
+
