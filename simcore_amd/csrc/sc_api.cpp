@@ -207,7 +207,7 @@ void enqueue_step(sc_system *h, bool odd) {
   sc::BinDesc bins[sc::kMaxBins];
   int nbins = 0;
   for (const Bin &b : h->bins) bins[nbins++] = {b.list, b.count, b.maxn};
-  sc::launch_integrate_all(d, g, bins, nbins, h->p.precision, h->tune, st);
+  sc::launch_integrate_all(d, g, bins, nbins, h->p.precision, h->tune, odd, st);
   if (pf) {
     (void)hipEventRecord(h->ev[5], st);
     (void)hipEventRecord(h->ev[2], st);
@@ -433,7 +433,8 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   const int64_t n_cells = n1d * n1d * n1d;
   const int64_t nb = ns - nf;
   auto R = [](int64_t n) { return Arena::round((size_t)(n > 0 ? n : 1)); };
-  size_t b = R(sizeof(Ctl)) + R(4 * (nf + 1)) + R(4 * (int64_t)nf) + R(64 * (int64_t)nf) +
+  size_t b = R(sizeof(Ctl)) + R(8 * sc::kDrSlots) + R(4 * (nf + 1)) + R(4 * (int64_t)nf) +
+             R(64 * (int64_t)nf) +
              R(8 * (int64_t)nf) + R(4 * nb);
   b += 4 * R(24 * ns) + 2 * R(8 * ns);                    // r, rp, fr, dr0, acc, acc_prev
   b += R(64 * nb) + 2 * R(24 * nb) + R(8 * nb);           // record, F, T, tension
@@ -509,6 +510,9 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   h->site_off.assign(site_off, site_off + nf + 1);
   const int64_t ns = g.ns, nb = g.nb;
   if ((rc = dalloc(h, &d.ctl, 1, false))) return rc;
+  if ((rc = dalloc(h, &d.dr_slots, sc::kDrSlots, false))) return rc;
+  HIPCHK(h, hipMemsetAsync(d.dr_slots, 0, sizeof(unsigned long long) * sc::kDrSlots,
+                           h->stream));
   if ((rc = dalloc(h, &d.site_off, nf + 1, false))) return rc;
   if ((rc = dalloc(h, &d.noise_off, nf, false))) return rc;
   if ((rc = dalloc(h, &d.fpar, 8 * (size_t)nf, false))) return rc;

@@ -646,7 +646,8 @@ __global__ __launch_bounds__(64) void k_integrate(Dev d, Geo g,
       lmax = y > lmax ? y : lmax;
     }
     if (lane == 0)
-      atomicMax((unsigned long long *)&d.ctl->dr_bits,
+      atomicMax(&d.dr_slots[(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) &
+                            (kDrSlots - 1)],
                 (unsigned long long)__double_as_longlong(lmax));
   }
 }
@@ -658,12 +659,12 @@ static int integrate_bin_lanes(int maxn) {
 }
 
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
-                      int bin_count, int bin_maxn, int precision, const Tune &t,
+                      int bin_count, int bin_maxn, int precision, const Tune &t, bool mid,
                       hipStream_t st) {
   if (bin_count <= 0) return;
   // fast path: register-resident variants
   if (launch_integrate_long(d, g, bin_list, bin_count, bin_maxn, precision, t.int_waves,
-                            t.int_layout, st))
+                            t.int_layout, mid, st))
     return;
   if (launch_integrate_reg(d, g, bin_list, bin_count, bin_maxn, precision, t.int_waves, st))
     return;
@@ -692,11 +693,11 @@ void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
 }
 
 void launch_integrate_all(const Dev &d, const Geo &g, const BinDesc *bins, int nbins,
-                          int precision, const Tune &t, hipStream_t st) {
+                          int precision, const Tune &t, bool mid, hipStream_t st) {
   // one launch per bin: a single launch over all bins (the waves of each
   // bin running their own variant) measured markedly slower than this
   for (int k = 0; k < nbins; ++k)
-    launch_integrate(d, g, bins[k].list, bins[k].count, bins[k].maxn, precision, t, st);
+    launch_integrate(d, g, bins[k].list, bins[k].count, bins[k].maxn, precision, t, mid, st);
 }
 
 }  // namespace sc

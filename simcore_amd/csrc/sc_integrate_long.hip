@@ -179,7 +179,10 @@ __device__ __forceinline__ void det_backward(const double (&N)[S][4], double (&o
 // One wavefront's filaments: wave `wave` of a bin holds list entries
 // wave*64/G ... (wave+1)*64/G - 1.  Everything stays in registers; there is
 // no LDS and no barrier.
-template <int G, int S>
+// MID: the step is a midstep (compiled separately from the full step, so
+// each gets its own register allocation; the midstep's random-force solve
+// sets the larger one).
+template <int G, int S, bool MID>
 __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int32_t *list,
                                           int count, int wave) {
   constexpr int NG = 64 / G;
@@ -193,7 +196,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   const int n = factive ? (int)(d.site_off[f + 1] - s0) : 0;
   const int64_t b0 = s0 - f;
   const int64_t ns = g.ns;
-  const bool mid = ctl->midstep != 0;
+  constexpr bool mid = MID;
   const int64_t step = ctl->cur_step;
   const int m = n - 1;
   double bl = 1.0, kappa = 0.0, gpar = 1.0, gperp = 1.0, sgpar = 0.0, sgperp = 0.0;
@@ -561,20 +564,28 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       lmax = y > lmax ? y : lmax;
     }
     if (lane == 0)
-      atomicMax((unsigned long long *)&d.ctl->dr_bits,
+      atomicMax(&d.dr_slots[wave & (kDrSlots - 1)],
                 (unsigned long long)__double_as_longlong(lmax));
   }
 }
 
-template <int G, int S>
+template <int G, int S, bool MID>
 __global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int32_t *list,
                                                         int count) {
-  if (d.ctl->abort) return;
-  long_wave<G, S>(d, g, list, count, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  Ctl *ctl = d.ctl;
+  if (ctl->abort) return;
+  if ((ctl->midstep != 0) != MID) {  // launched for the wrong kind of step
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl->err = SC_ERR_STATE;
+      ctl->abort = 1;
+    }
+    return;
+  }
+  long_wave<G, S, MID>(d, g, list, count, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
 bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list, int bin_count,
-                           int bin_maxn, int precision, int waves, int layout,
+                           int bin_maxn, int precision, int waves, int layout, bool mid,
                            hipStream_t st) {
   if (precision == SC_PREC_FP64 || bin_maxn > 256) return false;
   // up to 64 sites: only in the two-sites-per-lane layout
@@ -584,8 +595,12 @@ bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list, 
   const int per_block = (64 / G) * waves;
   dim3 grid((bin_count + per_block - 1) / per_block), block(64 * waves);
 #define SC_LONG(GG, SS)                                                                   \
-  hipLaunchKernelGGL((k_integrate_long<GG, SS>), grid, block, 0, st, d, g, bin_list,     \
-                     bin_count)
+  if (mid)                                                                                \
+    hipLaunchKernelGGL((k_integrate_long<GG, SS, true>), grid, block, 0, st, d, g,       \
+                       bin_list, bin_count);                                              \
+  else                                                                                    \
+    hipLaunchKernelGGL((k_integrate_long<GG, SS, false>), grid, block, 0, st, d, g,      \
+                       bin_list, bin_count)
   switch (bin_maxn) {
     case 8: SC_LONG(4, 2); break;
     case 16: SC_LONG(8, 2); break;

@@ -299,7 +299,8 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
       lmax = y > lmax ? y : lmax;
     }
     if (lane == 0)
-      atomicMax((unsigned long long *)&d.ctl->dr_bits,
+      atomicMax(&d.dr_slots[(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) &
+                            (kDrSlots - 1)],
                 (unsigned long long)__double_as_longlong(lmax));
   }
 }

@@ -62,8 +62,15 @@ struct Geo {
 };
 
 // Device pointers (a plain struct so it can be passed by value).
+// Per-wave maxima of the displacement accumulator go to kDrSlots slots
+// (atomicMax, wave index modulo kDrSlots) rather than one word, so that
+// thousands of waves do not serialise on one address; the commit folds the
+// slots into Ctl::dr_bits and clears them.
+constexpr int kDrSlots = 256;
+
 struct Dev {
   Ctl *ctl;
+  unsigned long long *dr_slots;  // kDrSlots, zero between steps
   // filaments
   int32_t *site_off;       // nf+1
   int32_t *noise_off;      // nf: site index the noise stream is keyed by
@@ -126,12 +133,13 @@ void launch_commit(const Dev &d, const Geo &g, hipStream_t st);
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st);
 void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t,
                  hipStream_t st);
+// `mid`: the step is a midstep (odd step index)
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
-                      int bin_count, int bin_maxn, int precision, const Tune &t,
+                      int bin_count, int bin_maxn, int precision, const Tune &t, bool mid,
                       hipStream_t st);
 // all length bins of a step, one launch per bin
 void launch_integrate_all(const Dev &d, const Geo &g, const BinDesc *bins, int nbins,
-                          int precision, const Tune &t, hipStream_t st);
+                          int precision, const Tune &t, bool mid, hipStream_t st);
 // register-resident fast-path integrator; false when the bin is not one it
 // serves (FP64 precision or more than 64 sites), nothing launched then
 bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,
@@ -141,7 +149,7 @@ bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,
 // up to 64 sites when layout == 1
 bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list,
                            int bin_count, int bin_maxn, int precision, int waves,
-                           int layout, hipStream_t st);
+                           int layout, bool mid, hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
 // crosslinks (sc_crosslink.hip)

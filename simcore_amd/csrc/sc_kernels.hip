@@ -47,19 +47,38 @@ __device__ __forceinline__ void decide_next(Ctl *c, double thr) {
   c->rebuild = rb;
 }
 
+// Folds the per-wave displacement maxima into Ctl::dr_bits and clears the
+// slots; one 64-lane block (the launch of k_decide / k_commit).
+__device__ __forceinline__ void fold_dr_slots(Ctl *c, unsigned long long *slots) {
+  const int lane = threadIdx.x;
+  unsigned long long m = 0ull;
+  for (int i = lane; i < kDrSlots; i += 64) {
+    const unsigned long long v = slots[i];
+    m = v > m ? v : m;
+    slots[i] = 0ull;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(m, o, 64);
+    m = y > m ? y : m;
+  }
+  if (lane == 0 && m > c->dr_bits) c->dr_bits = m;
+}
+
 // standalone decision: after loading or after the host changed the state
-__global__ void k_decide(Ctl *c, double thr) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (c->abort) return;
-  decide_next(c, thr);
+__global__ void k_decide(Ctl *c, unsigned long long *slots, double thr) {
+  if (blockIdx.x != 0 || c->abort) return;
+  fold_dr_slots(c, slots);
+  if (threadIdx.x == 0) decide_next(c, thr);
 }
 
 // end of a step: commit it and decide the next one (one launch for both)
-__global__ void k_commit(Ctl *c, double thr) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (c->abort) return;
-  c->i_step = c->cur_step;
-  decide_next(c, thr);
+__global__ void k_commit(Ctl *c, unsigned long long *slots, double thr) {
+  if (blockIdx.x != 0 || c->abort) return;
+  fold_dr_slots(c, slots);
+  if (threadIdx.x == 0) {
+    c->i_step = c->cur_step;
+    decide_next(c, thr);
+  }
 }
 
 __global__ void k_zero_i32(int32_t *p, int64_t n) {
@@ -439,7 +458,8 @@ __global__ void k_redo_dr(Dev d, Geo g) {
     a = y > a ? y : a;
   }
   if ((threadIdx.x & 63) == 0)
-    atomicMax((unsigned long long *)&d.ctl->dr_bits,
+    atomicMax(&d.dr_slots[(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) &
+                          (kDrSlots - 1)],
               (unsigned long long)__double_as_longlong(a));
 }
 
@@ -461,11 +481,11 @@ void launch_copy_probe(const void *src, void *dst, int64_t n32, hipStream_t st) 
 }
 
 void launch_decide(const Dev &d, const Geo &g, hipStream_t st) {
-  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, st, d.ctl, g.thr);
+  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, st, d.ctl, d.dr_slots, g.thr);
 }
 
 void launch_commit(const Dev &d, const Geo &g, hipStream_t st) {
-  hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, d.ctl, g.thr);
+  hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, d.ctl, d.dr_slots, g.thr);
 }
 
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t st) {
