@@ -270,8 +270,8 @@ def main():
     ap.add_argument("--no-fp64-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch-shape", default="",
-                    help="pair_block,pair_xcd,integrate_waves,integrate_layout instead of "
-                         "autotuning "
+                    help="pair_block,pair_xcd,integrate_waves,integrate_layout"
+                         "[,integrate_streams] instead of autotuning "
                          "(pair_block 0: load-balanced pair kernel)")
     ap.add_argument("--caller-order", action="store_true",
                     help="keep filaments in generation order (no spatial storage order)")
@@ -319,7 +319,8 @@ def main():
     log(f"[bench] warmup {args.warmup} steps done, counters {dev.counters()}")
     tune = None
     if args.launch_shape:
-        keys = ["pair_block", "pair_xcd", "integrate_waves", "integrate_layout"]
+        keys = ["pair_block", "pair_xcd", "integrate_waves", "integrate_layout",
+                "integrate_streams"]
         shape = dict(zip(keys, (int(v) for v in args.launch_shape.split(","))))
         dev.set_launch_shape(**shape)
         tune = dict(dev.launch_shape(), set="fixed")

@@ -184,17 +184,19 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
  *   pair_xcd            1: consecutive bond ranges on one XCD
  *   integrate_waves     integrator waves per block, 1/2/4
  *   integrate_layout    filaments of up to 64 sites: one (0) or two (1,
- *                       default) sites per lane */
+ *                       default) sites per lane
+ *   integrate_streams   1: the integrator's length bins run concurrently on
+ *                       side streams (forked and joined inside the step) */
 typedef struct sc_launch_shape {
-  int32_t pair_block, pair_xcd, integrate_waves, integrate_layout;
+  int32_t pair_block, pair_xcd, integrate_waves, integrate_layout, integrate_streams;
 } sc_launch_shape;
 int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape);
 int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
 
 /* Autotuner (no counterpart in the reference): times the launch shapes
  * that leave every result unchanged -- first the pair kernel's (block size
- * or balanced variant, XCD order), then the integrator's (waves per block;
- * the layout is kept) -- over `steps` steps each
+ * or balanced variant, XCD order), then the integrator's (waves per block,
+ * side streams; the layout is kept) -- over `steps` steps each
  * from a checkpoint of the current state, keeps the fastest and restores
  * the state, so the trajectory is the one without tuning.  The cell size
  * is not tuned: in the reference it sets the rebuild threshold and the

@@ -62,6 +62,11 @@ struct sc_system {
   int grow_rounds = 0;
   hipEvent_t done_ev = nullptr;
   Ctl *ctl_host = nullptr;
+  // length bins spread over side streams (fork/join around the integrator,
+  // captured into the graphs as parallel branches)
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
   // allocations (from `arena` when set, else hipMalloc)
   sc::Arena *arena = nullptr;
   bool own_stream = true;
@@ -207,7 +212,39 @@ void enqueue_step(sc_system *h, bool odd) {
   sc::BinDesc bins[sc::kMaxBins];
   int nbins = 0;
   for (const Bin &b : h->bins) bins[nbins++] = {b.list, b.count, b.maxn};
-  sc::launch_integrate_all(d, g, bins, nbins, h->p.precision, h->tune, odd, st);
+  if (h->tune.int_streams && nbins > 1 && h->side[0]) {
+    // bins are independent (disjoint filaments; the displacement maxima
+    // combine by max): deal them to the main and side streams, largest
+    // first onto the least loaded, then join
+    const int lanes = std::min(nbins, 1 + sc_system::kSide);
+    int order[sc::kMaxBins];
+    for (int k = 0; k < nbins; ++k) order[k] = k;
+    std::sort(order, order + nbins, [&](int a, int b) {
+      return (int64_t)bins[a].count * bins[a].maxn > (int64_t)bins[b].count * bins[b].maxn;
+    });
+    int64_t load[1 + sc_system::kSide] = {};
+    int lane_of[sc::kMaxBins];
+    for (int j = 0; j < nbins; ++j) {
+      int best = 0;
+      for (int l = 1; l < lanes; ++l)
+        if (load[l] < load[best]) best = l;
+      lane_of[order[j]] = best;
+      load[best] += (int64_t)bins[order[j]].count * bins[order[j]].maxn;
+    }
+    (void)hipEventRecord(h->fork_ev, st);
+    for (int l = 1; l < lanes; ++l) (void)hipStreamWaitEvent(h->side[l - 1], h->fork_ev, 0);
+    for (int k = 0; k < nbins; ++k) {
+      hipStream_t sk = lane_of[k] == 0 ? st : h->side[lane_of[k] - 1];
+      sc::launch_integrate(d, g, bins[k].list, bins[k].count, bins[k].maxn, h->p.precision,
+                           h->tune, odd, sk);
+    }
+    for (int l = 1; l < lanes; ++l) {
+      (void)hipEventRecord(h->join_ev[l - 1], h->side[l - 1]);
+      (void)hipStreamWaitEvent(st, h->join_ev[l - 1], 0);
+    }
+  } else {
+    sc::launch_integrate_all(d, g, bins, nbins, h->p.precision, h->tune, odd, st);
+  }
   if (pf) {
     (void)hipEventRecord(h->ev[5], st);
     (void)hipEventRecord(h->ev[2], st);
@@ -418,10 +455,17 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
   }
   void *pin = nullptr;
   if (hipEventCreateWithFlags(&h->done_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipHostMalloc(&pin, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) {
     sc_destroy(h);
     return SC_ERR_HIP;
   }
+  for (int k = 0; k < sc_system::kSide; ++k)
+    if (hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->join_ev[k], hipEventDisableTiming) != hipSuccess) {
+      sc_destroy(h);
+      return SC_ERR_HIP;
+    }
   h->ctl_host = static_cast<Ctl *>(pin);
   *h->ctl_host = Ctl{};
   *out = h;
@@ -1193,15 +1237,19 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
       t.pair_xcd = x;
       consider(t);
     }
-  // integrator: waves per block (the layout, sites per lane, changes the
-  // rounding of the mixed path and is not tuned)
+  // integrator: waves per block and, with several length bins, bins on
+  // side streams (the layout, sites per lane, changes the rounding of the
+  // mixed path and is not tuned)
   const sc::Tune pair_best = best;
   const int waves[] = {1, 2, 4};
-  for (int w : waves) {
-    sc::Tune t = pair_best;
-    t.int_waves = w;
-    consider(t);
-  }
+  const int n_str = h->bins.size() > 1 ? 2 : 1;
+  for (int w : waves)
+    for (int sm = 0; sm < n_str; ++sm) {
+      sc::Tune t = pair_best;
+      t.int_waves = w;
+      t.int_streams = sm;
+      consider(t);
+    }
   // leave the system where it was, with the chosen shapes
   h->tune = rc ? start : best;
   drop_graph(h);
@@ -1229,7 +1277,8 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
        v.pair_block != 512) ||
       (v.pair_xcd != 0 && v.pair_xcd != 1) ||
       (v.integrate_waves != 1 && v.integrate_waves != 2 && v.integrate_waves != 4) ||
-      (v.integrate_layout != 0 && v.integrate_layout != 1))
+      (v.integrate_layout != 0 && v.integrate_layout != 1) ||
+      (v.integrate_streams != 0 && v.integrate_streams != 1))
     return fail(h, SC_ERR_ARG, "unsupported launch shape");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   h->tune.pair_bal = v.pair_block == 0 ? 1 : 0;
@@ -1237,6 +1286,7 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
   h->tune.pair_xcd = v.pair_xcd;
   h->tune.int_waves = v.integrate_waves;
   h->tune.int_layout = v.integrate_layout;
+  h->tune.int_streams = v.integrate_streams;
   drop_graph(h);
   return SC_OK;
 }
@@ -1250,6 +1300,7 @@ int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape) {
   shape->pair_xcd = t.pair_xcd;
   shape->integrate_waves = t.int_waves;
   shape->integrate_layout = t.int_layout;
+  shape->integrate_streams = t.int_streams;
   return SC_OK;
 }
 
@@ -1328,6 +1379,14 @@ int sc_destroy(sc_handle h) {
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->done_ev) (void)hipEventDestroy(h->done_ev);
+  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+  for (int k = 0; k < sc_system::kSide; ++k) {
+    if (h->side[k]) {
+      (void)hipStreamSynchronize(h->side[k]);
+      (void)hipStreamDestroy(h->side[k]);
+    }
+    if (h->join_ev[k]) (void)hipEventDestroy(h->join_ev[k]);
+  }
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
