@@ -481,9 +481,10 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
              R(64 * (int64_t)nf) +
              R(8 * (int64_t)nf) + R(4 * nb);
   b += 4 * R(24 * ns) + 2 * R(8 * ns);                    // r, rp, fr, dr0, acc, acc_prev
-  b += R(64 * nb) + 2 * R(24 * nb) + R(8 * nb);           // record, F, T, tension
+  b += R(64 * nb) + 2 * R(24 * nb) + R(8 * nb);           // record, F, T (at most), tension
   b += R(4 * nb) + R(4 * n_cells) + R(4 * (n_cells + 1));  // keys, cell count/start
   b += 3 * R(4 * nb) + R(4 * (nb + 1)) + R(4 * nb) + R(4 * (nb + 1));
+  b += R(4 * nb) + R(4 * ((nb + sc::kScanBlock - 1) / sc::kScanBlock + 1));  // sorted_o, tsum
   const int64_t tmp = std::max(scan_tmp_elems(n_cells), scan_tmp_elems(nb));
   b += R(4 * tmp);
   b += R(4 * (int64_t)nf) + (size_t)n_bins * Arena::kAlign;  // bin lists
@@ -568,16 +569,24 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = dalloc(h, &d.acc, ns, false))) return rc;
   if ((rc = dalloc(h, &d.acc_prev, ns, false))) return rc;
   if ((rc = dalloc(h, &d.br, 8 * (size_t)nb, false))) return rc;
-  if ((rc = dalloc(h, &d.bf, 3 * (size_t)nb, false))) return rc;
-  if ((rc = dalloc(h, &d.bt, 3 * (size_t)nb, false))) return rc;
+  if (h->p.precision == SC_PREC_FP64) {
+    if ((rc = dalloc(h, &d.bf, 3 * (size_t)nb, false))) return rc;
+    if ((rc = dalloc(h, &d.bt, 3 * (size_t)nb, false))) return rc;
+  } else {
+    if ((rc = dalloc(h, &d.bf32, 3 * (size_t)nb, false))) return rc;
+    if ((rc = dalloc(h, &d.bt32, 3 * (size_t)nb, false))) return rc;
+  }
   if ((rc = dalloc(h, &d.tension, nb, false))) return rc;
   if ((rc = dalloc(h, &d.keys, nb, false))) return rc;
   if ((rc = dalloc(h, &d.cell_count, g.n_cells, false))) return rc;
   if ((rc = dalloc(h, &d.cell_start, g.n_cells + 1, false))) return rc;
   if ((rc = dalloc(h, &d.sorted, nb, false))) return rc;
   if ((rc = dalloc(h, &d.skey, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.sorted_o, nb, false))) return rc;
   if ((rc = dalloc(h, &d.ccount, nb, false))) return rc;
   if ((rc = dalloc(h, &d.coff, nb + 1, false))) return rc;
+  if ((rc = dalloc(h, &d.tsum, (nb + sc::kScanBlock - 1) / sc::kScanBlock + 1, false)))
+    return rc;
   if ((rc = dalloc(h, &d.deg, nb, false))) return rc;
   if ((rc = dalloc(h, &d.inc_start, nb + 1, false))) return rc;
   int64_t tmp = std::max(sc::scan_tmp_elems(g.n_cells), sc::scan_tmp_elems(nb));
@@ -616,8 +625,13 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   HIPCHK(h, hipMemsetAsync(d.fr, 0, sizeof(double) * 3 * ns, h->stream));
   HIPCHK(h, hipMemsetAsync(d.acc, 0, sizeof(double) * ns, h->stream));
   HIPCHK(h, hipMemsetAsync(d.acc_prev, 0, sizeof(double) * ns, h->stream));
-  HIPCHK(h, hipMemsetAsync(d.bf, 0, sizeof(double) * 3 * nb, h->stream));
-  HIPCHK(h, hipMemsetAsync(d.bt, 0, sizeof(double) * 3 * nb, h->stream));
+  if (d.bf) {
+    HIPCHK(h, hipMemsetAsync(d.bf, 0, sizeof(double) * 3 * nb, h->stream));
+    HIPCHK(h, hipMemsetAsync(d.bt, 0, sizeof(double) * 3 * nb, h->stream));
+  } else {
+    HIPCHK(h, hipMemsetAsync(d.bf32, 0, sizeof(float) * 3 * nb, h->stream));
+    HIPCHK(h, hipMemsetAsync(d.bt32, 0, sizeof(float) * 3 * nb, h->stream));
+  }
   HIPCHK(h, hipMemsetAsync(d.tension, 0, sizeof(double) * nb, h->stream));
   HIPCHK(h, hipMemsetAsync(d.cell_count, 0, sizeof(int32_t) * g.n_cells, h->stream));
   HIPCHK(h, hipMemsetAsync(d.deg, 0, sizeof(int32_t) * nb, h->stream));
@@ -875,8 +889,21 @@ int sc_get_bond_forces(sc_handle h, double *f, double *t) {
   int rc = check_handle(h, true);
   if (rc) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  if (f && (rc = fetch_mapped(h, h->d.bf, f, h->g.nb, 3, false))) return rc;
-  if (t && (rc = fetch_mapped(h, h->d.bt, t, h->g.nb, 3, false))) return rc;
+  const int64_t nb = h->g.nb;
+  const Dev &d = h->d;
+  for (int which = 0; which < 2; ++which) {
+    double *out = which == 0 ? f : t;
+    if (!out) continue;
+    if (d.bf) {
+      if ((rc = fetch_mapped(h, which == 0 ? d.bf : d.bt, out, nb, 3, false))) return rc;
+    } else {  // mixed precision: FP32 sums widened to FP64
+      std::vector<float> v32(3 * (size_t)nb);
+      HIPCHK(h, hipMemcpy(v32.data(), which == 0 ? d.bf32 : d.bt32, sizeof(float) * v32.size(),
+                          hipMemcpyDeviceToHost));
+      std::vector<double> v(v32.begin(), v32.end());
+      to_caller(h, h->bond_map, v.data(), out, nb, 3);
+    }
+  }
   return SC_OK;
 }
 
@@ -1072,8 +1099,10 @@ std::vector<CkField> ck_fields(sc_system *h, int64_t nc) {
   return {{d.r, 24 * (size_t)ns},        {d.rp, 24 * (size_t)ns},
           {d.fr, 24 * (size_t)ns},       {d.dr0, 24 * (size_t)ns},
           {d.acc, 8 * (size_t)ns},       {d.acc_prev, 8 * (size_t)ns},
-          {d.br, 64 * (size_t)nb},       {d.bf, 24 * (size_t)nb},
-          {d.bt, 24 * (size_t)nb},       {d.tension, 8 * (size_t)nb},
+          {d.br, 64 * (size_t)nb},
+          {d.bf ? (void *)d.bf : (void *)d.bf32, (d.bf ? 24 : 12) * (size_t)nb},
+          {d.bt ? (void *)d.bt : (void *)d.bt32, (d.bt ? 24 : 12) * (size_t)nb},
+          {d.tension, 8 * (size_t)nb},
           {d.keys, 4 * (size_t)nb},      {d.norm, 8 * (size_t)nf},
           {d.pairs, 8 * (size_t)nc},     {slots, slot * (size_t)nc},
           {d.inc_start, 4 * (size_t)(nb + 1)}, {d.inc, 16 * (size_t)nc},
@@ -1165,6 +1194,11 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size) {
     p += f.bytes;
   }
   if ((rc = write_ctl(h, hd.ctl))) return rc;
+  // cell membership of the saved state's last rebuild (used between
+  // rebuilds by the crosslink partner search)
+  sc::launch_cells_from_keys(h->d, h->g, h->stream);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipGetLastError());
   *h->ctl_host = hd.ctl;
   h->i_step_host = hd.ctl.i_step;
   return SC_OK;

@@ -121,8 +121,8 @@ __global__ __launch_bounds__(64) void k_integrate(Dev d, Geo g,
       const int64_t j = b0 + i;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        Fb[k][q] = d.bf[q * nb + j];
-        Tb[k][q] = d.bt[q * nb + j];
+        Fb[k][q] = bond_force(d, nb, q, j);
+        Tb[k][q] = bond_torque(d, nb, q, j);
         U[q][i] = d.br[8 * j + 3 + q];
       }
     } else if (i == n - 1) {

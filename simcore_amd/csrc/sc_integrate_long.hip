@@ -216,29 +216,53 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   // ---- loads (values needed only at the end are loaded there, which keeps
   // register pressure, hence occupancy, in check) --------------------------
   double u[S][3], Fb[S][3], Tb[S][3], fr[S][3];
+  {
+    // bond orientations from the site positions with the arithmetic of
+    // bond_reinit, so the bits equal those of the stored bond record (which
+    // is then not read here); the last site takes bond m-1's
+    double rc[S][3], rnx[S][3], ub[S][3], ubm[S][3];
+    // an opaque copy of the pointer: the midstep's second load of r (at the
+    // end) must not be merged with this one, which would keep the positions
+    // in registers across the whole kernel
+    const double *rsrc = d.r;
+    asm volatile("" : "+s"(rsrc));
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int i = base + k;
+      for (int q = 0; q < 3; ++q) rc[k][q] = i < n ? rsrc[q * ns + s0 + i] : 0.0;
+    }
+    site_at3<G, S, 1>(rc, rnx, lane, gl);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      double o[3], l = 0.0;
+      for (int q = 0; q < 3; ++q) {
+        o[q] = rnx[k][q] - rc[k][q];
+        l += o[q] * o[q];
+      }
+      l = sqrt(l);
+      for (int q = 0; q < 3; ++q) ub[k][q] = o[q] / l;
+    }
+    site_at3<G, S, -1>(ub, ubm, lane, gl);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int i = base + k;
+      for (int q = 0; q < 3; ++q)
+        u[k][q] = i < m ? ub[k][q] : (i == m ? ubm[k][q] : (q == 0 ? 1.0 : 0.0));
+    }
+  }
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     const int i = base + k;
-    const int64_t si = s0 + i;
     for (int q = 0; q < 3; ++q) {
-      u[k][q] = q == 0 ? 1.0 : 0.0;
       Fb[k][q] = 0.0;
       Tb[k][q] = 0.0;
       fr[k][q] = 0.0;
     }
-    if (i < n) {
-      const int64_t jb = b0 + (i < m ? i : m - 1);
-      const double4 *rec = reinterpret_cast<const double4 *>(d.br + 8 * jb);
-      const double4 r0 = rec[0], r1 = rec[1];
-      u[k][0] = r0.w;
-      u[k][1] = r1.x;
-      u[k][2] = r1.y;
-      if (i < m)
-        for (int q = 0; q < 3; ++q) {
-          Fb[k][q] = d.bf[q * g.nb + jb];
-          Tb[k][q] = d.bt[q * g.nb + jb];
-        }
-    }
+    if (i < m)
+      for (int q = 0; q < 3; ++q) {
+        Fb[k][q] = (double)d.bf32[q * g.nb + b0 + i];
+        Tb[k][q] = (double)d.bt32[q * g.nb + b0 + i];
+      }
   }
 
   // ---- ApplyInteractionForces (src/filament.cpp:992-1048) ---------------

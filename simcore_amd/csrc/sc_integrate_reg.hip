@@ -57,18 +57,29 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
   // ---- loads -------------------------------------------------------------
   double rp[3] = {0, 0, 0}, u[3] = {1.0, 0, 0}, Fb[3] = {0, 0, 0}, Tb[3] = {0, 0, 0};
   double fr[3] = {0, 0, 0}, acc = 0.0, dr0[3] = {0, 0, 0};
+  {
+    // bond orientation from the site positions with the arithmetic of
+    // bond_reinit, so the bits equal those of the stored bond record (which
+    // is then not read here); the last site takes bond m-1's
+    double rc[3], o[3], ub[3], l = 0.0;
+    for (int q = 0; q < 3; ++q) rc[q] = act ? d.r[q * ns + si] : 0.0;
+    for (int q = 0; q < 3; ++q) {
+      o[q] = d_next(rc[q]) - rc[q];
+      l += o[q] * o[q];
+    }
+    l = sqrt(l);
+    for (int q = 0; q < 3; ++q) ub[q] = o[q] / l;
+    for (int q = 0; q < 3; ++q) {
+      const double pv = d_prev(ub[q]);
+      u[q] = i < m ? ub[q] : (i == m ? pv : (q == 0 ? 1.0 : 0.0));
+    }
+  }
   if (act) {
     for (int q = 0; q < 3; ++q) rp[q] = mid ? d.r[q * ns + si] : d.rp[q * ns + si];
-    const int64_t jb = b0 + (i < m ? i : m - 1);
-    const double4 *rec = reinterpret_cast<const double4 *>(d.br + 8 * jb);
-    const double4 r0 = rec[0], r1 = rec[1];
-    u[0] = r0.w;
-    u[1] = r1.x;
-    u[2] = r1.y;
     if (i < m) {
       for (int q = 0; q < 3; ++q) {
-        Fb[q] = d.bf[q * g.nb + jb];
-        Tb[q] = d.bt[q * g.nb + jb];
+        Fb[q] = (double)d.bf32[q * g.nb + b0 + i];
+        Tb[q] = (double)d.bt32[q * g.nb + b0 + i];
       }
     }
     if (!mid && g.stoch)

@@ -81,10 +81,12 @@ struct Dev {
   double *r, *rp, *fr, *dr0, *acc, *acc_prev;
   // bonds
   double *br;              // nb*8: {s.x, s.y, s.z, u.x, u.y, u.z, len, 0}
-  double *bf, *bt, *tension;
+  double *bf, *bt, *tension;  // bf, bt: FP64 precision only
+  float *bf32, *bt32;          // per-bond force / torque sums, mixed precision
   // cell list
   int32_t *keys, *cell_count, *cell_start, *sorted, *skey;
-  int32_t *ccount, *coff;
+  int32_t *sorted_o;       // cell runs in bond order
+  int32_t *ccount, *coff, *tsum;
   // candidates
   int2 *pairs;
   double4 *slot64;         // {Fx,Fy,Fz,-}
@@ -122,8 +124,19 @@ struct BinDesc {
 };
 
 
+// Per-bond force / torque component q of bond j (summed in FP64 by the pair
+// kernel; stored in FP32 on the mixed path).
+__device__ __forceinline__ double bond_force(const Dev &d, int64_t nb, int q, int64_t j) {
+  return d.bf32 ? (double)d.bf32[q * nb + j] : d.bf[q * nb + j];
+}
+__device__ __forceinline__ double bond_torque(const Dev &d, int64_t nb, int q, int64_t j) {
+  return d.bt32 ? (double)d.bt32[q * nb + j] : d.bt[q * nb + j];
+}
+
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
+// grid cap of streaming and scan passes (they loop over their range)
+constexpr int kStreamBlocks = 1024;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
 enum Guard { kGuardStep = 0, kGuardRebuild = 1, kGuardXlink = 2 };
@@ -132,6 +145,7 @@ enum Guard { kGuardStep = 0, kGuardRebuild = 1, kGuardXlink = 2 };
 void launch_decide(const Dev &d, const Geo &g, hipStream_t st);
 void launch_commit(const Dev &d, const Geo &g, hipStream_t st);
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st);
+void launch_cells_from_keys(const Dev &d, const Geo &g, hipStream_t st);
 void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t,
                  hipStream_t st);
 // `mid`: the step is a midstep (odd step index)

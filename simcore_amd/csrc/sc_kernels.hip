@@ -125,16 +125,19 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(
     const int32_t *in, int64_t n, int32_t *bsum, const Ctl *ctl, int guard) {
   if (guard_off(ctl, guard)) return;
   __shared__ int wsum[kScanBlock / 64];
-  int64_t base = (int64_t)blockIdx.x * kScanTile;
-  int s = 0;
+  const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t base = t * kScanTile;
+    int s = 0;
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    int64_t i = base + (int64_t)j * kScanBlock + threadIdx.x;
-    if (i < n) s += in[i];
+    for (int j = 0; j < kScanItems; ++j) {
+      const int64_t i = base + (int64_t)j * kScanBlock + threadIdx.x;
+      if (i < n) s += in[i];
+    }
+    int tot;
+    block_excl_scan256(s, &tot, wsum);
+    if (threadIdx.x == 0) bsum[t] = tot;
   }
-  int tot;
-  block_excl_scan256(s, &tot, wsum);
-  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
 __device__ void tile_scan(int *tile, int *wsum, int carry_in, int *total) {
@@ -159,7 +162,8 @@ __device__ void tile_scan(int *tile, int *wsum, int carry_in, int *total) {
 
 // Second pass: each tile adds up the sums of the tiles before it (at most a
 // few thousand, read from L2) and scans itself; the last tile also writes
-// the total.  Two launches per scan.
+// the total.  Two launches per scan; blocks loop over tiles, so any grid
+// size serves.
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
                                                           int64_t n,
                                                           const int32_t *bsum,
@@ -169,19 +173,46 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
   if (guard_off(ctl, guard)) return;
   __shared__ int tile[kScanTile];
   __shared__ int wsum[kScanBlock / 64];
-  int part = 0;
-  for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += kScanBlock) part += bsum[i];
-  int carry;
-  block_excl_scan256(part, &carry, wsum);
-  int64_t base = (int64_t)blockIdx.x * kScanTile;
-  for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
-    tile[j] = (base + j < n) ? in[base + j] : 0;
-  __syncthreads();
-  int tot;
-  tile_scan(tile, wsum, carry, &tot);
-  for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
-    if (base + j < n) out[base + j] = tile[j];
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry + tot;
+  const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int part = 0;
+    for (int64_t i = threadIdx.x; i < t; i += kScanBlock) part += bsum[i];
+    int carry;
+    block_excl_scan256(part, &carry, wsum);
+    const int64_t base = t * kScanTile;
+    for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
+      tile[j] = (base + j < n) ? in[base + j] : 0;
+    __syncthreads();
+    int tot;
+    tile_scan(tile, wsum, carry, &tot);
+    for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
+      if (base + j < n) out[base + j] = tile[j];
+    if (t == ntiles - 1 && threadIdx.x == 0) out[n] = carry + tot;
+    __syncthreads();
+  }
+}
+
+// Down pass of a scan whose tile sums (tiles of kScanBlock entries) the
+// producing kernel already wrote (k_count): one launch instead of two.
+__global__ __launch_bounds__(kScanBlock) void k_scan_small_tiles(const int32_t *in, int64_t n,
+                                                                 const int32_t *tsum,
+                                                                 int32_t *out, const Ctl *ctl,
+                                                                 int guard) {
+  if (guard_off(ctl, guard)) return;
+  __shared__ int wsum[kScanBlock / 64];
+  const int64_t ntiles = (n + kScanBlock - 1) / kScanBlock;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int part = 0;
+    for (int64_t i = threadIdx.x; i < t; i += kScanBlock) part += tsum[i];
+    int carry;
+    block_excl_scan256(part, &carry, wsum);
+    const int64_t i = t * kScanBlock + threadIdx.x;
+    const int v = i < n ? in[i] : 0;
+    int tot;
+    const int ex = block_excl_scan256(v, &tot, wsum) + carry;
+    if (i < n) out[i] = ex;
+    if (t == ntiles - 1 && threadIdx.x == 0) out[n] = carry + tot;
+  }
 }
 
 int64_t scan_tmp_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
@@ -201,45 +232,36 @@ void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
 // ---------------------------------------------------------------------------
 
 // CellList::FindCellCoords over every bond (src/cell_list.cpp:139-157) and
-// the per-cell population count.
+// the per-cell population count.  Streaming passes loop over their range,
+// so any grid size serves.
 __global__ void k_keys(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= g.nb) return;
-  const double *rec = d.br + 8 * b;
-  int32_t k = cell_key(g.n1d, rec[0], rec[1], rec[2]);
-  d.keys[b] = k;
-  atomicAdd(&d.cell_count[k], 1);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
+    const double *rec = d.br + 8 * b;
+    const int32_t k = cell_key(g.n1d, rec[0], rec[1], rec[2]);
+    d.keys[b] = k;
+    atomicAdd(&d.cell_count[k], 1);
+  }
 }
 
-// Counting-sort scatter; returns cell_count to all-zero for the next build.
-__global__ void k_scatter(Dev d, Geo g) {
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= g.nb) return;
-  int32_t k = d.keys[b];
-  int32_t pos = d.cell_start[k] + atomicSub(&d.cell_count[k], 1) - 1;
-  d.sorted[pos] = (int32_t)b;
-  d.skey[pos] = k;
+// Population count from the stored keys (cell list restore).
+__global__ void k_key_count(Dev d, Geo g) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride)
+    atomicAdd(&d.cell_count[d.keys[b]], 1);
 }
 
-// Cell::AddObj appends in interactor (bond index) order; the scatter above
-// is unordered, so each cell's run is sorted by bond index.
-__global__ void k_run_sort(Dev d, Geo g) {
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.nb) return;
-  int32_t k = d.skey[p];
-  if (d.cell_start[k] != p) return;
-  int32_t e = d.cell_start[k + 1];
-  for (int32_t i = (int32_t)p + 1; i < e; ++i) {
-    int32_t v = d.sorted[i];
-    int32_t j = i - 1;
-    while (j >= p && d.sorted[j] > v) {
-      d.sorted[j + 1] = d.sorted[j];
-      --j;
-    }
-    d.sorted[j + 1] = v;
+// Counting-sort scatter (order inside a cell arbitrary; k_count orders the
+// runs); returns cell_count to all-zero for the next build.
+__global__ void k_scatter(Dev d, Geo g, int guard) {
+  if (guard_off(d.ctl, guard)) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
+    const int32_t k = d.keys[b];
+    const int32_t pos = d.cell_start[k] + atomicSub(&d.cell_count[k], 1) - 1;
+    d.sorted[pos] = (int32_t)b;
+    d.skey[pos] = k;
   }
 }
 
@@ -296,33 +318,56 @@ __device__ __forceinline__ void load_shell(const Dev &d, const Geo &g, int32_t k
 // Candidate count of each occupied home cell (Cell::MakePairs,
 // src/cell.cpp:14-70), excluded pairs already removed (the reference culls
 // them on the rebuild step, src/interaction_manager.cpp:485-497).  One
-// thread per sorted position; the first position of a cell's run counts.
-__global__ __launch_bounds__(256) void k_count(Dev d, Geo g) {
+// thread per sorted position, one block per tile of kScanBlock positions;
+// the first position of a cell's run does the work:
+//  - writes its run ordered by bond index to sorted_o (Cell::AddObj appends
+//    in interactor order, src/cell.cpp:5-8);
+//  - counts the run's candidates from the unordered runs (a count needs no
+//    order);
+// and the tile's sum goes to tsum for the candidate-offset scan.
+__global__ __launch_bounds__(kScanBlock) void k_count(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.nb) return;
-  const int32_t k = d.skey[p];
+  __shared__ int wsum[kScanBlock / 64];
+  const int64_t t = blockIdx.x;
+  const int64_t p = t * kScanBlock + threadIdx.x;
   int32_t cnt = 0;
-  if (d.cell_start[k] == p) {
-    const int32_t e = d.cell_start[k + 1];
-    Shell sh;
-    load_shell(d, g, k, &sh);
-    for (int32_t i = (int32_t)p; i < e; ++i)
-      for (int32_t j = i + 1; j < e; ++j) cnt += !excluded(d.bond_fil, d.sorted[i], d.sorted[j]);
-    for (int32_t i = (int32_t)p; i < e; ++i) {
-      const int32_t bi = d.sorted[i];
+  if (p < g.nb) {
+    const int32_t k = d.skey[p];
+    if (d.cell_start[k] == p) {
+      const int32_t e = d.cell_start[k + 1];
+      Shell sh;
+      load_shell(d, g, k, &sh);
+      for (int32_t i = (int32_t)p; i < e; ++i) {  // insertion into sorted_o
+        const int32_t v = d.sorted[i];
+        int32_t j = i - 1;
+        while (j >= p && d.sorted_o[j] > v) {
+          d.sorted_o[j + 1] = d.sorted_o[j];
+          --j;
+        }
+        d.sorted_o[j + 1] = v;
+      }
+      for (int32_t i = (int32_t)p; i < e; ++i)
+        for (int32_t j = i + 1; j < e; ++j)
+          cnt += !excluded(d.bond_fil, d.sorted[i], d.sorted[j]);
+      for (int32_t i = (int32_t)p; i < e; ++i) {
+        const int32_t bi = d.sorted[i];
 #pragma unroll
-      for (int q = 0; q < 13; ++q)
-        for (int32_t j = sh.o0[q]; j < sh.o1[q]; ++j) cnt += !excluded(d.bond_fil, bi, d.sorted[j]);
+        for (int q = 0; q < 13; ++q)
+          for (int32_t j = sh.o0[q]; j < sh.o1[q]; ++j)
+            cnt += !excluded(d.bond_fil, bi, d.sorted[j]);
+      }
     }
+    d.ccount[p] = cnt;
   }
-  d.ccount[p] = cnt;
+  int tot;
+  block_excl_scan256(cnt, &tot, wsum);
+  if (threadIdx.x == 0) d.tsum[t] = tot;
 }
-
 
 // Writes the candidate list in MakePairs order: home cells ascending; in
 // each, the self pairs (i<j) then, for each of the 13 half-shell
-// neighbours in AssignCellNeighbors order, all home x neighbour pairs.
+// neighbours in AssignCellNeighbors order, all home x neighbour pairs; runs
+// read in bond order from sorted_o.
 __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
   // capacity check (every thread reads the total; one records it): an
@@ -348,6 +393,7 @@ __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
   load_shell(d, g, k, &sh);
   int32_t w = d.coff[p];
   auto put = [&](int32_t b1, int32_t b2) {
+    if (excluded(d.bond_fil, b1, b2)) return;
     d.pairs[w] = make_int2(b1, b2);
     if (prec == SC_PREC_FP64) {
       d.slot64[w] = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -359,18 +405,12 @@ __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
     ++w;
   };
   for (int32_t i = (int32_t)p; i < e; ++i)
-    for (int32_t j = i + 1; j < e; ++j) {
-      const int32_t bi = d.sorted[i], bj = d.sorted[j];
-      if (!excluded(d.bond_fil, bi, bj)) put(bi, bj);
-    }
+    for (int32_t j = i + 1; j < e; ++j) put(d.sorted_o[i], d.sorted_o[j]);
 #pragma unroll
   for (int q = 0; q < 13; ++q)
     for (int32_t i = (int32_t)p; i < e; ++i) {
-      const int32_t bi = d.sorted[i];
-      for (int32_t j = sh.o0[q]; j < sh.o1[q]; ++j) {
-        const int32_t bj = d.sorted[j];
-        if (!excluded(d.bond_fil, bi, bj)) put(bi, bj);
-      }
+      const int32_t bi = d.sorted_o[i];
+      for (int32_t j = sh.o0[q]; j < sh.o1[q]; ++j) put(bi, d.sorted_o[j]);
     }
 }
 
@@ -396,22 +436,25 @@ __global__ void k_inc_fill(Dev d, Geo g) {
 // src/mesh.cpp:546-551).
 __global__ void k_inc_sort(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) d.ctl->n_rebuilds += 1;
-  if (i < g.ns) {
-    for (int q = 0; q < 3; ++q) d.dr0[q * g.ns + i] = d.r[q * g.ns + i];
-    d.acc[i] = 0.0;
-  }
-  if (i >= g.nb) return;
-  const int32_t s = d.inc_start[i], e = d.inc_start[i + 1];
-  for (int32_t a = s + 1; a < e; ++a) {
-    const int2 v = d.inc[a];
-    int32_t j = a - 1;
-    while (j >= s && d.inc[j].x > v.x) {
-      d.inc[j + 1] = d.inc[j];
-      --j;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t n = g.ns > g.nb ? g.ns : g.nb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (i == 0) d.ctl->n_rebuilds += 1;
+    if (i < g.ns) {
+      for (int q = 0; q < 3; ++q) d.dr0[q * g.ns + i] = d.r[q * g.ns + i];
+      d.acc[i] = 0.0;
     }
-    d.inc[j + 1] = v;
+    if (i >= g.nb) continue;
+    const int32_t s = d.inc_start[i], e = d.inc_start[i + 1];
+    for (int32_t a = s + 1; a < e; ++a) {
+      const int2 v = d.inc[a];
+      int32_t j = a - 1;
+      while (j >= s && d.inc[j].x > v.x) {
+        d.inc[j + 1] = d.inc[j];
+        --j;
+      }
+      d.inc[j + 1] = v;
+    }
   }
 }
 
@@ -420,22 +463,39 @@ static inline unsigned nblocks(int64_t n, int bs) {
   return (unsigned)(b > 0 ? b : 1);
 }
 
+static inline unsigned sgrid(int64_t n) {
+  const unsigned b = nblocks(n, 256);
+  return b < (unsigned)kStreamBlocks ? b : (unsigned)kStreamBlocks;
+}
+
+// keys -> cell scan (2) -> scatter -> count (+ run order, tile sums) ->
+// candidate offsets (1) -> emit (+ degrees) -> degree scan (2) ->
+// incidence fill -> incidence order (+ displacement restart): 11 launches.
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
   const int bs = 256;
-  unsigned gb = nblocks(g.nb, bs);
+  const unsigned gb = nblocks(g.nb, bs);
   hipLaunchKernelGGL(k_keys, dim3(gb), dim3(bs), 0, st, d, g);
   launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, d.ctl,
               kGuardRebuild, st);
-  hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(bs), 0, st, d, g);
-  hipLaunchKernelGGL(k_run_sort, dim3(gb), dim3(bs), 0, st, d, g);
-  hipLaunchKernelGGL(k_count, dim3(gb), dim3(bs), 0, st, d, g);
-  launch_scan(d.ccount, d.coff, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
+  hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(bs), 0, st, d, g, (int)kGuardRebuild);
+  hipLaunchKernelGGL(k_count, dim3(nblocks(g.nb, kScanBlock)), dim3(kScanBlock), 0, st, d, g);
+  hipLaunchKernelGGL(k_scan_small_tiles, dim3(sgrid(g.nb)), dim3(kScanBlock), 0, st,
+                     d.ccount, g.nb, d.tsum, d.coff, d.ctl, kGuardRebuild);
   hipLaunchKernelGGL(k_emit, dim3(gb), dim3(bs), 0, st, d, g,
                      d.slot64 ? (int)SC_PREC_FP64 : (int)SC_PREC_MIXED);
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
   hipLaunchKernelGGL(k_inc_fill, dim3(2048), dim3(bs), 0, st, d, g);
   hipLaunchKernelGGL(k_inc_sort, dim3(nblocks(g.ns > g.nb ? g.ns : g.nb, bs)), dim3(bs), 0, st,
                      d, g);
+}
+
+// Cell membership (cell_start, sorted) of the last rebuild from the stored
+// keys; after a checkpoint restore the crosslink partner search needs the
+// list the saved state was stepping with.
+void launch_cells_from_keys(const Dev &d, const Geo &g, hipStream_t st) {
+  hipLaunchKernelGGL(k_key_count, dim3(sgrid(g.nb)), dim3(256), 0, st, d, g);
+  launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, nullptr, 0, st);
+  hipLaunchKernelGGL(k_scatter, dim3(sgrid(g.nb)), dim3(256), 0, st, d, g, 0);
 }
 
 // Replays the displacement bookkeeping of the last full step after the host

@@ -232,6 +232,22 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
   }
 }
 
+// Per-bond sums, accumulated in FP64; FP32 storage on the mixed path (one
+// rounding of the sum).
+template <bool MIXED>
+__device__ __forceinline__ void store_bond_sums(const Dev &d, int64_t nb, int64_t b,
+                                                const double *F, const double *T) {
+  for (int k = 0; k < 3; ++k) {
+    if (MIXED) {
+      d.bf32[k * nb + b] = (float)F[k];
+      d.bt32[k * nb + b] = (float)T[k];
+    } else {
+      d.bf[k * nb + b] = F[k];
+      d.bt[k * nb + b] = T[k];
+    }
+  }
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   if (d.ctl->abort) return;
@@ -274,10 +290,7 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
     for (int k = 0; k < 3; ++k) F[k] += cf[k];
     for (int k = 0; k < 3; ++k) T[k] += ct[k];
   }
-  for (int k = 0; k < 3; ++k) {
-    d.bf[k * nb + b] = F[k];
-    d.bt[k * nb + b] = T[k];
-  }
+  store_bond_sums<MIXED>(d, nb, b, F, T);
 }
 
 // Load-balanced variant: one wavefront per 64 bonds.  A bond's candidate
@@ -348,11 +361,7 @@ __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
     }
     __syncthreads();
   }
-  if (valid)
-    for (int k = 0; k < 3; ++k) {
-      d.bf[k * nb + b] = F[k];
-      d.bt[k * nb + b] = T[k];
-    }
+  if (valid) store_bond_sums<MIXED>(d, nb, b, F, T);
 }
 
 void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t, hipStream_t st) {

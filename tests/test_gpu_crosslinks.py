@@ -113,3 +113,28 @@ def test_crosslinks_checkpoint_resume():
         np.testing.assert_array_equal(x, y)
     for x, y in zip(a.crosslinks(), c.crosslinks()):
         np.testing.assert_array_equal(x, y)
+
+
+def test_crosslinks_restore_into_system_loaded_elsewhere():
+    """The rebinding search reads the cell membership of the last rebuild:
+    a restore must bring that back too.  The restoring system was loaded
+    with other positions (its own cell list unrelated to the saved state);
+    after the restore both runs must agree bit for bit, rebinding included."""
+    print("case: network mixed, restore into a system loaded from other positions", flush=True)
+    p, fil, b, lam = _network()
+    _, fil_other, _, _ = _network(seed=8)
+    a = sc.FilamentSystem(p, fil, precision="mixed", use_graph=True)
+    a.load_crosslinks(b, lam, **XL)
+    a.step(21)
+    blob = a.checkpoint()
+    ev0 = a.crosslinks()[3]
+    a.step(30)
+    assert a.crosslinks()[3] > ev0  # rebinding happened after the checkpoint
+    c = sc.FilamentSystem(p, fil_other, precision="mixed", use_graph=True)
+    c.load_crosslinks(b, lam, **XL)
+    c.restore(blob)
+    c.step(30)
+    for x, y in zip(a.sites(), c.sites()):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(a.crosslinks(), c.crosslinks()):
+        np.testing.assert_array_equal(x, y)
