@@ -240,14 +240,15 @@ def measured_traffic(tag, kernel):
 
 def pair_bytes(precision, nb, nc):
     """Algorithmic HBM bytes of one launch of the pair/scatter kernel (either
-    launch shape):
-    per bond its 64-byte state record read, its force and torque (6 FP64)
-    written and its incidence offset read; per candidate the sticky force
-    slot and its two incidence entries {candidate, partner} read.  Re-reads of
-    a partner's record and the conditional slot write of in-range pairs are
+    launch shape): per bond its 64-byte state record read, its force and
+    torque sums written (6 FP32 on the mixed path, 6 FP64 on the FP64 path)
+    and its incidence offset read; per candidate the sticky force slot and
+    its two incidence entries {candidate, partner} read.  Re-reads of a
+    partner's record and the conditional slot write of in-range pairs are
     not counted."""
     slot = 16 if precision == "mixed" else 32
-    return nb * (64 + 48 + 4) + nc * (slot + 2 * 8)
+    sums = 24 if precision == "mixed" else 48
+    return nb * (64 + sums + 4) + nc * (slot + 2 * 8)
 
 
 def timed_steps(sysobj, steps):
@@ -327,7 +328,7 @@ def main():
     elif not args.no_autotune:
         # launch shapes that leave results unchanged, chosen from a
         # checkpoint of the warmed-up state (the state is restored)
-        tune = dev.autotune(steps=20)
+        tune = dev.autotune(steps=40)
         log(f"[bench] autotune: {tune}")
     c0 = dev.counters()
     wall = timed_steps(dev, args.steps)
