@@ -224,6 +224,12 @@ int sc_profile_read(sc_handle h, double *avg_ms, int64_t *launches,
  * reports the average duration; used to calibrate hardware byte counters
  * and as the achievable-bandwidth reference next to the roofline. */
 int sc_copy_probe(size_t bytes, int repeats, double *avg_ms);
+/* Measurement probe with no counterpart in the reference: runs the pair
+ * pass `launches` times back to back on the handle's stream between two
+ * HIP events and reports the average duration of one launch (the roofline
+ * figure).  The pass only rewrites the same slot values and per-bond sums on
+ * an unchanged state, so the system is left as it was. */
+int sc_time_pair(sc_handle h, int32_t launches, double *avg_ms);
 
 const char *sc_last_error(sc_handle h);
 int sc_destroy(sc_handle h);

@@ -29,7 +29,8 @@ EXPORTS = [
     "sc_get_dr_state", "sc_get_counters", "sc_set_sites", "sc_device_bytes",
     "sc_profile_enable", "sc_profile_read", "sc_last_error", "sc_destroy",
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
-    "sc_generate_filaments", "sc_copy_probe", "sc_step_async", "sc_step_poll",
+    "sc_generate_filaments", "sc_copy_probe", "sc_time_pair", "sc_step_async",
+    "sc_step_poll",
     "sc_filament_stats", "sc_theta_histogram", "sc_output_header", "sc_spec_frame",
     "sc_posit_frame", "sc_checkpoint_save", "sc_checkpoint_load", "sc_autotune",
     "sc_set_launch_shape", "sc_get_launch_shape",
@@ -156,6 +157,7 @@ def _load():
         "sc_profile_read": ([H, dp, lp, c_int], c_int),
         "sc_last_error": ([H], c_char_p),
         "sc_copy_probe": ([c_size_t, c_int, dp], c_int),
+        "sc_time_pair": ([H, c_int32, dp], c_int),
         "sc_destroy": ([H], c_int),
         "sc_ensemble_create": ([POINTER(ScEnsembleParams), POINTER(H)], c_int),
         "sc_ensemble_submit": ([H, POINTER(ScParams), c_int32, ip, dp,

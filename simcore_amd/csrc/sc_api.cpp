@@ -1365,6 +1365,34 @@ int sc_profile_read(sc_handle h, double *avg_ms, int64_t *launches, int n) {
   return SC_OK;
 }
 
+int sc_time_pair(sc_handle h, int32_t launches, double *avg_ms) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (launches < 1 || !avg_ms) return fail(h, SC_ERR_ARG, "need launches >= 1 and an output");
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(h, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(h, SC_ERR_HIP, "hipEventCreate");
+  }
+  // the pass rewrites the same slot values and sums on an unchanged state,
+  // so repeating it leaves the system as it was
+  sc::launch_pair(h->d, h->g, h->p.precision, h->tune, h->stream);  // warm
+  (void)hipEventRecord(e0, h->stream);
+  for (int k = 0; k < launches; ++k)
+    sc::launch_pair(h->d, h->g, h->p.precision, h->tune, h->stream);
+  (void)hipEventRecord(e1, h->stream);
+  const hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(h, SC_ERR_HIP, hipGetErrorString(e));
+  *avg_ms = ms / launches;
+  return SC_OK;
+}
+
 int sc_copy_probe(size_t bytes, int repeats, double *avg_ms) {
   if (bytes < 32 || repeats <= 0) return SC_ERR_ARG;
   const int64_t n32 = (int64_t)(bytes / 32);

@@ -274,6 +274,14 @@ class FilamentSystem:
                           integrate_streams)
         _check(lib.sc_set_launch_shape(self.h, byref(v)), self.h, "sc_set_launch_shape")
 
+    def time_pair(self, launches=50):
+        """Average duration (ms) of one pair-pass launch, timed back to back
+        with HIP events on the system's stream (sc_time_pair); the state is
+        left unchanged."""
+        ms = c_double()
+        _check(lib.sc_time_pair(self.h, int(launches), byref(ms)), self.h, "sc_time_pair")
+        return ms.value
+
     def launch_shape(self):
         v = ScLaunchShape()
         _check(lib.sc_get_launch_shape(self.h, byref(v)), self.h, "sc_get_launch_shape")

@@ -400,3 +400,22 @@ def test_length_bins_on_side_streams_are_bit_identical(precision):
             np.testing.assert_array_equal(x, y)
         np.testing.assert_array_equal(runs[0][1], runs[j][1])
         assert runs[0][2] == runs[j][2]
+
+
+def test_pair_timing_probe_leaves_state_unchanged():
+    """sc_time_pair repeats the pair pass on an unchanged state: the run
+    continues exactly as one that was never timed."""
+    print("case: uneven short mixed, 21 steps, pair timing probe, 20 steps", flush=True)
+    from tests.helpers import uneven_system
+    p, fil = uneven_system()
+    a = sc.FilamentSystem(p, fil, precision="mixed", use_graph=True)
+    b = sc.FilamentSystem(p, fil, precision="mixed", use_graph=True)
+    a.step(21)
+    b.step(21)
+    assert b.time_pair(5) > 0.0
+    a.step(20)
+    b.step(20)
+    for x, y in zip(a.sites(), b.sites()):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(a.bond_forces(), b.bond_forces()):
+        np.testing.assert_array_equal(x, y)
