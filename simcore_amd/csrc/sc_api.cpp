@@ -485,6 +485,7 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   b += R(4 * nb) + R(4 * n_cells) + R(4 * (n_cells + 1));  // keys, cell count/start
   b += 3 * R(4 * nb) + R(4 * (nb + 1)) + R(4 * nb) + R(4 * (nb + 1));
   b += R(4 * nb) + R(4 * ((nb + sc::kScanBlock - 1) / sc::kScanBlock + 1));  // sorted_o, tsum
+  b += R((n_cells + 7) / 8 + 8);                                               // occupancy
   const int64_t tmp = std::max(scan_tmp_elems(n_cells), scan_tmp_elems(nb));
   b += R(4 * tmp);
   b += R(4 * (int64_t)nf) + (size_t)n_bins * Arena::kAlign;  // bin lists
@@ -583,6 +584,7 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = dalloc(h, &d.sorted, nb, false))) return rc;
   if ((rc = dalloc(h, &d.skey, nb, false))) return rc;
   if ((rc = dalloc(h, &d.sorted_o, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.cell_occ, (size_t)(g.n_cells + 7) / 8 + 8, false))) return rc;
   if ((rc = dalloc(h, &d.ccount, nb, false))) return rc;
   if ((rc = dalloc(h, &d.coff, nb + 1, false))) return rc;
   if ((rc = dalloc(h, &d.tsum, (nb + sc::kScanBlock - 1) / sc::kScanBlock + 1, false)))

@@ -86,6 +86,7 @@ struct Dev {
   // cell list
   int32_t *keys, *cell_count, *cell_start, *sorted, *skey;
   int32_t *sorted_o;       // cell runs in bond order
+  uint8_t *cell_occ;       // n_cells bits: cell occupied in the last build
   int32_t *ccount, *coff, *tsum;
   // candidates
   int2 *pairs;
@@ -180,7 +181,7 @@ void launch_zero_i32(int32_t *p, int64_t n, hipStream_t st);
 void launch_redo_dr(const Dev &d, const Geo &g, hipStream_t st);
 void launch_copy_probe(const void *src, void *dst, int64_t n32, hipStream_t st);
 void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
-                 const Ctl *ctl, int guard, hipStream_t st);
+                 const Ctl *ctl, int guard, hipStream_t st, uint8_t *occ = nullptr);
 int64_t scan_tmp_elems(int64_t n);
 
 }  // namespace sc

@@ -71,14 +71,47 @@ __global__ void k_decide(Ctl *c, unsigned long long *slots, double thr) {
   if (threadIdx.x == 0) decide_next(c, thr);
 }
 
-// end of a step: commit it and decide the next one (one launch for both)
+// end of a step: commit it and decide the next one (one launch for both).
+// Every value it needs is loaded up front, so the kernel costs one memory
+// round trip; the decision is decide_next's on those values.
 __global__ void k_commit(Ctl *c, unsigned long long *slots, double thr) {
-  if (blockIdx.x != 0 || c->abort) return;
-  fold_dr_slots(c, slots);
-  if (threadIdx.x == 0) {
-    c->i_step = c->cur_step;
-    decide_next(c, thr);
+  static_assert(kDrSlots == 4 * 64, "four slots per lane");
+  const int lane = threadIdx.x;
+  unsigned long long v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = slots[lane + 64 * i];
+  const int32_t abort = c->abort;
+  const int64_t cur = c->cur_step;
+  const unsigned long long drb = c->dr_bits;
+  if (blockIdx.x != 0 || abort) return;
+  unsigned long long m = drb;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    m = v[i] > m ? v[i] : m;
+    slots[lane + 64 * i] = 0ull;
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(m, o, 64);
+    m = y > m ? y : m;
+  }
+  if (lane != 0) return;
+  const int64_t k = cur + 1;  // the next step
+  c->i_step = cur;
+  c->cur_step = k;
+  c->midstep = (k & 1) ? 1 : 0;
+  int rb = 0;
+  if ((k & 1) && k >= 3) {
+    const double drm = __longlong_as_double((long long)m);
+    if (drm > 10000) {  // include/simcore/species.hpp:114-118
+      c->dr_bits = m;
+      c->abort = 1;
+      c->err = SC_ERR_UNPHYSICAL;
+      return;
+    }
+    if (drm > thr) rb = 1;
+  }
+  c->dr_bits = (k & 1) ? m : 0ull;  // a full step clears what it will fill
+  c->rebuild = rb;
 }
 
 __global__ void k_zero_i32(int32_t *p, int64_t n) {
@@ -164,12 +197,14 @@ __device__ void tile_scan(int *tile, int *wsum, int carry_in, int *total) {
 // few thousand, read from L2) and scans itself; the last tile also writes
 // the total.  Two launches per scan; blocks loop over tiles, so any grid
 // size serves.
+// With `occ` set, the pass also writes one bit per entry, set when the
+// entry (a cell's population) is non-zero: the cell-occupancy mask.
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
                                                           int64_t n,
                                                           const int32_t *bsum,
                                                           int32_t *out,
                                                           const Ctl *ctl,
-                                                          int guard) {
+                                                          int guard, uint8_t *occ) {
   if (guard_off(ctl, guard)) return;
   __shared__ int tile[kScanTile];
   __shared__ int wsum[kScanBlock / 64];
@@ -183,6 +218,14 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
     for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
       tile[j] = (base + j < n) ? in[base + j] : 0;
     __syncthreads();
+    if (occ && base + threadIdx.x * kScanItems < n) {
+      static_assert(kScanItems == 8, "one mask byte per thread");
+      unsigned bits = 0;
+#pragma unroll
+      for (int j = 0; j < kScanItems; ++j)
+        bits |= (tile[threadIdx.x * kScanItems + j] != 0 ? 1u : 0u) << j;
+      occ[(base >> 3) + threadIdx.x] = (uint8_t)bits;
+    }
     int tot;
     tile_scan(tile, wsum, carry, &tot);
     for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
@@ -218,13 +261,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_small_tiles(const int32_t *
 int64_t scan_tmp_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
 void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
-                 const Ctl *ctl, int guard, hipStream_t st) {
+                 const Ctl *ctl, int guard, hipStream_t st, uint8_t *occ) {
   int64_t nblk = (n + kScanTile - 1) / kScanTile;
   if (nblk == 0) nblk = 1;
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblk), dim3(kScanBlock), 0,
                      st, in, n, tmp, ctl, guard);
   hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nblk), dim3(kScanBlock), 0,
-                     st, in, n, tmp, out, ctl, guard);
+                     st, in, n, tmp, out, ctl, guard, occ);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,15 +347,36 @@ struct Shell {
   int32_t o0[13], o1[13];
 };
 
+// Empty cells (most of them) are recognised from the occupancy mask, a
+// small array that stays in L2, so only occupied neighbours read the large
+// cell_start array.
 __device__ __forceinline__ void load_shell(const Dev &d, const Geo &g, int32_t k, Shell *sh) {
   const int n = g.n1d;
   const int x = k / (n * n), y = (k / n) % n, z = k % n;
+  int32_t nbk[13];
+  unsigned occ = 0;
 #pragma unroll
   for (int q = 0; q < 13; ++q) {
-    const int32_t nb = shell_key(n, x, y, z, q);
-    sh->o0[q] = d.cell_start[nb];
-    sh->o1[q] = d.cell_start[nb + 1];
+    nbk[q] = shell_key(n, x, y, z, q);
+    occ |= ((d.cell_occ[nbk[q] >> 3] >> (nbk[q] & 7)) & 1u) << q;
   }
+#pragma unroll
+  for (int q = 0; q < 13; ++q) {
+    sh->o0[q] = 0;
+    sh->o1[q] = 0;
+    if ((occ >> q) & 1u) {
+      sh->o0[q] = d.cell_start[nbk[q]];
+      sh->o1[q] = d.cell_start[nbk[q] + 1];
+    }
+  }
+}
+
+// End of the run of cell k that starts at sorted position p, from the
+// sorted keys (sequential reads; runs are short).
+__device__ __forceinline__ int32_t run_end(const Dev &d, const Geo &g, int64_t p, int32_t k) {
+  int64_t e = p + 1;
+  while (e < g.nb && d.skey[e] == k) ++e;
+  return (int32_t)e;
 }
 
 // Candidate count of each occupied home cell (Cell::MakePairs,
@@ -333,8 +397,8 @@ __global__ __launch_bounds__(kScanBlock) void k_count(Dev d, Geo g) {
   int32_t cnt = 0;
   if (p < g.nb) {
     const int32_t k = d.skey[p];
-    if (d.cell_start[k] == p) {
-      const int32_t e = d.cell_start[k + 1];
+    if (p == 0 || d.skey[p - 1] != k) {  // first position of its cell's run
+      const int32_t e = run_end(d, g, p, k);
       Shell sh;
       load_shell(d, g, k, &sh);
       for (int32_t i = (int32_t)p; i < e; ++i) {  // insertion into sorted_o
@@ -387,8 +451,8 @@ __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.nb) return;
   const int32_t k = d.skey[p];
-  if (d.cell_start[k] != p) return;
-  const int32_t e = d.cell_start[k + 1];
+  if (p > 0 && d.skey[p - 1] == k) return;  // not the first of its run
+  const int32_t e = run_end(d, g, p, k);
   Shell sh;
   load_shell(d, g, k, &sh);
   int32_t w = d.coff[p];
@@ -476,7 +540,7 @@ void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
   const unsigned gb = nblocks(g.nb, bs);
   hipLaunchKernelGGL(k_keys, dim3(gb), dim3(bs), 0, st, d, g);
   launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, d.ctl,
-              kGuardRebuild, st);
+              kGuardRebuild, st, d.cell_occ);
   hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(bs), 0, st, d, g, (int)kGuardRebuild);
   hipLaunchKernelGGL(k_count, dim3(nblocks(g.nb, kScanBlock)), dim3(kScanBlock), 0, st, d, g);
   hipLaunchKernelGGL(k_scan_small_tiles, dim3(sgrid(g.nb)), dim3(kScanBlock), 0, st,
@@ -494,7 +558,7 @@ void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
 // list the saved state was stepping with.
 void launch_cells_from_keys(const Dev &d, const Geo &g, hipStream_t st) {
   hipLaunchKernelGGL(k_key_count, dim3(sgrid(g.nb)), dim3(256), 0, st, d, g);
-  launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, nullptr, 0, st);
+  launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, nullptr, 0, st, d.cell_occ);
   hipLaunchKernelGGL(k_scatter, dim3(sgrid(g.nb)), dim3(256), 0, st, d, g, 0);
 }
 
