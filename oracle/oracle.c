@@ -140,6 +140,46 @@ static inline void cross3(const double *a, const double *b, double *c) {
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
+
+/* Filament::ApplyInteractionForces (src/filament.cpp:992-1048), driving
+ * zero: bond i (orientation u, force F, torque T) adds F/2 + (u x T)/l to
+ * site i and F/2 - (u x T)/l to site i+1.  u, F, T per bond as three
+ * component arrays; fs accumulated per site, AoS (3 per site). */
+static void bond_site_forces(int n, double bl, const double *const u3[3],
+                             const double *const f3[3], const double *const t3[3],
+                             double *fs) {
+  double linv = 1.0 / bl;
+  for (int i = 0; i < n - 1; ++i) {
+    double F[3] = {f3[0][i], f3[1][i], f3[2][i]};
+    double T[3] = {t3[0][i], t3[1][i], t3[2][i]};
+    double u[3] = {u3[0][i], u3[1][i], u3[2][i]};
+    double pt[3], sf[3];
+    cross3(u, T, pt);
+    for (int d = 0; d < 3; ++d) {
+      pt[d] *= linv;
+      sf[d] = 0.5 * F[d];
+    }
+    for (int d = 0; d < 3; ++d) fs[3 * i + d] += sf[d];
+    for (int d = 0; d < 3; ++d) fs[3 * i + d] += pt[d];
+    for (int d = 0; d < 3; ++d) pt[d] *= -1;
+    for (int d = 0; d < 3; ++d) fs[3 * (i + 1) + d] += sf[d];
+    for (int d = 0; d < 3; ++d) fs[3 * (i + 1) + d] += pt[d];
+  }
+}
+
+void orc_bond_site_forces(int n_sites, double bond_length, const double *u, const double *f,
+                          const double *t, double *f_site) {
+  const int nb = n_sites - 1;
+  const double *U[3] = {u, u + nb, u + 2 * nb};
+  const double *F[3] = {f, f + nb, f + 2 * nb};
+  const double *T[3] = {t, t + nb, t + 2 * nb};
+  double *fs = (double *)calloc((size_t)3 * n_sites, sizeof(double));
+  if (!fs) return;
+  bond_site_forces(n_sites, bond_length, U, F, T, fs);
+  for (int i = 0; i < n_sites; ++i)
+    for (int d = 0; d < 3; ++d) f_site[d * n_sites + i] = fs[3 * i + d];
+  free(fs);
+}
 static inline void normalize3(double *a) {
   double m = 0.0;
   m += a[0] * a[0];
@@ -686,23 +726,10 @@ static void update_filament(orc_sys *s, int f, int midstep, double *wk) {
 
   /* ApplyInteractionForces (src/filament.cpp:992-1048); driving is zero */
   {
-    double linv = 1.0 / bl;
-    for (int i = 0; i < n - 1; ++i) {
-      double F[3] = {s->bf[0][b0 + i], s->bf[1][b0 + i], s->bf[2][b0 + i]};
-      double T[3] = {s->bt[0][b0 + i], s->bt[1][b0 + i], s->bt[2][b0 + i]};
-      double u[3] = {SU(i, 0), SU(i, 1), SU(i, 2)};
-      double pt[3], sf[3];
-      cross3(u, T, pt);
-      for (int d = 0; d < 3; ++d) {
-        pt[d] *= linv;
-        sf[d] = 0.5 * F[d];
-      }
-      for (int d = 0; d < 3; ++d) FS(i, d) += sf[d];
-      for (int d = 0; d < 3; ++d) FS(i, d) += pt[d];
-      for (int d = 0; d < 3; ++d) pt[d] *= -1;
-      for (int d = 0; d < 3; ++d) FS(i + 1, d) += sf[d];
-      for (int d = 0; d < 3; ++d) FS(i + 1, d) += pt[d];
-    }
+    const double *U[3] = {s->u[0] + b0, s->u[1] + b0, s->u[2] + b0};
+    const double *BF[3] = {s->bf[0] + b0, s->bf[1] + b0, s->bf[2] + b0};
+    const double *BT[3] = {s->bt[0] + b0, s->bt[1] + b0, s->bt[2] + b0};
+    bond_site_forces(n, bl, U, BF, BT, fs);
   }
   /* CalculateAngles (src/filament.cpp:430-437) */
   for (int i = 0; i < n - 2; ++i) {

@@ -160,6 +160,11 @@ void orc_noise(uint64_t seed, int64_t site, int64_t m, double xi[3]);
  * (3 x n_sites SoA, overwritten, not accumulated) and g_inv / k_eff. */
 void orc_bending_forces(int n_sites, const double *u, double kappa,
                         double bond_length, double *f_site, double *k_eff);
+/* Filament::ApplyInteractionForces for one filament: per-bond orientation,
+ * force and torque (3 x nb SoA each, nb = n_sites-1) to site forces
+ * (3 x n_sites SoA, overwritten). */
+void orc_bond_site_forces(int n_sites, double bond_length, const double *u, const double *f,
+                          const double *t, double *f_site);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,7 @@ def _load():
         "orc_philox4x32": ([POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)], None),
         "orc_noise": ([c_uint64, c_int64, c_int64, dp], None),
         "orc_bending_forces": ([c_int, dp, c_double, c_double, dp, dp], None),
+        "orc_bond_site_forces": ([c_int, c_double, dp, dp, dp, dp], None),
         "orc_load_crosslinks": ([V, c_int64, ip, dp, POINTER(OrcXlinkParams)], c_int),
         "orc_get_crosslinks": ([V, ip, dp, dp], None),
         "orc_xlink_events": ([V], c_int64),
@@ -261,6 +262,19 @@ def bending_forces(u_soa, kappa, bl):
     u = np.ascontiguousarray(u_soa, dtype=np.float64)
     lib.orc_bending_forces(n, _dp(u), kappa, bl, _dp(f), _dp(ke))
     return f, ke[:n - 2]
+
+
+def bond_site_forces(u_soa, f_soa, t_soa, bl):
+    """Site forces (3 x n SoA) of one filament from per-bond orientation,
+    force and torque (3 x nb SoA each)."""
+    nb = len(u_soa) // 3
+    n = nb + 1
+    out = np.empty(3 * n)
+    u = np.ascontiguousarray(u_soa, dtype=np.float64)
+    f = np.ascontiguousarray(f_soa, dtype=np.float64)
+    t = np.ascontiguousarray(t_soa, dtype=np.float64)
+    lib.orc_bond_site_forces(n, bl, _dp(u), _dp(f), _dp(t), _dp(out))
+    return out
 
 
 def xlink_force(h, h_inv, p0, p1, k, r0, kc=-1.0):

@@ -352,3 +352,59 @@ def test_unphysical_displacement_is_an_error():
     # step 3 checks the displacement (the last update was a full step):
     # 2 * (2e4)^2 > 10000 -> the reference throws
     assert o.step(1) == -1
+
+
+def test_bond_forces_to_sites_keep_net_force_and_perpendicular_torque():
+    """A14 (src/filament.cpp:992-1048): each bond's force F and torque T
+    become two site forces whose sum is F and whose moment about the bond
+    centre is the torque's part perpendicular to the bond, T - u(u.T);
+    contributions of different bonds add."""
+    from oracle.oracle_py import bond_site_forces
+    rng = np.random.default_rng(5)
+    n, bl = 7, 0.83
+    nb = n - 1
+    u = rng.normal(size=(3, nb))
+    u /= np.linalg.norm(u, axis=0)
+    F = rng.normal(size=(3, nb))
+    T = rng.normal(size=(3, nb))
+    total = bond_site_forces(u.ravel(), F.ravel(), T.ravel(), bl).reshape(3, n)
+    acc = np.zeros((3, n))
+    for i in range(nb):
+        Fi = np.zeros((3, nb))
+        Ti = np.zeros((3, nb))
+        Fi[:, i] = F[:, i]
+        Ti[:, i] = T[:, i]
+        fs = bond_site_forces(u.ravel(), Fi.ravel(), Ti.ravel(), bl).reshape(3, n)
+        acc += fs
+        others = [j for j in range(n) if j not in (i, i + 1)]
+        assert np.all(fs[:, others] == 0.0)
+        fa, fb = fs[:, i], fs[:, i + 1]
+        np.testing.assert_allclose(fa + fb, F[:, i], rtol=0, atol=1e-13)
+        half = 0.5 * bl * u[:, i]  # site i at centre - half, site i+1 at centre + half
+        moment = np.cross(-half, fa) + np.cross(half, fb)
+        t_perp = T[:, i] - u[:, i] * np.dot(u[:, i], T[:, i])
+        np.testing.assert_allclose(moment, t_perp, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(total, acc, rtol=0, atol=1e-12)
+
+
+def test_bond_lengths_stay_within_the_renormalisation_threshold():
+    """A21 (src/filament.cpp:840-951): after every step each bond length
+    stays within 1e-3 relative of the filament's bond length (the update
+    re-places the sites whenever a bond leaves that band)."""
+    from tests.helpers import uneven_system
+    p, fil = uneven_system()
+    ora = O.OracleSystem(p, fil)
+    so = np.asarray(fil.site_off)
+    bl = fil.fil_array("bond_length")
+    worst = 0.0
+    for _ in range(20):
+        assert ora.step(5) == 0
+        r = ora.sites()[0]
+        ns = len(r) // 3
+        x = r.reshape(3, ns)
+        for f in range(len(so) - 1):
+            seg = x[:, so[f]:so[f + 1]]
+            ln = np.linalg.norm(np.diff(seg, axis=1), axis=0)
+            worst = max(worst, float(np.max(np.abs(ln - bl[f]) / bl[f])))
+    print("worst relative bond-length deviation", worst)
+    assert worst <= 1e-3
