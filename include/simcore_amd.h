@@ -37,8 +37,10 @@ extern "C" {
 
 /* arithmetic of the pair path */
 #define SC_PREC_FP64 0  /* everything FP64, bit-compatible with the CPU path */
-#define SC_PREC_MIXED 1 /* FP32 distance/WCA math, FP64 minimum image,
-                           near-parallel branch, tensions and positions      */
+#define SC_PREC_MIXED 1 /* FP64 pair geometry with the hardware FP32
+                           reciprocal / reciprocal square root, FP32 sticky
+                           forces and per-bond sums, FP64 tensions and
+                           positions                                         */
 
 typedef struct sc_system *sc_handle;
 
