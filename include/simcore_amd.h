@@ -187,8 +187,9 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
  *   integrate_waves     integrator waves per block, 1/2/4
  *   integrate_layout    filaments of up to 64 sites: one (0) or two (1,
  *                       default) sites per lane
- *   integrate_streams   1: the integrator's length bins run concurrently on
- *                       side streams (forked and joined inside the step) */
+ *   integrate_streams   0-3: the integrator's length bins are spread over the
+ *                       system's stream and this many side streams (forked
+ *                       and joined inside the step) and run concurrently */
 typedef struct sc_launch_shape {
   int32_t pair_block, pair_xcd, integrate_waves, integrate_layout, integrate_streams;
 } sc_launch_shape;

@@ -216,7 +216,7 @@ void enqueue_step(sc_system *h, bool odd) {
     // bins are independent (disjoint filaments; the displacement maxima
     // combine by max): deal them to the main and side streams, largest
     // first onto the least loaded, then join
-    const int lanes = std::min(nbins, 1 + sc_system::kSide);
+    const int lanes = std::min(nbins, 1 + std::min(h->tune.int_streams, sc_system::kSide));
     int order[sc::kMaxBins];
     for (int k = 0; k < nbins; ++k) order[k] = k;
     std::sort(order, order + nbins, [&](int a, int b) {
@@ -1278,7 +1278,7 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   // mixed path and is not tuned)
   const sc::Tune pair_best = best;
   const int waves[] = {1, 2, 4};
-  const int n_str = h->bins.size() > 1 ? 2 : 1;
+  const int n_str = h->bins.size() > 1 ? 1 + sc_system::kSide : 1;  // 0..3 side streams
   for (int w : waves)
     for (int sm = 0; sm < n_str; ++sm) {
       sc::Tune t = pair_best;
@@ -1314,7 +1314,7 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
       (v.pair_xcd != 0 && v.pair_xcd != 1) ||
       (v.integrate_waves != 1 && v.integrate_waves != 2 && v.integrate_waves != 4) ||
       (v.integrate_layout != 0 && v.integrate_layout != 1) ||
-      (v.integrate_streams != 0 && v.integrate_streams != 1))
+      (v.integrate_streams < 0 || v.integrate_streams > 3))
     return fail(h, SC_ERR_ARG, "unsupported launch shape");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   h->tune.pair_bal = v.pair_block == 0 ? 1 : 0;

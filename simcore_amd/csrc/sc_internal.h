@@ -113,7 +113,7 @@ struct Tune {
   int int_waves = 1;     // waves per block of the register integrator
   int int_layout = 1;    // up to 64 sites: 0 one site per lane, 1 two (the
                          // two round differently; not changed by tuning)
-  int int_streams = 0;   // 1: length bins on side streams, concurrently
+  int int_streams = 0;   // length bins spread over this many side streams (0-3)
 };
 
 // Length bins of the integrator, as the step loop hands them over.

@@ -321,8 +321,8 @@ def test_autotune_leaves_trajectory_unchanged(precision):
     b.step(31)
     rep = b.autotune(steps=6)
     print(rep)
-    # 8 pair shapes, then 3 waves per block x bins on side streams or not
-    assert rep["n_trials"] == 14 and rep["step_ms_best"] <= rep["step_ms_start"]
+    # 8 pair shapes, then 3 waves per block x 0-3 side streams
+    assert rep["n_trials"] == 20 and rep["step_ms_best"] <= rep["step_ms_start"]
     assert b.counters()["i_step"] == 31
     a.step(40)
     b.step(40)
@@ -389,13 +389,13 @@ def test_length_bins_on_side_streams_are_bit_identical(precision):
     from tests.helpers import uneven_system
     p, fil = uneven_system()
     runs = []
-    for streams, graph in ((0, True), (1, True), (1, False)):
+    for streams, graph in ((0, True), (1, True), (3, True), (3, False)):
         s = sc.FilamentSystem(p, fil, precision=precision, use_graph=graph)
         s.set_launch_shape(pair_block=0, integrate_waves=2, integrate_streams=streams)
         s.step(41)
         runs.append((s.sites(), s.tensions(), s.counters()))
         s.close()
-    for j in (1, 2):
+    for j in (1, 2, 3):
         for x, y in zip(runs[0][0], runs[j][0]):
             np.testing.assert_array_equal(x, y)
         np.testing.assert_array_equal(runs[0][1], runs[j][1])
