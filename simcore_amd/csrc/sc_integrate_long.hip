@@ -4,7 +4,7 @@
 // 64/G filaments per wavefront.
 //
 // Group lane l owns the S consecutive sites lS .. lS+S-1, so a neighbour is
-// either in the lane's own registers or one lane-shuffle away.
+// either in the lane's own registers or one lane away (a DPP move).
 // Tridiagonal systems: log2(S) levels of cyclic reduction inside each lane
 // leave one row per lane, solved by parallel cyclic reduction across the
 // group's lanes, then back-substituted level by level.  Metric
@@ -38,7 +38,7 @@ __device__ __forceinline__ void site_at(const double (&v)[S], double (&o)[S], in
       o[k] = v[ks];
     } else {
       const bool ok = gl + q >= 0 && gl + q < G;
-      const double y = __shfl(v[ks], ok ? lane + q : lane, 64);
+      const double y = lane_shift_c<G>(v[ks], lane, q);
       o[k] = ok ? y : fb;
     }
   }
@@ -81,9 +81,8 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
     // shuffles on every lane (a lane-dependent condition around them would
     // switch off the lanes they read from), then the boundary select
     const bool first = gl == 0;
-    const int src = first ? lane : lane - 1;
-    const double ya = __shfl(a[S - st], src, 64), yb = __shfl(b[S - st], src, 64);
-    const double yc = __shfl(c[S - st], src, 64), yd = __shfl(d[S - st], src, 64);
+    const double ya = lane_shift<G, -1>(a[S - st], lane), yb = lane_shift<G, -1>(b[S - st], lane);
+    const double yc = lane_shift<G, -1>(c[S - st], lane), yd = lane_shift<G, -1>(d[S - st], lane);
     const double pa = first ? 0.0 : ya, pb = first ? 1.0 : yb;
     const double pc = first ? 0.0 : yc, pd = first ? 0.0 : yd;
 #pragma unroll
@@ -104,7 +103,7 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
 #pragma unroll
   for (int st = S / 2; st >= 1; st >>= 1) {
     const bool last = gl == G - 1;
-    const double nx = __shfl(x[0], last ? lane : lane + 1, 64);
+    const double nx = lane_shift<G, 1>(x[0], lane);
 #pragma unroll
     for (int k = st; k < S; k += 2 * st) {
       const double xl = x[k - st];
@@ -137,7 +136,7 @@ __device__ __forceinline__ void det_forward(const double (&M)[S][4], double (&ou
   const double id[4] = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const double y = __shfl(A[q], gl > 0 ? lane - 1 : lane, 64);
+    const double y = lane_shift<G, -1>(A[q], lane);
     E[q] = gl > 0 ? y : id[q];
   }
 #pragma unroll
@@ -163,7 +162,7 @@ __device__ __forceinline__ void det_backward(const double (&N)[S][4], double (&o
   const double id[4] = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const double y = __shfl(B[q], gl < G - 1 ? lane + 1 : lane, 64);
+    const double y = lane_shift<G, 1>(B[q], lane);
     F[q] = gl < G - 1 ? y : id[q];
   }
 #pragma unroll
@@ -523,7 +522,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
         }
         tot = run;
         const double incl = sum_scan<G>(tot, lane, gl, -1);
-        const double y = __shfl(incl, gl > 0 ? lane - 1 : lane, 64);
+        const double y = lane_shift<G, -1>(incl, lane);
         const double before = gl > 0 ? y : 0.0;
         const double r0 = __shfl(rn[0][q], lane - gl, 64);
 #pragma unroll
@@ -539,7 +538,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
         }
         tot = run;
         const double incl = sum_scan<G>(tot, lane, gl, +1);
-        const double y = __shfl(incl, gl < G - 1 ? lane + 1 : lane, 64);
+        const double y = lane_shift<G, 1>(incl, lane);
         const double after = gl < G - 1 ? y : 0.0;
         double pick = 0.0;
 #pragma unroll
