@@ -11,7 +11,10 @@
 // determinants: 2x2 matrix products, sequential inside a lane, scanned
 // across lanes.  Renormalisation: prefix sums the same way.  No LDS, no
 // barriers.  Against one site per lane (sc_integrate_reg.hip) this halves
-// the lanes, hence the shuffles, per filament.  Same reduced algebra.
+// the lanes, hence the shuffles, per filament.  Same reduced algebra;
+// reciprocals (lengths, pivots, determinants) by rcp_fast rather than IEEE
+// division, bond records written by bond_store from the new lengths and
+// unit vectors.
 #include <hip/hip_runtime.h>
 
 #include "sc_internal.h"
@@ -90,7 +93,7 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
       const double la = k == 0 ? pa : a[k - st], lb = k == 0 ? pb : b[k - st];
       const double lc = k == 0 ? pc : c[k - st], ld = k == 0 ? pd : d[k - st];
       const double ua = a[k + st], ub = b[k + st], uc = c[k + st], ud = d[k + st];
-      const double al = -a[k] / lb, ga = -c[k] / ub;
+      const double al = -a[k] * rcp_fast(lb), ga = -c[k] * rcp_fast(ub);
       a[k] = al * la;
       c[k] = ga * uc;
       b[k] = b[k] + al * lc + ga * ua;
@@ -108,7 +111,7 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
     for (int k = st; k < S; k += 2 * st) {
       const double xl = x[k - st];
       const double xu = k + st < S ? x[k + st] : (last ? 0.0 : nx);
-      x[k] = (d[k] - a[k] * xl - c[k] * xu) / b[k];
+      x[k] = (d[k] - a[k] * xl - c[k] * xu) * rcp_fast(b[k]);
     }
   }
 }
@@ -238,8 +241,8 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
         o[q] = rnx[k][q] - rc[k][q];
         l += o[q] * o[q];
       }
-      l = sqrt(l);
-      for (int q = 0; q < 3; ++q) ub[k][q] = o[q] / l;
+      const double il = rcp_fast(sqrt(l));
+      for (int q = 0; q < 3; ++q) ub[k][q] = o[q] * il;
     }
     site_at3<G, S, -1>(ub, ubm, lane, gl);
 #pragma unroll
@@ -265,7 +268,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   }
 
   // ---- ApplyInteractionForces (src/filament.cpp:992-1048) ---------------
-  const double linv = 1.0 / bl;
+  const double linv = rcp_fast(bl);
   double sf[S][3], pt[S][3], sfm[S][3], ptm[S][3], fs[S][3];
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -299,7 +302,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       for (int q = 0; q < 3; ++q) t[k][q] = u[k][q];
     } else {
       for (int q = 0; q < 3; ++q) t[k][q] = um[k][q] + u[k][q];
-      const double inv = 1.0 / sqrt(dot(t[k], t[k]));
+      const double inv = rcp_fast(sqrt(dot(t[k], t[k])));
       for (int q = 0; q < 3; ++q) t[k][q] *= inv;
     }
   }
@@ -395,7 +398,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       if (i + 3 >= n) db3[k] = 1.0;
       ke[k] = 0.0;
       if (i < n - 2) {
-        const double ginv = cs[k] * det_t[k] * db3[k] * (1.0 / det_g);
+        const double ginv = cs[k] * det_t[k] * db3[k] * rcp_fast(det_g);
         ke[k] = (kappa + bl * ginv) * (linv * linv);
       }
     }
@@ -425,7 +428,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   }
 
   // ---- tensions (src/filament.cpp:778-838) -------------------------------
-  const double igpar = 1.0 / gpar, igperp = 1.0 / gperp, dg = igpar - igperp;
+  const double igpar = rcp_fast(gpar), igperp = rcp_fast(gperp), dg = igpar - igperp;
   double T[S], Tm[S];
   {
     double tn[S][3], fn[S][3], hu[S], hun[S], a[S], b[S], c[S], rh[S];
@@ -482,6 +485,8 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     mob(igperp, dg, t[k], fs[k], mv);
     for (int q = 0; q < 3; ++q) rn[k][q] = rp[q] + delta * mv[q];
   }
+  // new bond lengths mm and unit vectors uo, kept for the bond records
+  double mm[S];
   site_at3<G, S, 1>(rn, rnn, lane, gl);
   int bad = 0;
 #pragma unroll
@@ -489,12 +494,12 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     const int i = base + k;
     double rd[3];
     for (int q = 0; q < 3; ++q) rd[q] = rnn[k][q] - rn[k][q];
-    const double mm = sqrt(dot(rd, rd));
-    const double inv = 1.0 / mm;
+    mm[k] = sqrt(dot(rd, rd));
+    const double inv = rcp_fast(mm[k]);
     for (int q = 0; q < 3; ++q) uo[k][q] = rd[q] * inv;
     if (i < m) {
-      const double diff = bl - mm;
-      if ((diff < 0 ? -diff : diff) / bl > 1e-3) bad = 1;
+      const double diff = bl - mm[k];
+      if ((diff < 0 ? -diff : diff) * linv > 1e-3) bad = 1;
     }
   }
   const unsigned long long bal = __ballot(bad);
@@ -551,6 +556,14 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       }
     }
     site_at3<G, S, 1>(rn, rnn, lane, gl);
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      double rd[3];
+      for (int q = 0; q < 3; ++q) rd[q] = rnn[k][q] - rn[k][q];
+      mm[k] = sqrt(dot(rd, rd));
+      const double inv = rcp_fast(mm[k]);
+      for (int q = 0; q < 3; ++q) uo[k][q] = rd[q] * inv;
+    }
     if (gl == 0 && factive) {
       d.norm[2 * f] = sw;
       d.norm[2 * f + 1] = 1;
@@ -567,7 +580,12 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     if (i >= n) continue;
     const int64_t si = s0 + i;
     for (int q = 0; q < 3; ++q) d.r[q * ns + si] = rn[k][q];
-    if (i < m) bond_reinit(d, g, b0 + i, rn[k], rnn[k], SC_PREC_MIXED);
+    if (i < m) {
+      // the record bond_reinit (mixed) computes, from the values at hand
+      double rd[3];
+      for (int q = 0; q < 3; ++q) rd[q] = rnn[k][q] - rn[k][q];
+      bond_store(d, g, b0 + i, rn[k], rd, mm[k], uo[k]);
+    }
     if (!mid) {
       double a = d.acc[si], dr0[3];
       for (int q = 0; q < 3; ++q) dr0[q] = d.dr0[q * ns + si];

@@ -10,8 +10,9 @@
 // both over in-group lane shuffles.  No LDS and no barriers: the
 // wave runs its filaments start to finish without waiting on anything but
 // its own loads.  The algebra uses the reduced forms of the fast path
-// (mobility f/gperp + (1/gpar-1/gperp) t(t.f), bending (I - b b^T) a); the
-// FP64 reference-order kernel remains the exact path.
+// (mobility f/gperp + (1/gpar-1/gperp) t(t.f), bending (I - b b^T) a) and
+// reciprocals by rcp_fast; the FP64 reference-order kernel remains the
+// exact path.
 #include <hip/hip_runtime.h>
 
 #include "sc_internal.h"
@@ -67,8 +68,8 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
       o[q] = d_next(rc[q]) - rc[q];
       l += o[q] * o[q];
     }
-    l = sqrt(l);
-    for (int q = 0; q < 3; ++q) ub[q] = o[q] / l;
+    const double il = rcp_fast(sqrt(l));
+    for (int q = 0; q < 3; ++q) ub[q] = o[q] * il;
     for (int q = 0; q < 3; ++q) {
       const double pv = d_prev(ub[q]);
       u[q] = i < m ? ub[q] : (i == m ? pv : (q == 0 ? 1.0 : 0.0));
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
     }
   }
   // ---- ApplyInteractionForces (src/filament.cpp:992-1048) ---------------
-  const double linv = 1.0 / bl;
+  const double linv = rcp_fast(bl);
   double sf[3], pt[3];
   cross3(u, Tb, pt);
   for (int q = 0; q < 3; ++q) {
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
     for (int q = 0; q < 3; ++q) t[q] = u[q];
   } else {
     for (int q = 0; q < 3; ++q) t[q] = um[q] + u[q];
-    const double inv = 1.0 / sqrt(dot(t, t));
+    const double inv = rcp_fast(sqrt(dot(t, t)));
     for (int q = 0; q < 3; ++q) t[q] *= inv;
   }
   // ---- random forces (midstep): construct and project ------------------
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
     if (i + 3 >= n) db3 = 1.0;  // det_b[n] = 1
     double ke = 0.0;
     if (i < n - 2) {
-      const double ginv = cs * det_t * db3 * (1.0 / det_g);
+      const double ginv = cs * det_t * db3 * rcp_fast(det_g);
       ke = (kappa + bl * ginv) * (linv * linv);
     }
     const double kem = d_prev(ke), kemm = d_prev(kem);
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
   }
 
   // ---- tensions (src/filament.cpp:778-838) -------------------------------
-  const double igpar = 1.0 / gpar, igperp = 1.0 / gperp, dg = igpar - igperp;
+  const double igpar = rcp_fast(gpar), igperp = rcp_fast(gperp), dg = igpar - igperp;
   double T;
   {
     double tn[3], fn[3], m1[3], m2[3], fd[3];
@@ -244,11 +245,11 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
     double rd[3];
     for (int q = 0; q < 3; ++q) rd[q] = rnn[q] - rn[q];
     const double mm = sqrt(dot(rd, rd));
-    const double inv = 1.0 / mm;
+    const double inv = rcp_fast(mm);
     for (int q = 0; q < 3; ++q) uo[q] = rd[q] * inv;
     if (i < m) {
       const double diff = bl - mm;
-      if ((diff < 0 ? -diff : diff) / bl > 1e-3) bad = 1;
+      if ((diff < 0 ? -diff : diff) * linv > 1e-3) bad = 1;
     }
   }
   const unsigned long long bal = __ballot(bad);

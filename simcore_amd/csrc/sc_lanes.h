@@ -18,6 +18,8 @@
 
 #include <cstdint>
 
+#include "sc_math.h"
+
 namespace sc {
 namespace lanes {
 
@@ -189,7 +191,7 @@ __device__ __forceinline__ void mob(double igperp, double dg, const double *t,
 
 // Parallel cyclic reduction of rows i < m (row i in lane gl = i): lower a
 // (x[i-1]), diagonal b, upper c (x[i+1]), right-hand side d.  Rows >= m are
-// inert.  Returns x[i].
+// inert.  Returns x[i].  Reciprocals by rcp_fast (mixed-precision path).
 template <int G>
 __device__ __forceinline__ double pcr(double a, double b, double c, double d, int m, int lane,
                                       int gl) {
@@ -201,7 +203,7 @@ __device__ __forceinline__ double pcr(double a, double b, double c, double d, in
   }
 #pragma unroll
   for (int s = 1; s < G; s <<= 1) {
-    const double rb = 1.0 / b;
+    const double rb = rcp_fast(b);
     const double am = g_from<G>(a, lane, gl, -s, 0.0), cm = g_from<G>(c, lane, gl, -s, 0.0);
     const double dm = g_from<G>(d, lane, gl, -s, 0.0), rm = g_from<G>(rb, lane, gl, -s, 0.0);
     const double ap = g_from<G>(a, lane, gl, s, 0.0), cp = g_from<G>(c, lane, gl, s, 0.0);
@@ -216,7 +218,7 @@ __device__ __forceinline__ double pcr(double a, double b, double c, double d, in
     a = na;
     c = nc;
   }
-  return d / b;
+  return d * rcp_fast(b);
 }
 
 // inclusive prefix (dir = -1: over lower lanes, newest on the left) or

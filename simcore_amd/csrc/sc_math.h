@@ -7,7 +7,23 @@
 
 #include <cstdint>
 
+#include "../../include/simcore_amd.h"
+
 namespace sc {
+
+// 1/x for the mixed-precision path: the v_rcp_f64 estimate refined by two
+// Newton steps, e = 1 - x r, r += r e (error squared per step), within about
+// an ulp of the correctly rounded reciprocal.  The IEEE division sequence
+// adds operand scaling and a final rounding fix-up, about twice the
+// instructions; x here is a length, pivot or determinant of order one (0
+// gives a NaN instead of an infinity, in lanes whose values are unused).
+__device__ __forceinline__ double rcp_fast(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
 
 // NINT (include/simcore/macros.hpp:6): round half away from zero to int.
 __device__ __forceinline__ int nint(double x) {
@@ -299,10 +315,16 @@ __device__ __forceinline__ void bond_reinit(const DevT &d, const GeoT &g,
     l += o[k] * o[k];
   }
   l = sqrt(l);
+  // mixed precision: u = o (1/l), the arithmetic of the register integrators,
+  // which recompute u from the site positions
+  const double il = prec == SC_PREC_MIXED ? rcp_fast(l) : 0.0;
   double pos[3];
   for (int k = 0; k < 3; ++k) {
     pos[k] = r1[k] + 0.5 * o[k];
-    o[k] /= l;
+    if (prec == SC_PREC_MIXED)
+      o[k] *= il;
+    else
+      o[k] /= l;
   }
   double sp[3];
   for (int k = 0; k < 3; ++k) {
@@ -314,7 +336,27 @@ __device__ __forceinline__ void bond_reinit(const DevT &d, const GeoT &g,
   double4 *rec = reinterpret_cast<double4 *>(d.br + 8 * b);
   rec[0] = make_double4(sp[0], sp[1], sp[2], o[0]);
   rec[1] = make_double4(o[1], o[2], l, 0.0);
-  (void)prec;
+}
+
+// The mixed-precision bond record of bond_reinit from a bond vector o =
+// r2 - r1, its length l = sqrt(o.o) and unit vector u = o (1/l) that the
+// caller already holds (the same bits bond_reinit would compute).
+template <class DevT, class GeoT>
+__device__ __forceinline__ void bond_store(const DevT &d, const GeoT &g, int64_t b,
+                                           const double *r1, const double *o, double l,
+                                           const double *u) {
+  double pos[3];
+  for (int k = 0; k < 3; ++k) pos[k] = r1[k] + 0.5 * o[k];
+  double sp[3];
+  for (int k = 0; k < 3; ++k) {
+    double si = 0.0;
+    for (int j = 0; j < 3; ++j) si += g.hinv[3 * k + j] * pos[j];
+    si -= (double)nint(si);
+    sp[k] = si;
+  }
+  double4 *rec = reinterpret_cast<double4 *>(d.br + 8 * b);
+  rec[0] = make_double4(sp[0], sp[1], sp[2], u[0]);
+  rec[1] = make_double4(u[1], u[2], l, 0.0);
 }
 
 }  // namespace sc
