@@ -8,8 +8,8 @@
 //
 // One step, in stream order (all guarded by the device control block,
 // which holds the decision for the step about to run):
-//   [rebuild: keys, scan, scatter, run sort, count, scan, cap, emit, scan,
-//    incidence fill, incidence sort, zero dr]
+//   [rebuild: keys, scan, scatter, count (+ run order, degrees), scan,
+//    cap, emit, scan, incidence fill, incidence sort, zero dr]
 //   -> pair+scatter -> integrate (one launch per length bin)
 //   -> commit (and decide the next step)
 #include <hip/hip_runtime.h>
@@ -371,6 +371,55 @@ __device__ __forceinline__ void load_shell(const Dev &d, const Geo &g, int32_t k
   }
 }
 
+// q-th cell of the mirrored half shell: the 13 cells whose half shell holds
+// (x, y, z), i.e. the offsets of shell_key negated (wrapped).  With at
+// least 3 cells per side the 13 + 13 + 1 cells are distinct.
+__device__ __forceinline__ int32_t back_shell_key(int n, int x, int y, int z, int q) {
+  int dx, dy, dz;
+  if (q == 0) {
+    dx = -1;
+    dy = 0;
+    dz = 0;
+  } else if (q < 4) {
+    dx = 2 - q;
+    dy = -1;
+    dz = 0;
+  } else {
+    dx = 1 - (q - 4) % 3;
+    dy = 1 - (q - 4) / 3;
+    dz = -1;
+  }
+  int xp = x + dx, yp = y + dy, zp = z + dz;
+  xp = xp < 0 ? n - 1 : (xp == n ? 0 : xp);
+  yp = yp < 0 ? n - 1 : (yp == n ? 0 : yp);
+  zp = zp < 0 ? n - 1 : zp;
+  return ((int32_t)xp * n + yp) * n + zp;
+}
+
+// The runs of the 13 mirrored half-shell cells of k (load_shell's
+// counterpart).
+__device__ __forceinline__ void load_back_shell(const Dev &d, const Geo &g, int32_t k,
+                                                Shell *sh) {
+  const int n = g.n1d;
+  const int x = k / (n * n), y = (k / n) % n, z = k % n;
+  int32_t nbk[13];
+  unsigned occ = 0;
+#pragma unroll
+  for (int q = 0; q < 13; ++q) {
+    nbk[q] = back_shell_key(n, x, y, z, q);
+    occ |= ((d.cell_occ[nbk[q] >> 3] >> (nbk[q] & 7)) & 1u) << q;
+  }
+#pragma unroll
+  for (int q = 0; q < 13; ++q) {
+    sh->o0[q] = 0;
+    sh->o1[q] = 0;
+    if ((occ >> q) & 1u) {
+      sh->o0[q] = d.cell_start[nbk[q]];
+      sh->o1[q] = d.cell_start[nbk[q] + 1];
+    }
+  }
+}
+
 // End of the run of cell k that starts at sorted position p, from the
 // sorted keys (sequential reads; runs are short).
 __device__ __forceinline__ int32_t run_end(const Dev &d, const Geo &g, int64_t p, int32_t k) {
@@ -379,15 +428,19 @@ __device__ __forceinline__ int32_t run_end(const Dev &d, const Geo &g, int64_t p
   return (int32_t)e;
 }
 
-// Candidate count of each occupied home cell (Cell::MakePairs,
-// src/cell.cpp:14-70), excluded pairs already removed (the reference culls
-// them on the rebuild step, src/interaction_manager.cpp:485-497).  One
-// thread per sorted position, one block per tile of kScanBlock positions;
-// the first position of a cell's run does the work:
-//  - writes its run ordered by bond index to sorted_o (Cell::AddObj appends
-//    in interactor order, src/cell.cpp:5-8);
-//  - counts the run's candidates from the unordered runs (a count needs no
-//    order);
+// Candidate counts (Cell::MakePairs, src/cell.cpp:14-70), excluded pairs
+// already removed (the reference culls them on the rebuild step,
+// src/interaction_manager.cpp:485-497).  One thread per sorted position,
+// i.e. per bond b, with no atomics:
+//  - places b in its cell's run of sorted_o by its rank in bond order
+//    (Cell::AddObj appends in interactor order, src/cell.cpp:5-8);
+//  - counts the candidates that list b first: run members of higher bond
+//    index and the bonds of the 13 half-shell cells; a run's total, the
+//    sum over its positions, is what its first position emits, so the scan
+//    of these counts gives each run's offset into the candidate list;
+//  - counts the candidates that list b second (run members of lower index,
+//    the mirrored half shell) and writes b's degree, the length of its
+//    incidence list;
 // and the tile's sum goes to tsum for the candidate-offset scan.
 __global__ __launch_bounds__(kScanBlock) void k_count(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
@@ -397,31 +450,34 @@ __global__ __launch_bounds__(kScanBlock) void k_count(Dev d, Geo g) {
   int32_t cnt = 0;
   if (p < g.nb) {
     const int32_t k = d.skey[p];
-    if (p == 0 || d.skey[p - 1] != k) {  // first position of its cell's run
-      const int32_t e = run_end(d, g, p, k);
-      Shell sh;
-      load_shell(d, g, k, &sh);
-      for (int32_t i = (int32_t)p; i < e; ++i) {  // insertion into sorted_o
-        const int32_t v = d.sorted[i];
-        int32_t j = i - 1;
-        while (j >= p && d.sorted_o[j] > v) {
-          d.sorted_o[j + 1] = d.sorted_o[j];
-          --j;
-        }
-        d.sorted_o[j + 1] = v;
-      }
-      for (int32_t i = (int32_t)p; i < e; ++i)
-        for (int32_t j = i + 1; j < e; ++j)
-          cnt += !excluded(d.bond_fil, d.sorted[i], d.sorted[j]);
-      for (int32_t i = (int32_t)p; i < e; ++i) {
-        const int32_t bi = d.sorted[i];
-#pragma unroll
-        for (int q = 0; q < 13; ++q)
-          for (int32_t j = sh.o0[q]; j < sh.o1[q]; ++j)
-            cnt += !excluded(d.bond_fil, bi, d.sorted[j]);
+    const int32_t b = d.sorted[p];
+    int64_t s0 = p;
+    while (s0 > 0 && d.skey[s0 - 1] == k) --s0;
+    const int32_t e = run_end(d, g, p, k);
+    Shell fw, bw;
+    load_shell(d, g, k, &fw);
+    load_back_shell(d, g, k, &bw);
+    int32_t rank = 0, first = 0, second = 0;
+    for (int32_t i = (int32_t)s0; i < e; ++i) {
+      const int32_t o = d.sorted[i];
+      const int ok = o != b && !excluded(d.bond_fil, b, o);
+      if (o < b) {
+        ++rank;
+        second += ok;
+      } else {
+        first += ok;
       }
     }
-    d.ccount[p] = cnt;
+    d.sorted_o[s0 + rank] = b;
+#pragma unroll
+    for (int q = 0; q < 13; ++q) {
+      for (int32_t j = fw.o0[q]; j < fw.o1[q]; ++j) first += !excluded(d.bond_fil, b, d.sorted[j]);
+      for (int32_t j = bw.o0[q]; j < bw.o1[q]; ++j)
+        second += !excluded(d.bond_fil, b, d.sorted[j]);
+    }
+    cnt = first;
+    d.ccount[p] = first;
+    d.deg[b] = first + second;
   }
   int tot;
   block_excl_scan256(cnt, &tot, wsum);
@@ -464,8 +520,6 @@ __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
     } else {
       d.slot32[w] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    atomicAdd(&d.deg[b1], 1);
-    atomicAdd(&d.deg[b2], 1);
     ++w;
   };
   for (int32_t i = (int32_t)p; i < e; ++i)
@@ -532,8 +586,8 @@ static inline unsigned sgrid(int64_t n) {
   return b < (unsigned)kStreamBlocks ? b : (unsigned)kStreamBlocks;
 }
 
-// keys -> cell scan (2) -> scatter -> count (+ run order, tile sums) ->
-// candidate offsets (1) -> emit (+ degrees) -> degree scan (2) ->
+// keys -> cell scan (2) -> scatter -> count (+ run order, degrees, tile
+// sums) -> candidate offsets (1) -> emit -> degree scan (2) ->
 // incidence fill -> incidence order (+ displacement restart): 11 launches.
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
   const int bs = 256;
