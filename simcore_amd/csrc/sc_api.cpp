@@ -483,7 +483,7 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   b += 4 * R(24 * ns) + 2 * R(8 * ns);                    // r, rp, fr, dr0, acc, acc_prev
   b += R(64 * nb) + 2 * R(24 * nb) + R(8 * nb);           // record, F, T (at most), tension
   b += R(4 * nb) + R(4 * n_cells) + R(4 * (n_cells + 1));  // keys, cell count/start
-  b += 3 * R(4 * nb) + R(4 * (nb + 1)) + R(4 * nb) + R(4 * (nb + 1));
+  b += 3 * R(4 * nb) + R(4 * nb) + R(4 * (nb + 1));  // sorted, skey, ccount, deg, inc_start
   b += R(4 * nb) + R(4 * ((nb + sc::kScanBlock - 1) / sc::kScanBlock + 1));  // sorted_o, tsum
   b += R((n_cells + 7) / 8 + 8);                                               // occupancy
   const int64_t tmp = std::max(scan_tmp_elems(n_cells), scan_tmp_elems(nb));
@@ -586,7 +586,6 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = dalloc(h, &d.sorted_o, nb, false))) return rc;
   if ((rc = dalloc(h, &d.cell_occ, (size_t)(g.n_cells + 7) / 8 + 8, false))) return rc;
   if ((rc = dalloc(h, &d.ccount, nb, false))) return rc;
-  if ((rc = dalloc(h, &d.coff, nb + 1, false))) return rc;
   if ((rc = dalloc(h, &d.tsum, (nb + sc::kScanBlock - 1) / sc::kScanBlock + 1, false)))
     return rc;
   if ((rc = dalloc(h, &d.deg, nb, false))) return rc;

@@ -87,7 +87,7 @@ struct Dev {
   int32_t *keys, *cell_count, *cell_start, *sorted, *skey;
   int32_t *sorted_o;       // cell runs in bond order
   uint8_t *cell_occ;       // n_cells bits: cell occupied in the last build
-  int32_t *ccount, *coff, *tsum;
+  int32_t *ccount, *tsum;
   // candidates
   int2 *pairs;
   double4 *slot64;         // {Fx,Fy,Fz,-}

@@ -8,8 +8,8 @@
 //
 // One step, in stream order (all guarded by the device control block,
 // which holds the decision for the step about to run):
-//   [rebuild: keys, scan, scatter, count (+ run order, degrees), scan,
-//    cap, emit, scan, incidence fill, incidence sort, zero dr]
+//   [rebuild: keys, scan, scatter, count (+ run order, degrees), emit
+//    (+ offsets, cap), scan, incidence fill, incidence sort, zero dr]
 //   -> pair+scatter -> integrate (one launch per length bin)
 //   -> commit (and decide the next step)
 #include <hip/hip_runtime.h>
@@ -232,29 +232,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
       if (base + j < n) out[base + j] = tile[j];
     if (t == ntiles - 1 && threadIdx.x == 0) out[n] = carry + tot;
     __syncthreads();
-  }
-}
-
-// Down pass of a scan whose tile sums (tiles of kScanBlock entries) the
-// producing kernel already wrote (k_count): one launch instead of two.
-__global__ __launch_bounds__(kScanBlock) void k_scan_small_tiles(const int32_t *in, int64_t n,
-                                                                 const int32_t *tsum,
-                                                                 int32_t *out, const Ctl *ctl,
-                                                                 int guard) {
-  if (guard_off(ctl, guard)) return;
-  __shared__ int wsum[kScanBlock / 64];
-  const int64_t ntiles = (n + kScanBlock - 1) / kScanBlock;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    int part = 0;
-    for (int64_t i = threadIdx.x; i < t; i += kScanBlock) part += tsum[i];
-    int carry;
-    block_excl_scan256(part, &carry, wsum);
-    const int64_t i = t * kScanBlock + threadIdx.x;
-    const int v = i < n ? in[i] : 0;
-    int tot;
-    const int ex = block_excl_scan256(v, &tot, wsum) + carry;
-    if (i < n) out[i] = ex;
-    if (t == ntiles - 1 && threadIdx.x == 0) out[n] = carry + tot;
   }
 }
 
@@ -488,13 +465,33 @@ __global__ __launch_bounds__(kScanBlock) void k_count(Dev d, Geo g) {
 // each, the self pairs (i<j) then, for each of the 13 half-shell
 // neighbours in AssignCellNeighbors order, all home x neighbour pairs; runs
 // read in bond order from sorted_o.
-__global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
+//
+// One block per tile of kScanBlock sorted positions, the tiles of k_count,
+// so the candidate offsets need no separate scan: the tile sums before the
+// block's tile plus the exclusive scan of its own counts (the sums, a few
+// thousand entries, are read from L2 by every block; their total is the
+// list length).
+__global__ __launch_bounds__(kScanBlock) void k_emit(Dev d, Geo g, int prec) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  // capacity check (every thread reads the total; one records it): an
-  // overflowing list aborts the step, the host grows the buffers and
-  // re-runs it
-  const int64_t need = d.coff[g.nb];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  __shared__ int wsum[kScanBlock / 64];
+  const int64_t t = blockIdx.x;
+  const int64_t ntiles = (g.nb + kScanBlock - 1) / kScanBlock;
+  int before = 0, all = 0;
+  for (int64_t i = threadIdx.x; i < ntiles; i += kScanBlock) {
+    const int v = d.tsum[i];
+    all += v;
+    before += i < t ? v : 0;
+  }
+  int carry, total;
+  block_excl_scan256(before, &carry, wsum);
+  block_excl_scan256(all, &total, wsum);
+  const int64_t p = t * kScanBlock + threadIdx.x;
+  int tile_tot;
+  const int32_t off = carry + block_excl_scan256(p < g.nb ? d.ccount[p] : 0, &tile_tot, wsum);
+  // capacity check (one thread records it): an overflowing list aborts the
+  // step, the host grows the buffers and re-runs it
+  const int64_t need = total;
+  if (t == 0 && threadIdx.x == 0) {
     d.ctl->nc_need = need;
     if (need > g.cap) {
       d.ctl->abort = 1;
@@ -504,14 +501,13 @@ __global__ __launch_bounds__(256) void k_emit(Dev d, Geo g, int prec) {
     }
   }
   if (need > g.cap) return;
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.nb) return;
   const int32_t k = d.skey[p];
   if (p > 0 && d.skey[p - 1] == k) return;  // not the first of its run
   const int32_t e = run_end(d, g, p, k);
   Shell sh;
   load_shell(d, g, k, &sh);
-  int32_t w = d.coff[p];
+  int32_t w = off;
   auto put = [&](int32_t b1, int32_t b2) {
     if (excluded(d.bond_fil, b1, b2)) return;
     d.pairs[w] = make_int2(b1, b2);
@@ -587,24 +583,23 @@ static inline unsigned sgrid(int64_t n) {
 }
 
 // keys -> cell scan (2) -> scatter -> count (+ run order, degrees, tile
-// sums) -> candidate offsets (1) -> emit -> degree scan (2) ->
-// incidence fill -> incidence order (+ displacement restart): 11 launches.
+// sums) -> emit (+ candidate offsets) -> degree scan (2) -> incidence fill
+// -> incidence order (+ displacement restart): 10 launches.  (Running the
+// degree scan on a forked side stream beside the emit measured slower in
+// graph replay than the launches in a row.)
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
-  const int bs = 256;
-  const unsigned gb = nblocks(g.nb, bs);
-  hipLaunchKernelGGL(k_keys, dim3(gb), dim3(bs), 0, st, d, g);
+  const unsigned gb = nblocks(g.nb, kScanBlock);
+  hipLaunchKernelGGL(k_keys, dim3(gb), dim3(kScanBlock), 0, st, d, g);
   launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, d.ctl,
               kGuardRebuild, st, d.cell_occ);
-  hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(bs), 0, st, d, g, (int)kGuardRebuild);
-  hipLaunchKernelGGL(k_count, dim3(nblocks(g.nb, kScanBlock)), dim3(kScanBlock), 0, st, d, g);
-  hipLaunchKernelGGL(k_scan_small_tiles, dim3(sgrid(g.nb)), dim3(kScanBlock), 0, st,
-                     d.ccount, g.nb, d.tsum, d.coff, d.ctl, kGuardRebuild);
-  hipLaunchKernelGGL(k_emit, dim3(gb), dim3(bs), 0, st, d, g,
+  hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kScanBlock), 0, st, d, g, (int)kGuardRebuild);
+  hipLaunchKernelGGL(k_count, dim3(gb), dim3(kScanBlock), 0, st, d, g);
+  hipLaunchKernelGGL(k_emit, dim3(gb), dim3(kScanBlock), 0, st, d, g,
                      d.slot64 ? (int)SC_PREC_FP64 : (int)SC_PREC_MIXED);
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
-  hipLaunchKernelGGL(k_inc_fill, dim3(2048), dim3(bs), 0, st, d, g);
-  hipLaunchKernelGGL(k_inc_sort, dim3(nblocks(g.ns > g.nb ? g.ns : g.nb, bs)), dim3(bs), 0, st,
-                     d, g);
+  hipLaunchKernelGGL(k_inc_fill, dim3(2048), dim3(kScanBlock), 0, st, d, g);
+  hipLaunchKernelGGL(k_inc_sort, dim3(nblocks(g.ns > g.nb ? g.ns : g.nb, kScanBlock)),
+                     dim3(kScanBlock), 0, st, d, g);
 }
 
 // Cell membership (cell_start, sorted) of the last rebuild from the stored
