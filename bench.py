@@ -251,6 +251,15 @@ def pair_bytes(precision, nb, nc):
     return nb * (64 + sums + 4) + nc * (slot + 2 * 8)
 
 
+def pair_bytes_fp32_state(nb, nc):
+    """The same launch priced for a 28-byte FP32 bond state (s, u, l/2) and
+    12-byte FP32 slots: per bond the state read and the FP32 sums written,
+    per candidate its index pair read and its slot read and written.  This
+    build keeps FP64 records (the mixed path's geometry is FP64), so the
+    figure is reported beside the roofline, not as it."""
+    return nb * (28 + 24) + nc * (8 + 12 + 12)
+
+
 def timed_steps(sysobj, steps):
     t0 = time.perf_counter()
     sysobj.step(steps)  # returns after the stream has drained
@@ -364,6 +373,9 @@ def main():
                 "kernel": kname,
                 "kernel_avg_ms": pair_ms, "kernel_avg_ms_back_to_back": pair_warm_ms,
                 "algorithmic_bytes": bytes_pair,
+                "algorithmic_bytes_fp32_state": pair_bytes_fp32_state(fil.nb, nc),
+                "frac_fp32_state": round(pair_bytes_fp32_state(fil.nb, nc) / (pair_ms * 1e-3)
+                                         / 1e9 / HBM_PEAK_GBS, 4) if pair_ms > 0 else 0.0,
                 "candidates": nc, "per_step_ms": {k: v[0] for k, v in prof.items()}}
     dev.close()
 
