@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sc_internal.h"
+#include "sc_math.h"
 
 namespace sc {
 
@@ -40,8 +41,9 @@ __global__ __launch_bounds__(256) void k_theta_hist(Dev d, Geo g, int n_bins, in
   if (j + 1 >= b1) return;
   const int row = (int)(j - b0);
   if (row >= n_rows) return;
-  const double *a = d.br + 8 * j + 3;
-  const double *b = d.br + 8 * (j + 1) + 3;
+  double s[3], a[3], b[3], l;
+  rec_read(d.br, g, j, s, a, &l);
+  rec_read(d.br, g, j + 1, s, b, &l);
   const double c = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
   int bin = (int)floor((1.0 + c) * (n_bins / 2));
   if (bin >= n_bins) bin = n_bins - 1;

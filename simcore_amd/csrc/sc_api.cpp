@@ -445,6 +445,14 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
   }
   g.stoch = p.stoch_flag ? 1 : 0;
   g.seed = p.seed;
+  // mixed precision keeps 32-byte bond records whose cell coordinates are
+  // packed in 10 bits each
+  g.compact = p.precision == SC_PREC_MIXED ? 1 : 0;
+  g.inv_n1d = 1.0 / n1d;
+  if (g.compact && n1d > 1024) {
+    delete h;
+    return SC_ERR_ARG;
+  }
   h->arena = arena;
   if (stream) {
     h->stream = stream;
@@ -481,7 +489,8 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
              R(64 * (int64_t)nf) +
              R(8 * (int64_t)nf) + R(4 * nb);
   b += 4 * R(24 * ns) + 2 * R(8 * ns);                    // r, rp, fr, dr0, acc, acc_prev
-  b += R(64 * nb) + 2 * R(24 * nb) + R(8 * nb);           // record, F, T (at most), tension
+  const int64_t rec_bytes = params->precision == SC_PREC_MIXED ? 32 : 64;
+  b += R(rec_bytes * nb) + 2 * R(24 * nb) + R(8 * nb);    // record, F, T (at most), tension
   b += R(4 * nb) + R(4 * n_cells) + R(4 * (n_cells + 1));  // keys, cell count/start
   b += 3 * R(4 * nb) + R(4 * nb) + R(4 * (nb + 1));  // sorted, skey, ccount, deg, inc_start
   b += R(4 * nb) + R(4 * ((nb + sc::kScanBlock - 1) / sc::kScanBlock + 1));  // sorted_o, tsum
@@ -569,7 +578,7 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     if ((rc = dalloc(h, a, 3 * (size_t)ns, false))) return rc;
   if ((rc = dalloc(h, &d.acc, ns, false))) return rc;
   if ((rc = dalloc(h, &d.acc_prev, ns, false))) return rc;
-  if ((rc = dalloc(h, &d.br, 8 * (size_t)nb, false))) return rc;
+  if ((rc = dalloc(h, &d.br, (g.compact ? 4 : 8) * (size_t)nb, false))) return rc;
   if (h->p.precision == SC_PREC_FP64) {
     if ((rc = dalloc(h, &d.bf, 3 * (size_t)nb, false))) return rc;
     if ((rc = dalloc(h, &d.bt, 3 * (size_t)nb, false))) return rc;
@@ -854,14 +863,33 @@ int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u, double *len)
   if (rc) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const int64_t nb = h->g.nb, ns = h->g.ns;
-  std::vector<double> rec(8 * (size_t)nb), sp(3 * (size_t)nb), uu(3 * (size_t)nb), ln(nb);
-  HIPCHK(h, hipMemcpy(rec.data(), h->d.br, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
-  for (int64_t b = 0; b < nb; ++b) {
-    for (int k = 0; k < 3; ++k) {
-      sp[k * nb + b] = rec[8 * b + k];
-      uu[k * nb + b] = rec[8 * b + 3 + k];
+  std::vector<double> sp(3 * (size_t)nb), uu(3 * (size_t)nb), ln(nb);
+  if (h->g.compact) {
+    // mixed precision: the 32-byte record {cell offset x, y, z, length},
+    // {unit vector, packed cell coordinates}; scaled position rebuilt as the
+    // kernels rebuild it
+    std::vector<float> rec(8 * (size_t)nb);
+    HIPCHK(h, hipMemcpy(rec.data(), h->d.br, sizeof(float) * rec.size(), hipMemcpyDeviceToHost));
+    for (int64_t b = 0; b < nb; ++b) {
+      int32_t packed;
+      std::memcpy(&packed, &rec[8 * b + 7], sizeof(packed));
+      const int c[3] = {packed & 1023, (packed >> 10) & 1023, (packed >> 20) & 1023};
+      for (int k = 0; k < 3; ++k) {
+        sp[k * nb + b] = ((double)c[k] + (double)rec[8 * b + k]) * h->g.inv_n1d - 0.5;
+        uu[k * nb + b] = rec[8 * b + 4 + k];
+      }
+      ln[b] = rec[8 * b + 3];
     }
-    ln[b] = rec[8 * b + 6];
+  } else {
+    std::vector<double> rec(8 * (size_t)nb);
+    HIPCHK(h, hipMemcpy(rec.data(), h->d.br, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+    for (int64_t b = 0; b < nb; ++b) {
+      for (int k = 0; k < 3; ++k) {
+        sp[k * nb + b] = rec[8 * b + k];
+        uu[k * nb + b] = rec[8 * b + 3 + k];
+      }
+      ln[b] = rec[8 * b + 6];
+    }
   }
   if (spos) to_caller(h, h->bond_map, sp.data(), spos, nb, 3);
   if (u) to_caller(h, h->bond_map, uu.data(), u, nb, 3);
@@ -1100,7 +1128,7 @@ std::vector<CkField> ck_fields(sc_system *h, int64_t nc) {
   return {{d.r, 24 * (size_t)ns},        {d.rp, 24 * (size_t)ns},
           {d.fr, 24 * (size_t)ns},       {d.dr0, 24 * (size_t)ns},
           {d.acc, 8 * (size_t)ns},       {d.acc_prev, 8 * (size_t)ns},
-          {d.br, 64 * (size_t)nb},
+          {d.br, (h->g.compact ? 32 : 64) * (size_t)nb},
           {d.bf ? (void *)d.bf : (void *)d.bf32, (d.bf ? 24 : 12) * (size_t)nb},
           {d.bt ? (void *)d.bt : (void *)d.bt32, (d.bt ? 24 : 12) * (size_t)nb},
           {d.tension, 8 * (size_t)nb},

@@ -53,6 +53,28 @@ __device__ __forceinline__ void scaled(const Geo &g, const double *P, double *sp
   }
 }
 
+// scaled centre, orientation and length of bond b from its sites with the
+// arithmetic of the FP64 bond record (Bond::ReInit + Object::UpdatePeriodic,
+// src/bond.cpp:22-37, src/object.cpp:202-210): the partner search reads
+// these rather than the record, which on the mixed path is compact (FP32)
+__device__ __forceinline__ void bond_state(const Dev &d, const Geo &g, int32_t b, double *sp,
+                                           double *u, double *len) {
+  const int64_t site = (int64_t)b + d.bond_fil[b];
+  const int64_t ns = g.ns;
+  double o[3], l = 0.0, pos[3];
+  for (int k = 0; k < 3; ++k) {
+    o[k] = d.r[k * ns + site + 1] - d.r[k * ns + site];
+    l += o[k] * o[k];
+  }
+  l = sqrt(l);
+  for (int k = 0; k < 3; ++k) {
+    pos[k] = d.r[k * ns + site] + 0.5 * o[k];
+    u[k] = o[k] / l;
+  }
+  *len = l;
+  scaled(g, pos, sp);
+}
+
 __device__ __forceinline__ double unit_draw(uint32_t v) {
   return ((double)v + 0.5) * 2.3283064365386962890625e-10;
 }
@@ -130,7 +152,8 @@ __global__ __launch_bounds__(256) void k_xl_events(Dev d, Geo g) {
         for (int32_t j = d.cell_start[c]; j < d.cell_start[c + 1]; ++j) {
           const int32_t b = d.sorted[j];
           if (b == bs) continue;
-          const double *rec = d.br + 8 * (int64_t)b;
+          double rec[7];
+          bond_state(d, g, b, rec, rec + 3, rec + 6);
           double ds[3], dr[3], t = 0.0, d2 = 0.0;
           for (int i = 0; i < 3; ++i) {
             ds[i] = rec[i] - sP[i];

@@ -119,15 +119,19 @@ __global__ __launch_bounds__(64) void k_integrate(Dev d, Geo g,
     }
     if (i < n - 1) {
       const int64_t j = b0 + i;
+      double rs[3], ru[3], rl;
+      rec_read(d.br, g, j, rs, ru, &rl);
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         Fb[k][q] = bond_force(d, nb, q, j);
         Tb[k][q] = bond_torque(d, nb, q, j);
-        U[q][i] = d.br[8 * j + 3 + q];
+        U[q][i] = ru[q];
       }
     } else if (i == n - 1) {
+      double rs[3], ru[3], rl;
+      rec_read(d.br, g, b0 + n - 2, rs, ru, &rl);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) U[q][i] = d.br[8 * (b0 + n - 2) + 3 + q];
+      for (int q = 0; q < 3; ++q) U[q][i] = ru[q];
     }
     if (i < n && !mid && g.stoch) {
 #pragma unroll

@@ -219,9 +219,9 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   // register pressure, hence occupancy, in check) --------------------------
   double u[S][3], Fb[S][3], Tb[S][3], fr[S][3];
   {
-    // bond orientations from the site positions with the arithmetic of
-    // bond_reinit, so the bits equal those of the stored bond record (which
-    // is then not read here); the last site takes bond m-1's
+    // bond orientations from the site positions in FP64 with the arithmetic
+    // of bond_reinit (the record, compact on this path, keeps them rounded
+    // to FP32 and is not read here); the last site takes bond m-1's
     double rc[S][3], rnx[S][3], ub[S][3], ubm[S][3];
     // an opaque copy of the pointer: the midstep's second load of r (at the
     // end) must not be merged with this one, which would keep the positions

@@ -59,9 +59,9 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
   double rp[3] = {0, 0, 0}, u[3] = {1.0, 0, 0}, Fb[3] = {0, 0, 0}, Tb[3] = {0, 0, 0};
   double fr[3] = {0, 0, 0}, acc = 0.0, dr0[3] = {0, 0, 0};
   {
-    // bond orientation from the site positions with the arithmetic of
-    // bond_reinit, so the bits equal those of the stored bond record (which
-    // is then not read here); the last site takes bond m-1's
+    // bond orientation from the site positions in FP64 with the arithmetic
+    // of bond_reinit (the record, compact on this path, keeps it rounded to
+    // FP32 and is not read here); the last site takes bond m-1's
     double rc[3], o[3], ub[3], l = 0.0;
     for (int q = 0; q < 3; ++q) rc[q] = act ? d.r[q * ns + si] : 0.0;
     for (int q = 0; q < 3; ++q) {

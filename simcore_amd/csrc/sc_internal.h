@@ -50,6 +50,8 @@ struct Geo {
   double thr;              // 0.25 * cell_len^2
   int n1d;
   int stoch;
+  int compact;             // mixed precision: 32-byte bond records (sc_math.h)
+  double inv_n1d;
   uint64_t seed;
   int32_t nf;
   int64_t ns, nb, n_cells, cap;

@@ -258,8 +258,13 @@ __global__ void k_keys(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
-    const double *rec = d.br + 8 * b;
-    const int32_t k = cell_key(g.n1d, rec[0], rec[1], rec[2]);
+    int32_t k;
+    if (g.compact) {
+      k = rec_key_compact(d.br, g, b);
+    } else {
+      const double *rec = d.br + 8 * b;
+      k = cell_key(g.n1d, rec[0], rec[1], rec[2]);
+    }
     d.keys[b] = k;
     atomicAdd(&d.cell_count[k], 1);
   }

@@ -34,19 +34,92 @@ __device__ __forceinline__ int nint(double x) {
 // (x*n + y)*n + z, the order of MakePairs' cell_[x][y][z] walk.  Scaled
 // coordinates lie in [-0.5, 0.5]; the clamp to [0, n-1] only guards the
 // half-ulp case s = -0.5 - 2^-54, where the reference would index cell -1.
+// cell coordinate of one scaled coordinate s, and n1d (s + 1/2) itself
+__device__ __forceinline__ int cell_coord(int n1d, double s, double *t) {
+  const double x = s + 0.5;
+  *t = n1d * x;
+  int xc = (int)floor(*t);
+  if (xc == n1d) xc -= 1;
+  return xc < 0 ? 0 : (xc >= n1d ? n1d - 1 : xc);
+}
+
 __device__ __forceinline__ int32_t cell_key(int n1d, double sx, double sy,
                                             double sz) {
-  double s[3] = {sx, sy, sz};
+  double t;
+  const int x = cell_coord(n1d, sx, &t), y = cell_coord(n1d, sy, &t),
+            z = cell_coord(n1d, sz, &t);
+  return ((int32_t)x * n1d + y) * n1d + z;
+}
+
+// Mixed-precision bond record, 32 bytes (4 doubles of storage per bond):
+// float4 {offset of the scaled position inside its cell, in cell units,
+// x, y, z; length} and float4 {unit vector; cell coordinates x | y << 10 |
+// z << 20 as bits}.  The scaled position comes back in FP64 as
+// (c + offset) / n1d - 1/2, within about 6e-8 of a cell (7e-8 sigma for
+// cells of 1.13 sigma); the cell coordinates, hence the cell key, are those
+// cell_key computes from the FP64 scaled position.  At most 1024 cells per
+// side (checked at creation).
+template <class GeoT>
+__device__ __forceinline__ void rec_write_compact(double *br, const GeoT &g, int64_t b,
+                                                  const double *sp, const double *u,
+                                                  double l) {
+  double t[3];
   int c[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double x = s[k] + 0.5;
-    int xc = (int)floor(n1d * x);
-    if (xc == n1d) xc -= 1;
-    xc = xc < 0 ? 0 : (xc >= n1d ? n1d - 1 : xc);
-    c[k] = xc;
-  }
-  return ((int32_t)c[0] * n1d + c[1]) * n1d + c[2];
+  for (int k = 0; k < 3; ++k) c[k] = cell_coord(g.n1d, sp[k], &t[k]);
+  float4 *rec = reinterpret_cast<float4 *>(br + 4 * b);
+  rec[0] = make_float4((float)(t[0] - c[0]), (float)(t[1] - c[1]), (float)(t[2] - c[2]),
+                       (float)l);
+  rec[1] = make_float4((float)u[0], (float)u[1], (float)u[2],
+                       __int_as_float(c[0] | (c[1] << 10) | (c[2] << 20)));
+}
+
+// cell key of a compact record (the key cell_key gives its scaled position)
+template <class GeoT>
+__device__ __forceinline__ int32_t rec_key_compact(const double *br, const GeoT &g,
+                                                   int64_t b) {
+  const float4 v = reinterpret_cast<const float4 *>(br + 4 * b)[1];
+  const int packed = __float_as_int(v.w);
+  const int x = packed & 1023, y = (packed >> 10) & 1023, z = (packed >> 20) & 1023;
+  return ((int32_t)x * g.n1d + y) * g.n1d + z;
+}
+
+// scaled position s, unit vector u and length l of bond b: compact record
+__device__ __forceinline__ void rec_read_compact(const double *br, double inv_n1d, int64_t b,
+                                                 double *s, double *u, double *l) {
+  const float4 *rec = reinterpret_cast<const float4 *>(br + 4 * b);
+  const float4 a = rec[0], v = rec[1];
+  const int packed = __float_as_int(v.w);
+  s[0] = ((double)(packed & 1023) + (double)a.x) * inv_n1d - 0.5;
+  s[1] = ((double)((packed >> 10) & 1023) + (double)a.y) * inv_n1d - 0.5;
+  s[2] = ((double)((packed >> 20) & 1023) + (double)a.z) * inv_n1d - 0.5;
+  u[0] = v.x;
+  u[1] = v.y;
+  u[2] = v.z;
+  *l = a.w;
+}
+
+// the same from a 64-byte FP64 record
+__device__ __forceinline__ void rec_read_full(const double *br, int64_t b, double *s, double *u,
+                                              double *l) {
+  const double4 *rec = reinterpret_cast<const double4 *>(br + 8 * b);
+  const double4 a = rec[0], v = rec[1];
+  s[0] = a.x;
+  s[1] = a.y;
+  s[2] = a.z;
+  u[0] = a.w;
+  u[1] = v.x;
+  u[2] = v.y;
+  *l = v.z;
+}
+
+// either layout, as the system keeps it
+template <class GeoT>
+__device__ __forceinline__ void rec_read(const double *br, const GeoT &g, int64_t b, double *s,
+                                         double *u, double *l) {
+  if (g.compact)
+    rec_read_compact(br, g.inv_n1d, b, s, u, l);
+  else
+    rec_read_full(br, b, s, u, l);
 }
 
 // Non-redundant neighbour stencil of AssignCellNeighbors(false)
@@ -303,8 +376,9 @@ __device__ __forceinline__ void noise_xi(uint64_t seed, int64_t site,
 
 // Bond::ReInit + Object::UpdatePeriodic (src/bond.cpp:22-37,
 // src/object.cpp:202-210, src/linear_algebra.cpp:100-117) from the two site
-// positions; writes the 64-byte bond record {scaled position, unit
-// orientation, length}.  (Templated on the pointer/constant structs.)
+// positions; writes the bond record {scaled position, unit orientation,
+// length}: 64 bytes of FP64 on the FP64 path, the 32-byte compact record on
+// the mixed one.  (Templated on the pointer/constant structs.)
 template <class DevT, class GeoT>
 __device__ __forceinline__ void bond_reinit(const DevT &d, const GeoT &g,
                                             int64_t b, const double *r1,
@@ -333,14 +407,18 @@ __device__ __forceinline__ void bond_reinit(const DevT &d, const GeoT &g,
     si -= (double)nint(si);
     sp[k] = si;
   }
+  if (prec == SC_PREC_MIXED) {
+    rec_write_compact(d.br, g, b, sp, o, l);
+    return;
+  }
   double4 *rec = reinterpret_cast<double4 *>(d.br + 8 * b);
   rec[0] = make_double4(sp[0], sp[1], sp[2], o[0]);
   rec[1] = make_double4(o[1], o[2], l, 0.0);
 }
 
-// The mixed-precision bond record of bond_reinit from a bond vector o =
-// r2 - r1, its length l = sqrt(o.o) and unit vector u = o (1/l) that the
-// caller already holds (the same bits bond_reinit would compute).
+// The mixed-precision (compact) bond record of bond_reinit from a bond
+// vector o = r2 - r1, its length l = sqrt(o.o) and unit vector u = o (1/l)
+// that the caller already holds (the same bits bond_reinit would compute).
 template <class DevT, class GeoT>
 __device__ __forceinline__ void bond_store(const DevT &d, const GeoT &g, int64_t b,
                                            const double *r1, const double *o, double l,
@@ -354,9 +432,7 @@ __device__ __forceinline__ void bond_store(const DevT &d, const GeoT &g, int64_t
     si -= (double)nint(si);
     sp[k] = si;
   }
-  double4 *rec = reinterpret_cast<double4 *>(d.br + 8 * b);
-  rec[0] = make_double4(sp[0], sp[1], sp[2], u[0]);
-  rec[1] = make_double4(u[1], u[2], l, 0.0);
+  rec_write_compact(d.br, g, b, sp, u, l);
 }
 
 }  // namespace sc

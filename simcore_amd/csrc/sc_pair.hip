@@ -11,8 +11,10 @@
 // -contact2 x F (obj2) in candidate order: the serial scatter's summation
 // order, with no atomics and no second pass.
 //
-// Bond state is one 64-byte record per bond (scaled position, orientation,
-// length), so each gather touches one half cache line.
+// Bond state is one record per bond (scaled position, orientation,
+// length): 64 bytes of FP64 on the FP64 path, so each gather touches half a
+// cache line; 32 bytes on the mixed path (FP32 offset inside the bond's
+// cell, orientation and length; the scaled position is rebuilt in FP64).
 //
 // FP64 path: bit-for-bit the reference's arithmetic.  Mixed path: the same
 // FP64 geometry and branch rules with the divisions and square roots taken
@@ -31,17 +33,15 @@ struct BondRec {
   double s[3], u[3], len;
 };
 
-__device__ __forceinline__ BondRec load_rec(const double *br, int64_t b) {
-  const double4 *p = reinterpret_cast<const double4 *>(br + 8 * b);
-  const double4 a = p[0], c = p[1];
+// a bond's record: 64 bytes on the FP64 path, 32 on the mixed one (the
+// layout is a template argument here, so each kernel reads one)
+template <bool MIXED>
+__device__ __forceinline__ BondRec load_rec(const double *br, const Geo &g, int64_t b) {
   BondRec r;
-  r.s[0] = a.x;
-  r.s[1] = a.y;
-  r.s[2] = a.z;
-  r.u[0] = a.w;
-  r.u[1] = c.x;
-  r.u[2] = c.y;
-  r.len = c.z;
+  if (MIXED)
+    rec_read_compact(br, g.inv_n1d, b, r.s, r.u, &r.len);
+  else
+    rec_read_full(br, b, r.s, r.u, &r.len);
   return r;
 }
 
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   const int64_t nb = g.nb;
   const int64_t b = logical_block(xcd) * blockDim.x + threadIdx.x;
   if (b >= nb) return;
-  const BondRec own = load_rec(d.br, b);
+  const BondRec own = load_rec<MIXED>(d.br, g, b);
   double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
   if (g.nx > 0) {
     // crosslink forces first, in crosslink order (the reference applies
@@ -276,14 +276,14 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   BondRec oth_n{};
   if (q0 < q1) {
     ent_n = d.inc[q0];
-    oth_n = load_rec(d.br, ent_n.y);
+    oth_n = load_rec<MIXED>(d.br, g, ent_n.y);
   }
   for (int32_t q = q0; q < q1; ++q) {
     const int2 ent = ent_n;
     const BondRec oth = oth_n;
     if (q + 1 < q1) {
       ent_n = d.inc[q + 1];
-      oth_n = load_rec(d.br, ent_n.y);
+      oth_n = load_rec<MIXED>(d.br, g, ent_n.y);
     }
     double cf[3], ct[3];
     pair_contrib<MIXED>(d, g, own, ent, oth, cf, ct);
@@ -342,9 +342,9 @@ __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
           else
             hi = mid;
         }
-        const BondRec own = load_rec(d.br, b0 + lo);
+        const BondRec own = load_rec<MIXED>(d.br, g, b0 + lo);
         const int2 ent = d.inc[q];
-        const BondRec oth = load_rec(d.br, ent.y);
+        const BondRec oth = load_rec<MIXED>(d.br, g, ent.y);
         double cf[3], ct[3];
         pair_contrib<MIXED>(d, g, own, ent, oth, cf, ct);
         for (int k = 0; k < 3; ++k) {
