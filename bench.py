@@ -240,15 +240,27 @@ def measured_traffic(tag, kernel):
 
 def pair_bytes(precision, nb, nc):
     """Algorithmic HBM bytes of one launch of the pair/scatter kernel (either
-    launch shape): per bond its 64-byte state record read, its force and
-    torque sums written (6 FP32 on the mixed path, 6 FP64 on the FP64 path)
-    and its incidence offset read; per candidate the sticky force slot and
-    its two incidence entries {candidate, partner} read.  Re-reads of a
-    partner's record and the conditional slot write of in-range pairs are
-    not counted."""
+    launch shape): per bond its state record read (32 bytes on the mixed
+    path, 64 on the FP64 path), its force and torque sums written (6 FP32 on
+    the mixed path, 6 FP64 on the FP64 path) and its incidence offset read;
+    per candidate the sticky force slot and its two incidence entries
+    {candidate, partner} read.  Re-reads of a partner's record and the
+    conditional slot write of in-range pairs are not counted."""
+    rec = 32 if precision == "mixed" else 64
     slot = 16 if precision == "mixed" else 32
     sums = 24 if precision == "mixed" else 48
-    return nb * (64 + sums + 4) + nc * (slot + 2 * 8)
+    return nb * (rec + sums + 4) + nc * (slot + 2 * 8)
+
+
+def integrator_bytes(ns, nb):
+    """Algorithmic HBM bytes of the integrator per step (all its length-bin
+    launches), the per-unit figure of the step's layout: per site 148 bytes
+    (position read and written 48, previous position and random force 24
+    each -- written on a midstep, read on a full step --, the 12-byte noise
+    draw, displacement origin read 24 and accumulator read and written 16),
+    per bond 52 (force and torque sums read 24, the new bond state written
+    28)."""
+    return ns * 148 + nb * 52
 
 
 def pair_bytes_fp32_state(nb, nc):
@@ -367,16 +379,34 @@ def main():
     traffic = None
     if args.workload == "c1":
         traffic = measured_traffic(f"dense620k_{args.precision}", kname)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic,
-                "kernel": kname,
-                "kernel_avg_ms": pair_ms, "kernel_avg_ms_back_to_back": pair_warm_ms,
-                "algorithmic_bytes": bytes_pair,
-                "algorithmic_bytes_fp32_state": pair_bytes_fp32_state(fil.nb, nc),
-                "frac_fp32_state": round(pair_bytes_fp32_state(fil.nb, nc) / (pair_ms * 1e-3)
-                                         / 1e9 / HBM_PEAK_GBS, 4) if pair_ms > 0 else 0.0,
-                "candidates": nc, "per_step_ms": {k: v[0] for k, v in prof.items()}}
+    roofline_pair = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": kname,
+                     "kernel_avg_ms": pair_ms, "kernel_avg_ms_back_to_back": pair_warm_ms,
+                     "algorithmic_bytes": bytes_pair,
+                     "algorithmic_bytes_fp32_state": pair_bytes_fp32_state(fil.nb, nc),
+                     "frac_fp32_state": round(pair_bytes_fp32_state(fil.nb, nc)
+                                              / (pair_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                     if pair_ms > 0 else 0.0,
+                     "candidates": nc}
+    # the dominant kernel of the step: the integrator (its launches of one
+    # step, HIP events around them on the system's stream)
+    int_ms = prof["integrate"][0]
+    bytes_int = integrator_bytes(fil.ns, fil.nb)
+    achieved_int = bytes_int / (int_ms * 1e-3) / 1e9 if int_ms > 0 else 0.0
+    int_traffic = None
+    if args.workload == "c1" and args.precision == "mixed":
+        tf = measured_traffic(f"dense620k_{args.precision}", "k_integrate_long<16, 2, false>")
+        tm = measured_traffic(f"dense620k_{args.precision}", "k_integrate_long<16, 2, true>")
+        if tf is not None and tm is not None:
+            int_traffic = 0.5 * (tf + tm)  # a full step and a midstep, averaged
+    roofline = {"bound": "hbm", "achieved": round(achieved_int, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved_int / HBM_PEAK_GBS, 4),
+                "traffic": int_traffic,
+                "kernel": "integrator (k_integrate_long, all length bins of a step)",
+                "kernel_avg_ms": int_ms, "algorithmic_bytes": bytes_int,
+                "per_step_ms": {k: v[0] for k, v in prof.items()}}
     dev.close()
 
     extra = {}
@@ -430,7 +460,7 @@ def main():
         "data": "synthetic",
         "config": dict(desc, precision=args.precision, graph=True, spatial_order=spatial,
                        launch_shape=tune),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "roofline_pair": roofline_pair, "cpu_baseline": cpu,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
