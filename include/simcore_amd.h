@@ -38,9 +38,11 @@ extern "C" {
 /* arithmetic of the pair path */
 #define SC_PREC_FP64 0  /* everything FP64, bit-compatible with the CPU path */
 #define SC_PREC_MIXED 1 /* FP64 pair geometry with the hardware FP32
-                           reciprocal / reciprocal square root, FP32 sticky
-                           forces and per-bond sums, FP64 tensions and
-                           positions                                         */
+                           reciprocal / reciprocal square root, 32-byte bond
+                           records (FP32 offset inside the bond's cell, FP32
+                           orientation and length), FP32 sticky forces and
+                           per-bond sums, FP64 tensions and positions; at
+                           most 1024 cells per side                          */
 
 typedef struct sc_system *sc_handle;
 
@@ -122,7 +124,10 @@ int sc_step_poll(sc_handle h, int *done);
  * Any pointer may be NULL. */
 int sc_get_sites(sc_handle h, double *r, double *r_prev, double *f_rand);
 /* Bond state after Bond::ReInit (src/bond.cpp:22-37): centre, scaled
- * position, unit orientation, length.  Any pointer may be NULL. */
+ * position, unit orientation, length.  Any pointer may be NULL.  Mixed
+ * precision: the scaled position, orientation and length as the compact
+ * record holds them (within 3e-8 of a cell, FP32); the centre is computed
+ * from the FP64 sites. */
 int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u,
                  double *len);
 /* Per-bond force/torque after ApplyPairInteractions of the last step

@@ -420,6 +420,12 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
     for (int j = 0; j < 3; ++j)
       if (i != j && (p.unit_cell[3 * i + j] != 0.0 || p.unit_cell_inv[3 * i + j] != 0.0))
         return SC_ERR_ARG;  // orthorhombic boxes only
+  {
+    const int64_t n = cells_per_side(p);
+    if (n * n * n >= (int64_t)1 << 31) return SC_ERR_ARG;
+    // the mixed path's compact bond records pack 10 bits per cell coordinate
+    if (p.precision == SC_PREC_MIXED && n > 1024) return SC_ERR_ARG;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SC_ERR_HIP;
   sc_system *h = new sc_system();
@@ -439,20 +445,12 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
   g.n1d = n1d;
   g.thr = 0.25 * cell_len * cell_len;
   g.n_cells = (int64_t)n1d * n1d * n1d;
-  if (g.n_cells >= (int64_t)1 << 31) {
-    delete h;
-    return SC_ERR_ARG;
-  }
   g.stoch = p.stoch_flag ? 1 : 0;
   g.seed = p.seed;
   // mixed precision keeps 32-byte bond records whose cell coordinates are
-  // packed in 10 bits each
+  // packed in 10 bits each (at most 1024 cells per side, checked above)
   g.compact = p.precision == SC_PREC_MIXED ? 1 : 0;
   g.inv_n1d = 1.0 / n1d;
-  if (g.compact && n1d > 1024) {
-    delete h;
-    return SC_ERR_ARG;
-  }
   h->arena = arena;
   if (stream) {
     h->stream = stream;

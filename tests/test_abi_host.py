@@ -125,3 +125,19 @@ def test_ensemble_create_without_device_fails_cleanly():
     h = ctypes.c_void_p()
     rc = sc.lib.sc_ensemble_create(ctypes.byref(prm), ctypes.byref(h))
     assert rc == -2 and not h.value
+
+
+def test_mixed_precision_limits_cells_per_side():
+    """The mixed path's compact bond records pack 10 bits per cell
+    coordinate: more than 1024 cells per side is refused at creation
+    (before any device is touched); FP64 systems keep the 2^31-cell limit."""
+    from ctypes import byref, c_void_p
+    h = c_void_p()
+    # side 1200 sigma, cells >= 2^(1/6) sigma: 1069 per side
+    p = sc.params_init(600.0, delta=1e-4, min_cell_length=1.0, seed=1,
+                       precision=sc.SC_PREC_MIXED)
+    assert sc.lib.sc_create(byref(p), byref(h)) == -1
+    # side 2600 sigma: 2316 per side, more than 2^31 cells
+    p = sc.params_init(1300.0, delta=1e-4, min_cell_length=1.0, seed=1,
+                       precision=sc.SC_PREC_FP64)
+    assert sc.lib.sc_create(byref(p), byref(h)) == -1
