@@ -253,42 +253,10 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     }
   }
 #pragma unroll
-  for (int k = 0; k < S; ++k) {
-    const int i = base + k;
-    for (int q = 0; q < 3; ++q) {
-      Fb[k][q] = 0.0;
-      Tb[k][q] = 0.0;
-      fr[k][q] = 0.0;
-    }
-    if (i < m)
-      for (int q = 0; q < 3; ++q) {
-        Fb[k][q] = (double)d.bf32[q * g.nb + b0 + i];
-        Tb[k][q] = (double)d.bt32[q * g.nb + b0 + i];
-      }
-  }
+  for (int k = 0; k < S; ++k)
+    for (int q = 0; q < 3; ++q) fr[k][q] = 0.0;
 
-  // ---- ApplyInteractionForces (src/filament.cpp:992-1048) ---------------
   const double linv = rcp_fast(bl);
-  double sf[S][3], pt[S][3], sfm[S][3], ptm[S][3], fs[S][3];
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    cross3(u[k], Tb[k], pt[k]);
-    for (int q = 0; q < 3; ++q) {
-      pt[k][q] *= linv;
-      sf[k][q] = 0.5 * Fb[k][q];
-    }
-  }
-  site_at3<G, S, -1>(sf, sfm, lane, gl);
-  site_at3<G, S, -1>(pt, ptm, lane, gl);
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    const int i = base + k;
-    for (int q = 0; q < 3; ++q) fs[k][q] = 0.0;
-    if (i >= 1)
-      for (int q = 0; q < 3; ++q) fs[k][q] = fs[k][q] + sfm[k][q] + (-ptm[k][q]);
-    if (i <= n - 2)
-      for (int q = 0; q < 3; ++q) fs[k][q] = fs[k][q] + sf[k][q] + pt[k][q];
-  }
 
   // ---- angles and tangents ----------------------------------------------
   double um[S][3], up[S][3], t[S][3], cs[S];
@@ -350,6 +318,43 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       }
     }
   }
+  // ---- ApplyInteractionForces (src/filament.cpp:992-1048) ---------------
+  // (after the random forces: the bond sums are loaded here, so they and
+  // the site forces are not live across the random-force solve)
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int i = base + k;
+    for (int q = 0; q < 3; ++q) {
+      Fb[k][q] = 0.0;
+      Tb[k][q] = 0.0;
+    }
+    if (i < m)
+      for (int q = 0; q < 3; ++q) {
+        Fb[k][q] = (double)d.bf32[q * g.nb + b0 + i];
+        Tb[k][q] = (double)d.bt32[q * g.nb + b0 + i];
+      }
+  }
+  double sf[S][3], pt[S][3], sfm[S][3], ptm[S][3], fs[S][3];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    cross3(u[k], Tb[k], pt[k]);
+    for (int q = 0; q < 3; ++q) {
+      pt[k][q] *= linv;
+      sf[k][q] = 0.5 * Fb[k][q];
+    }
+  }
+  site_at3<G, S, -1>(sf, sfm, lane, gl);
+  site_at3<G, S, -1>(pt, ptm, lane, gl);
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const int i = base + k;
+    for (int q = 0; q < 3; ++q) fs[k][q] = 0.0;
+    if (i >= 1)
+      for (int q = 0; q < 3; ++q) fs[k][q] = fs[k][q] + sfm[k][q] + (-ptm[k][q]);
+    if (i <= n - 2)
+      for (int q = 0; q < 3; ++q) fs[k][q] = fs[k][q] + sf[k][q] + pt[k][q];
+  }
+
   if (g.stoch)
 #pragma unroll
     for (int k = 0; k < S; ++k) {
@@ -610,8 +615,11 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   }
 }
 
+// two sites per lane: at most 128 VGPRs, four waves per SIMD (the midstep
+// needs 129 left to itself); four sites per lane: two waves
 template <int G, int S, bool MID>
-__global__ __launch_bounds__(256) void k_integrate_long(Dev d, Geo g, const int32_t *list,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 2 ? 4 : 1)))
+void k_integrate_long(Dev d, Geo g, const int32_t *list,
                                                         int count) {
   Ctl *ctl = d.ctl;
   if (ctl->abort) return;

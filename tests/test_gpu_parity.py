@@ -37,6 +37,32 @@ def test_initial_cell_keys_and_candidates_bit_exact():
         np.testing.assert_array_equal(a, b)
 
 
+def test_mixed_compact_bond_records_at_load():
+    """Mixed path: 32-byte bond records (FP32 offset inside the bond's cell,
+    packed cell coordinates, FP32 orientation and length).  Against the FP64
+    records of the CPU path: scaled positions within 4e-8 of a cell (1/35
+    of the box per cell here), orientations and lengths to FP32 rounding,
+    identical cell keys and candidates, centres from the FP64 sites
+    bit-identical."""
+    print("case: C0 64x20 seed 7 mixed, load-time records", flush=True)
+    p, fil, dev, ora = _pair(precision="mixed")
+    dpos, dsp, du, dln = dev.bonds()
+    opos, osp, ou, oln = ora.bonds()
+    np.testing.assert_array_equal(dpos, opos)
+    d = np.abs(dsp - osp)
+    d = np.minimum(d, 1.0 - d)  # periodic in scaled units
+    n1d = dev.counters()["n_cells_1d"]
+    print("max scaled-position error in cells", d.max() * n1d)
+    assert d.max() * n1d <= 4e-8
+    assert np.abs(du - ou).max() <= 1e-7
+    np.testing.assert_allclose(dln, oln, rtol=1e-7)
+    np.testing.assert_array_equal(dev.cell_keys(), ora.cell_keys())
+    ob1, ob2 = culled(*ora.candidates(), fil.site_off)
+    db1, db2 = dev.candidates()
+    np.testing.assert_array_equal(db1, ob1)
+    np.testing.assert_array_equal(db2, ob2)
+
+
 def test_fp64_one_step_bit_exact():
     p, fil, dev, ora = _pair()
     print("case: C0 fp64 1 step", flush=True)
