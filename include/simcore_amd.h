@@ -40,9 +40,9 @@ extern "C" {
 #define SC_PREC_MIXED 1 /* FP64 pair geometry with the hardware FP32
                            reciprocal / reciprocal square root, 32-byte bond
                            records (FP32 offset inside the bond's cell, FP32
-                           orientation and length), FP32 sticky forces and
-                           per-bond sums, FP64 tensions and positions; at
-                           most 1024 cells per side                          */
+                           orientation and length), FP32 sticky forces,
+                           per-bond sums and random forces, FP64 tensions
+                           and positions; at most 1024 cells per side        */
 
 typedef struct sc_system *sc_handle;
 
@@ -120,8 +120,8 @@ int sc_step_async(sc_handle h, int32_t n_steps);
 int sc_step_poll(sc_handle h, int *done);
 
 /* Object::GetPosition / GetPrevPosition of sites (include/simcore/
- * object.hpp:82-88); f_rand is Site::random_force_ (src/site.cpp:103-107).
- * Any pointer may be NULL. */
+ * object.hpp:82-88); f_rand is Site::random_force_ (src/site.cpp:103-107),
+ * FP32-rounded on the mixed path.  Any pointer may be NULL. */
 int sc_get_sites(sc_handle h, double *r, double *r_prev, double *f_rand);
 /* Bond state after Bond::ReInit (src/bond.cpp:22-37): centre, scaled
  * position, unit orientation, length.  Any pointer may be NULL.  Mixed

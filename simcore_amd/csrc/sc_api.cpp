@@ -852,7 +852,17 @@ int sc_get_sites(sc_handle h, double *r, double *r_prev, double *f_rand) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (r && (rc = fetch_mapped(h, h->d.r, r, h->g.ns, 3, true))) return rc;
   if (r_prev && (rc = fetch_mapped(h, h->d.rp, r_prev, h->g.ns, 3, true))) return rc;
-  if (f_rand && (rc = fetch_mapped(h, h->d.fr, f_rand, h->g.ns, 3, true))) return rc;
+  if (f_rand) {
+    if (h->g.compact) {  // mixed precision: FP32 random forces
+      const int64_t ns = h->g.ns;
+      std::vector<float> tmp(3 * (size_t)ns);
+      HIPCHK(h, hipMemcpy(tmp.data(), h->d.fr, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost));
+      std::vector<double> wide(tmp.begin(), tmp.end());
+      to_caller(h, h->site_map, wide.data(), f_rand, ns, 3);
+    } else if ((rc = fetch_mapped(h, h->d.fr, f_rand, h->g.ns, 3, true))) {
+      return rc;
+    }
+  }
   return SC_OK;
 }
 
@@ -1124,7 +1134,7 @@ std::vector<CkField> ck_fields(sc_system *h, int64_t nc) {
   const size_t slot = h->p.precision == SC_PREC_FP64 ? sizeof(double4) : sizeof(float4);
   void *slots = h->p.precision == SC_PREC_FP64 ? (void *)d.slot64 : (void *)d.slot32;
   return {{d.r, 24 * (size_t)ns},        {d.rp, 24 * (size_t)ns},
-          {d.fr, 24 * (size_t)ns},       {d.dr0, 24 * (size_t)ns},
+          {d.fr, (h->g.compact ? 12 : 24) * (size_t)ns}, {d.dr0, 24 * (size_t)ns},
           {d.acc, 8 * (size_t)ns},       {d.acc_prev, 8 * (size_t)ns},
           {d.br, (h->g.compact ? 32 : 64) * (size_t)nb},
           {d.bf ? (void *)d.bf : (void *)d.bf32, (d.bf ? 24 : 12) * (size_t)nb},

@@ -135,7 +135,9 @@ __global__ __launch_bounds__(64) void k_integrate(Dev d, Geo g,
     }
     if (i < n && !mid && g.stoch) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) RR[q][i] = d.fr[q * ns + s0 + i];
+      for (int q = 0; q < 3; ++q)
+        RR[q][i] = g.compact ? (double)reinterpret_cast<const float *>(d.fr)[q * ns + s0 + i]
+                             : d.fr[q * ns + s0 + i];
     }
   }
   __syncthreads();
@@ -351,8 +353,13 @@ __global__ __launch_bounds__(64) void k_integrate(Dev d, Geo g,
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
+          if (g.compact) {  // mixed precision keeps the random force in FP32
+            p[q] = (double)(float)p[q];
+            reinterpret_cast<float *>(d.fr)[q * ns + s0 + i] = (float)p[q];
+          } else {
+            d.fr[q * ns + s0 + i] = p[q];
+          }
           RR[q][i] = p[q];
-          d.fr[q * ns + s0 + i] = p[q];
         }
       }
     }

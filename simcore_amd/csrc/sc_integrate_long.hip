@@ -313,8 +313,11 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
         } else {
           for (int q = 0; q < 3; ++q) fr[k][q] = fr[k][q] + Tp[k] * u[k][q] - Tm[k] * um[k][q];
         }
+        // stored in FP32 (both half steps use the rounded value)
+        for (int q = 0; q < 3; ++q) fr[k][q] = (double)(float)fr[k][q];
         if (i < n)
-          for (int q = 0; q < 3; ++q) d.fr[q * ns + s0 + i] = fr[k][q];
+          for (int q = 0; q < 3; ++q)
+            reinterpret_cast<float *>(d.fr)[q * ns + s0 + i] = (float)fr[k][q];
       }
     }
   }
@@ -360,7 +363,8 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     for (int k = 0; k < S; ++k) {
       // full step: the projected random force of the midstep
       if (!mid && base + k < n)
-        for (int q = 0; q < 3; ++q) fr[k][q] = d.fr[q * ns + s0 + base + k];
+        for (int q = 0; q < 3; ++q)
+          fr[k][q] = reinterpret_cast<const float *>(d.fr)[q * ns + s0 + base + k];
       for (int q = 0; q < 3; ++q) fs[k][q] += fr[k][q];
     }
 

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
       }
     }
     if (!mid && g.stoch)
-      for (int q = 0; q < 3; ++q) fr[q] = d.fr[q * ns + si];
+      for (int q = 0; q < 3; ++q) fr[q] = reinterpret_cast<const float *>(d.fr)[q * ns + si];
     if (!mid) {
       acc = d.acc[si];
       for (int q = 0; q < 3; ++q) dr0[q] = d.dr0[q * ns + si];
@@ -141,8 +141,10 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
       } else {
         for (int q = 0; q < 3; ++q) fr[q] = fr[q] + Tp * u[q] - Tm * um[q];
       }
+      // stored in FP32 (both half steps use the rounded value)
+      for (int q = 0; q < 3; ++q) fr[q] = (double)(float)fr[q];
       if (act)
-        for (int q = 0; q < 3; ++q) d.fr[q * ns + si] = fr[q];
+        for (int q = 0; q < 3; ++q) reinterpret_cast<float *>(d.fr)[q * ns + si] = (float)fr[q];
     }
     if (act)
       for (int q = 0; q < 3; ++q) d.rp[q * ns + si] = rp[q];
