@@ -349,7 +349,7 @@ def main():
     elif not args.no_autotune:
         # launch shapes that leave results unchanged, chosen from a
         # checkpoint of the warmed-up state (the state is restored)
-        tune = dev.autotune(steps=120)
+        tune = dev.autotune(steps=300)
         log(f"[bench] autotune: {tune}")
     c0 = dev.counters()
     wall = timed_steps(dev, args.steps)
