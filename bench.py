@@ -238,6 +238,24 @@ def measured_traffic(tag, kernel):
     return None
 
 
+def measured_traffic_sum(tag, part, suffix):
+    """Measured bytes of all kernels whose name contains `part` and ends with
+    `suffix` (e.g. every length bin's integrator launch of one step kind),
+    summed; None without a measurement."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{tag}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    vals = [e["traffic_bytes"] for k, e in d.get("kernels", {}).items()
+            if part in k and k.endswith(suffix) and "traffic_bytes" in e]
+    return float(sum(vals)) if vals else None
+
+
+# workloads with a committed PMC traffic measurement
+TRAFFIC_TAG = {"c1": "dense620k", "c2": "uneven1600k"}
+
+
 def pair_bytes(precision, nb, nc):
     """Algorithmic HBM bytes of one launch of the pair/scatter kernel (either
     launch shape): per bond its state record read (32 bytes on the mixed
@@ -377,8 +395,9 @@ def main():
     kname = ("k_pair_bal" if balanced else "k_pair_fused") + (
         "<true>" if args.precision == "mixed" else "<false>")
     traffic = None
-    if args.workload == "c1":
-        traffic = measured_traffic(f"dense620k_{args.precision}", kname)
+    ttag = f"{TRAFFIC_TAG[args.workload]}_{args.precision}" if args.workload in TRAFFIC_TAG else None
+    if ttag:
+        traffic = measured_traffic(ttag, kname)
     roofline_pair = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
@@ -396,11 +415,12 @@ def main():
     bytes_int = integrator_bytes(fil.ns, fil.nb)
     achieved_int = bytes_int / (int_ms * 1e-3) / 1e9 if int_ms > 0 else 0.0
     int_traffic = None
-    if args.workload == "c1" and args.precision == "mixed":
-        tf = measured_traffic(f"dense620k_{args.precision}", "k_integrate_long<16, 2, false>")
-        tm = measured_traffic(f"dense620k_{args.precision}", "k_integrate_long<16, 2, true>")
+    if ttag and args.precision == "mixed":
+        # every length bin's launch of a full step, of a midstep; averaged
+        tf = measured_traffic_sum(ttag, "k_integrate_long<", ", false>")
+        tm = measured_traffic_sum(ttag, "k_integrate_long<", ", true>")
         if tf is not None and tm is not None:
-            int_traffic = 0.5 * (tf + tm)  # a full step and a midstep, averaged
+            int_traffic = 0.5 * (tf + tm)
     roofline = {"bound": "hbm", "achieved": round(achieved_int, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved_int / HBM_PEAK_GBS, 4),
                 "traffic": int_traffic,

@@ -325,11 +325,15 @@ int fetch_mapped(sc_system *h, const double *dev, double *out, int64_t n, int co
 }
 
 // Space-filling-curve order of the filaments' centres: 10-bit Morton code
-// of the wrapped scaled centre per dimension, stable in caller order.
+// of the wrapped scaled centre per dimension, stable in caller order.  The
+// top 3 bits per dimension (512 blocks) come first, then the integrator's
+// length bin, then the rest of the code: inside a block each bin's filaments
+// are contiguous, so one bin's launch does not fetch cache lines that
+// mostly hold another bin's sites, while a bond's partners stay in its block.
 std::vector<int32_t> spatial_permutation(const sc_params &p, int32_t nf, const int32_t *so,
                                          const double *r) {
   const int64_t ns = so[nf];
-  std::vector<uint32_t> key(nf);
+  std::vector<uint64_t> key(nf);
   for (int f = 0; f < nf; ++f) {
     double c[3] = {0, 0, 0};
     for (int64_t i = so[f]; i < so[f + 1]; ++i)
@@ -342,7 +346,10 @@ std::vector<int32_t> spatial_permutation(const sc_params &p, int32_t nf, const i
       uint32_t q = (uint32_t)std::min(1023.0, std::max(0.0, std::floor((sk + 0.5) * 1024.0)));
       for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + k);
     }
-    key[f] = code;
+    const int n = so[f + 1] - so[f];
+    uint64_t bin = 0;
+    while (bin < 5 && n > (8 << bin)) ++bin;  // bins of 8, 16, ..., 256 sites
+    key[f] = ((uint64_t)(code >> 21) << 33) | (bin << 30) | code;
   }
   std::vector<int32_t> perm(nf);
   for (int f = 0; f < nf; ++f) perm[f] = f;
