@@ -408,3 +408,47 @@ def test_bond_lengths_stay_within_the_renormalisation_threshold():
             worst = max(worst, float(np.max(np.abs(ln - bl[f]) / bl[f])))
     print("worst relative bond-length deviation", worst)
     assert worst <= 1e-3
+
+
+# ------------------------------------------- trajectory-level rules ----
+# Closed-form scenarios (tests/scenarios.py) for the rules the reference's
+# source fixes but no published vector does: the sticky pair force, the
+# double-counted rebuild trigger, the alternating renormalisation and the
+# MakePairs emission order.
+from tests import scenarios as S  # noqa: E402
+
+
+def test_sticky_force_persists_until_rebuild():
+    p, fil = S.sticky_setup()
+    S.run_sticky(O.OracleSystem(p, fil, n_threads=1))
+
+
+def test_sticky_scenario_rejects_a_fresh_evaluation():
+    """The scenario's expectation is not met by a pass that re-evaluates an
+    out-of-range pair to zero: at h1 the closed form is 0, not F(h0)."""
+    assert S.wca_force_magnitude(S.STICKY_H0) > 1.0
+    assert S.STICKY_H1 > S.RCUT > S.STICKY_H0
+    assert math.sqrt(0.25 + S.STICKY_H0 ** 2) > S.RCUT
+
+
+def test_rebuild_trigger_counts_each_displacement_twice():
+    p, fil = S.trigger_setup()
+    S.run_trigger(O.OracleSystem(p, fil, n_threads=1), cycles=2)
+
+
+def test_renormalisation_alternates_head_and_tail():
+    p, fil, r0, u = S.renorm_setup()
+    S.run_renorm(O.OracleSystem(p, fil, n_threads=1), r0, u)
+
+
+@pytest.mark.parametrize("radius,n1d", [(1.8, 3), (2.4, 4), (3.0, 5), (3.5, 6)])
+def test_candidate_order_is_the_reference_walk(radius, n1d):
+    p = sc.params_init(radius, delta=1e-4, min_cell_length=1.0, seed=3)
+    fil = S.make_filaments(S.random_sticks(60, 4, radius, seed=n1d))
+    o = O.OracleSystem(p, fil, n_threads=2)
+    assert o.counters()["n_cells_1d"] == n1d
+    keys = o.cell_keys()
+    b1, b2 = o.candidates()               # as emitted (the cull comes later)
+    walk = S.reference_pair_walk(keys, n1d)
+    assert len(walk) > 1000
+    assert list(zip(b1.tolist(), b2.tolist())) == walk
