@@ -167,6 +167,16 @@ int sc_set_sites(sc_handle h, const double *r);
 int sc_filament_stats(sc_handle h, double *e2e, double *com);
 int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows,
                        int64_t *hist);
+/* No counterpart in the reference: sc_theta_histogram for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  It
+ * calls no hipMalloc/hipFree: its counters live in a per-GPU device array of
+ * at most 65536 counters (n_bins * n_rows; larger shapes return SC_ERR_ARG),
+ * cleared at each call, and the counts are ADDED to hist as in
+ * sc_theta_histogram.  Every call on one GPU shares that array, so calls on
+ * one GPU that overlap in time (two host threads, or handles on different
+ * streams without ordering) race. */
+int sc_theta_histogram_noalloc(sc_handle h, int32_t n_bins, int32_t n_rows,
+                               int64_t *hist);
 
 /* Checkpoint / resume.  The reference's checkpoint (Species::WriteCheckpoints,
  * include/simcore/species.hpp:238-258; Mesh::WriteCheckpoint,

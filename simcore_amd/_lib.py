@@ -31,7 +31,8 @@ EXPORTS = [
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
     "sc_generate_filaments", "sc_copy_probe", "sc_time_pair", "sc_step_async",
     "sc_step_poll",
-    "sc_filament_stats", "sc_theta_histogram", "sc_output_header", "sc_spec_frame",
+    "sc_filament_stats", "sc_theta_histogram", "sc_theta_histogram_noalloc",
+    "sc_output_header", "sc_spec_frame",
     "sc_posit_frame", "sc_checkpoint_save", "sc_checkpoint_load", "sc_autotune",
     "sc_set_launch_shape", "sc_get_launch_shape",
     "sc_load_crosslinks", "sc_get_crosslinks", "sc_generate_crosslinks",
@@ -152,6 +153,7 @@ def _load():
         "sc_generate_crosslinks": ([POINTER(ScParams), c_int32, ip, dp, POINTER(ScFilamentParams),
                                     c_int64, c_double, c_uint64, ip, dp], c_int),
         "sc_theta_histogram": ([H, c_int32, c_int32, lp], c_int),
+        "sc_theta_histogram_noalloc": ([H, c_int32, c_int32, lp], c_int),
         "sc_device_bytes": ([H, POINTER(c_size_t)], c_int),
         "sc_profile_enable": ([H, c_int], c_int),
         "sc_profile_read": ([H, dp, lp, c_int], c_int),

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,12 +71,43 @@ struct sc_system {
   // allocations (from `arena` when set, else hipMalloc)
   sc::Arena *arena = nullptr;
   bool own_stream = true;
+  int device = 0;  // HIP device current at creation
   std::vector<void *> fixed_allocs;
   std::vector<void *> cand_allocs;
   size_t fixed_bytes = 0, cand_bytes = 0;
 };
 
 namespace {
+
+// Scratch of sc_theta_histogram_noalloc: histogram counters in a file-scope
+// device array, so that entry point allocates nothing.  Each GPU holds its
+// own copy for the life of the process; its device address is looked up
+// once per device (scratch_setup, from system_create) and used from then on
+// -- the array's name is a host-side stand-in, never a device address.
+constexpr int64_t kScratchCounters = (int64_t)1 << 16;  // 512 KiB
+static __device__ __align__(256) unsigned long long scratch[kScratchCounters];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: the device address of its scratch copy.
+hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = static_cast<unsigned long long *>(ptr);
+  return hipSuccess;
+}
+
+unsigned long long *scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 // InteractionManager::InitInteractions -> CellList::SetMinCellLength /
 // Init (src/interaction_manager.cpp:46-54, src/cell_list.cpp:9-64)
@@ -433,9 +465,13 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
     // the mixed path's compact bond records pack 10 bits per cell coordinate
     if (p.precision == SC_PREC_MIXED && n > 1024) return SC_ERR_ARG;
   }
-  int ndev = 0;
+  int ndev = 0, device = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SC_ERR_HIP;
+  if (hipGetDevice(&device) != hipSuccess) return SC_ERR_HIP;
+  // the allocation-free histogram's scratch address on this device
+  if (scratch_setup(device) != hipSuccess) return SC_ERR_HIP;
   sc_system *h = new sc_system();
+  h->device = device;
   h->p = p;
   Geo &g = h->g;
   std::memcpy(g.h, p.unit_cell, sizeof(g.h));
@@ -1113,6 +1149,35 @@ int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows, int64_t *his
   if (e != hipSuccess) return fail(h, SC_ERR_HIP, std::string("sc_theta_histogram: ") +
                                                       hipGetErrorString(e));
   for (size_t k = 0; k < n; ++k) hist[k] += (int64_t)host[k];
+  return SC_OK;
+}
+
+int sc_theta_histogram_noalloc(sc_handle h, int32_t n_bins, int32_t n_rows, int64_t *hist) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (n_bins <= 0 || n_rows <= 0 || !hist) return fail(h, SC_ERR_ARG, "bad histogram shape");
+  const int64_t n = (int64_t)n_bins * (int64_t)n_rows;
+  if (n > kScratchCounters)
+    return fail(h, SC_ERR_ARG, "histogram of " + std::to_string(n) + " counters exceeds the " +
+                                   std::to_string(kScratchCounters) +
+                                   "-counter scratch of sc_theta_histogram_noalloc");
+  unsigned long long *buf = scratch_ptr(h->device);
+  if (!buf) return fail(h, SC_ERR_STATE, "no scratch address for this device");
+  // the array keeps the last call's counts: clear the part this call uses
+  // (a fill kernel, not a memset node, so the call may be captured)
+  sc::launch_zero_i32(reinterpret_cast<int32_t *>(buf), 2 * n, h->stream);
+  sc::launch_theta_hist(h->d, h->g, n_bins, n_rows, buf, h->stream);
+  HIPCHK(h, hipGetLastError());
+  // counts come back through a fixed host block (no allocation either side)
+  constexpr int64_t kChunk = 512;
+  unsigned long long host[kChunk];
+  for (int64_t k0 = 0; k0 < n; k0 += kChunk) {
+    const int64_t m = std::min(kChunk, n - k0);
+    HIPCHK(h, hipMemcpyAsync(host, buf + k0, sizeof(unsigned long long) * m,
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int64_t k = 0; k < m; ++k) hist[k0 + k] += (int64_t)host[k];
+  }
   return SC_OK;
 }
 
