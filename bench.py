@@ -2,16 +2,23 @@
 """Benchmark of the filament BD step on one MI355X.
 
 Metric: segment-steps/s = bonds x steps / wall-seconds, one step being one
-reference i_step (two per Delta t).  Default workload: the dense single
-system, 20,000 filaments x 32 sites (620,000 bonds), persistence length
-50 sigma, periodic cube at volume fraction 0.05, WCA eps=1 cutoff
-2^(1/6) sigma, dt=1e-4; FP32 pair forces, FP64 tension solve and positions,
-the step replayed from a hipGraph.  Synthetic initial state (reference-style
-random insertion), state resident in HBM before the timed region.
+reference i_step (two per Delta t), with the mixed path's force error
+against the FP64 path reported beside it.  Default workload: the
+uneven-length system, 50,000 filaments with log-normal site counts (median
+24, sigma_ln 0.8, clipped to [8, 256]; 1.61M bonds), persistence length
+U[10, 1000] sigma, periodic cube at volume fraction 0.08, WCA eps=1 cutoff
+2^(1/6) sigma, dt=1e-5 (the explicit midstep scheme is unstable at 1e-4 for
+lp = 1000 sigma); FP32 pair forces, FP64 tension solve and positions, the
+step replayed from a hipGraph, launch shapes autotuned.  Synthetic initial
+state (reference-style random insertion), state resident in HBM before the
+timed region.
 
-Printed on one JSON line: the metric, the pair kernel's roofline (HIP-event
-timing of the same kernel), the plain FP64 GPU path on the same workload and
-the CPU oracle (OpenMP, the reference's decomposition) on a bounded sample.
+Printed on one JSON line: the metric; the roofline of the pair kernel (the
+step's largest kernel) and of the integrator, from HIP-event timing on the
+system's stream; the per-bond force error of the mixed path against the
+FP64 path from identical state; the plain FP64 GPU path on the same
+workload (eager and graph-replayed); and the CPU oracle (OpenMP, the
+reference's decomposition) on a bounded sample with the host CPU named.
 """
 import argparse
 import json
@@ -32,52 +39,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def box_for_fraction(n_sites, bond_length, diameter, phi):
-    n_sites = np.asarray(n_sites, dtype=float)
-    L = (n_sites - 1) * bond_length
-    v = np.sum(0.25 * np.pi * diameter ** 2 * L + np.pi / 6.0 * diameter ** 3)
-    return 0.5 * (v / phi) ** (1.0 / 3.0)
-
-
 def workload(name, sc):
-    """(params, filaments, description dict)."""
+    """(params, filaments, description dict, crosslinks or None, crosslink
+    parameters or None) of a named configuration (simcore_amd.workloads)."""
+    from simcore_amd import workloads as W
     if name == "c1":
-        nf, ns = 20000, 32
-        R = box_for_fraction([ns] * nf, 1.0, 1.0, 0.05)
-        p = sc.params_init(R, delta=1e-4, min_cell_length=1.0, seed=20240001)
-        fil = sc.generate_filaments(p, [ns] * nf, float(ns - 1), 50.0, 1.0, 2.0, seed=11)
-        desc = {"workload": "dense single system: 20000 filaments x 32 sites, lp 50 sigma, "
-                            "volume fraction 0.05, periodic cube, WCA, dt 1e-4",
-                "n_filaments": nf, "sites_per_filament": ns, "persistence_length": 50.0,
-                "volume_fraction": 0.05, "box_side": 2 * R}
-    elif name == "c2":
-        rng = np.random.default_rng(20240002)
-        nf = 50000
-        n = np.rint(rng.lognormal(np.log(24.0), 0.8, nf)).astype(np.int64)
-        n = np.clip(n, 8, 256)
-        lp = rng.uniform(10.0, 1000.0, nf)
-        R = box_for_fraction(n, 1.0, 1.0, 0.08)
-        # dt 1e-5: the midstep scheme is explicit, and lp = 1000 sigma puts
-        # the stiffest bending mode at ~16 kappa/(l^3 gamma_perp) ~ 7e4 per
-        # unit time, beyond the 2/dt stability limit at dt = 1e-4
-        p = sc.params_init(R, delta=1e-5, min_cell_length=1.0, seed=20240003)
-        fil = sc.generate_filaments(p, n, (n - 1).astype(float), lp, 1.0, 2.0, seed=12)
-        desc = {"workload": "uneven lengths: 50000 filaments, sites lognormal(median 24, "
-                            "sigma_ln 0.8) clipped to [8,256], lp U[10,1000] sigma, "
-                            "volume fraction 0.08, periodic cube, WCA, dt 1e-5",
-                "n_filaments": nf, "volume_fraction": 0.08, "box_side": 2 * R}
-    elif name == "c3":
-        p, fil, desc = workload("c2", sc)
-        b, lam = sc.generate_crosslinks(p, fil, 200000, r_bind=2.0, seed=13)
-        desc = dict(desc)
-        desc["workload"] = ("crosslinked network: the uneven-length system plus 200000 harmonic "
-                            "crosslinks (k 10 kT/sigma^2, rest length 1 sigma) between random "
-                            "bond pairs within 2 sigma at uniform positions, unbinding at rate "
-                            "0.1/tau with rebinding to a random in-range bond, dt 1e-5")
-        desc["crosslinks"] = 200000
-        desc["_xlinks"] = (b, lam)
-        return p, fil, desc
-    elif name == "c0":
+        p, fil, desc = W.dense()
+        return p, fil, desc, None, None
+    if name == "c2":
+        p, fil, desc = W.uneven()
+        return p, fil, desc, None, None
+    if name == "c3":
+        return W.crosslinked()
+    if name == "c0":
         p, spec = sc.params_from_yaml(os.path.join(ROOT, "simcore_amd", "configs",
                                                    "c0_plain_fp64.yaml"))
         nb, _ = sc.filament_layout(spec["length"], spec["min_bond_length"],
@@ -86,11 +60,10 @@ def workload(name, sc):
                                     spec["persistence_length"], spec["diameter"],
                                     spec["friction_ratio"], seed=7)
         desc = {"workload": "parity system: 64 filaments x 20 sites, lp 100 sigma, box 40 "
-                            "sigma, WCA, dt 1e-4", "n_filaments": 64, "sites_per_filament": 20}
-    else:
-        raise ValueError(name)
-    desc["bonds"] = int(fil.nb)
-    return p, fil, desc
+                            "sigma, WCA, dt 1e-4", "n_filaments": 64, "sites_per_filament": 20,
+                "bonds": int(fil.nb)}
+        return p, fil, desc, None, None
+    raise ValueError(name)
 
 
 def serving_requests(sc, n_requests, rate, seed=20240004):
@@ -99,6 +72,7 @@ def serving_requests(sc, n_requests, rate, seed=20240004):
     fraction 0.05, 2000 steps each, persistence length and noise seed drawn
     per request (lp in [10, 1000] sigma, hence dt 1e-5 as for the
     uneven-length system)."""
+    from simcore_amd.workloads import box_for_fraction
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
     out = []
@@ -301,7 +275,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--fp64-steps", type=int, default=200)
@@ -309,6 +283,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune-steps", type=int, default=300)
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the mixed-vs-FP64 force error measurement")
     ap.add_argument("--launch-shape", default="",
                     help="pair_block,pair_xcd,integrate_waves,integrate_layout"
                          "[,integrate_streams] instead of autotuning "
@@ -330,8 +307,9 @@ def main():
     ap.add_argument("--pmc-tag", default="dense620k_mixed")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(systems shard by ensemble member; see DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: there is no multi-GPU path here "
+              "(independent systems would shard by ensemble member, one large system "
+              "by bricks of cells with a halo exchange)", file=sys.stderr)
         sys.exit(2)
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, args.pmc_tag, args.copy_probe or (256 << 20))
@@ -346,11 +324,9 @@ def main():
         return
 
     log(f"[bench] building workload {args.workload}")
-    p, fil, desc = workload(args.workload, sc)
+    p, fil, desc, xl, xl_params = workload(args.workload, sc)
     log(f"[bench] {fil.nf} filaments, {fil.nb} bonds")
     spatial = args.precision == "mixed" and not args.caller_order
-    xl = desc.pop("_xlinks", None)
-    xl_params = dict(k_spring=10.0, rest_length=1.0, k_off=0.1, r_bind=2.0, seed=14)
     dev = sc.FilamentSystem(p, fil, precision=args.precision, use_graph=True,
                             spatial_order=spatial)
     if xl is not None:
@@ -367,7 +343,7 @@ def main():
     elif not args.no_autotune:
         # launch shapes that leave results unchanged, chosen from a
         # checkpoint of the warmed-up state (the state is restored)
-        tune = dev.autotune(steps=300)
+        tune = dev.autotune(steps=args.tune_steps)
         log(f"[bench] autotune: {tune}")
     c0 = dev.counters()
     wall = timed_steps(dev, args.steps)
@@ -376,77 +352,88 @@ def main():
     log(f"[bench] timed {args.steps} steps in {wall:.4f} s -> {value:.4e} segment-steps/s, "
         f"rebuilds {c1['n_rebuilds'] - c0['n_rebuilds']}")
 
-    # per-kernel HIP-event timing of the same kernels (eager launches)
+    # per-kernel HIP-event timing: the same steps launched eagerly with
+    # events right before and after each kernel family on the system's stream
     dev.profile(True)
     dev.step(args.profile_steps)
     prof = dev.profile_read()
     dev.profile(False)
     nc = dev.counters()["n_candidates"]
-    # the roofline's kernel time: the pair pass inside the steps above
-    # (HIP events on the system's stream right before and after it, so the
-    # figure includes the launch's dispatch and completion); the same pass
-    # repeated back to back on warm caches is reported beside it
     pair_ms = prof["pair"][0]
     pair_warm_ms = dev.time_pair(50)
-    bytes_pair = pair_bytes(args.precision, fil.nb, nc)
-    achieved = bytes_pair / (pair_ms * 1e-3) / 1e9 if pair_ms > 0 else 0.0
-    log(f"[bench] profile: {json.dumps(prof)} nc={nc}")
+    log(f"[bench] profile: {json.dumps(prof)} nc={nc} pair back-to-back {pair_warm_ms:.5f} ms")
     balanced = int(dev.launch_shape()["pair_block"]) == 0
     kname = ("k_pair_bal" if balanced else "k_pair_fused") + (
         "<true>" if args.precision == "mixed" else "<false>")
-    traffic = None
     ttag = f"{TRAFFIC_TAG[args.workload]}_{args.precision}" if args.workload in TRAFFIC_TAG else None
-    if ttag:
-        traffic = measured_traffic(ttag, kname)
-    roofline_pair = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "kernel": kname,
-                     "kernel_avg_ms": pair_ms, "kernel_avg_ms_back_to_back": pair_warm_ms,
-                     "algorithmic_bytes": bytes_pair,
-                     "algorithmic_bytes_fp32_state": pair_bytes_fp32_state(fil.nb, nc),
-                     "frac_fp32_state": round(pair_bytes_fp32_state(fil.nb, nc)
-                                              / (pair_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                     if pair_ms > 0 else 0.0,
-                     "candidates": nc}
-    # the dominant kernel of the step: the integrator (its launches of one
-    # step, HIP events around them on the system's stream)
+    # the pair pass: the step's largest single kernel on the uneven-length
+    # system and the kernel the HBM target names
+    bytes_pair = pair_bytes(args.precision, fil.nb, nc)
+    achieved = bytes_pair / (pair_ms * 1e-3) / 1e9 if pair_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(ttag, kname) if ttag else None,
+                "kernel": kname,
+                "kernel_avg_ms": pair_ms, "kernel_avg_ms_back_to_back": pair_warm_ms,
+                "algorithmic_bytes": bytes_pair,
+                "algorithmic_bytes_fp32_state": pair_bytes_fp32_state(fil.nb, nc),
+                "candidates": nc,
+                "per_step_ms": {k: v[0] for k, v in prof.items()}}
+    # the integrator: all its length-bin launches of one step
     int_ms = prof["integrate"][0]
     bytes_int = integrator_bytes(fil.ns, fil.nb)
     achieved_int = bytes_int / (int_ms * 1e-3) / 1e9 if int_ms > 0 else 0.0
     int_traffic = None
     if ttag and args.precision == "mixed":
-        # every length bin's launch of a full step, of a midstep; averaged
         tf = measured_traffic_sum(ttag, "k_integrate_long<", ", false>")
         tm = measured_traffic_sum(ttag, "k_integrate_long<", ", true>")
         if tf is not None and tm is not None:
             int_traffic = 0.5 * (tf + tm)
-    roofline = {"bound": "hbm", "achieved": round(achieved_int, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved_int / HBM_PEAK_GBS, 4),
-                "traffic": int_traffic,
-                "kernel": "integrator (k_integrate_long, all length bins of a step)",
-                "kernel_avg_ms": int_ms, "algorithmic_bytes": bytes_int,
-                "per_step_ms": {k: v[0] for k, v in prof.items()}}
+    roofline_int = {"bound": "hbm", "achieved": round(achieved_int, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved_int / HBM_PEAK_GBS, 4),
+                    "traffic": int_traffic,
+                    "kernel": "integrator (k_integrate_long, all length bins of a step)",
+                    "kernel_avg_ms": int_ms, "algorithmic_bytes": bytes_int}
+    r_end = dev.sites()[0]
     dev.close()
 
     extra = {}
+    if not args.no_accuracy and args.precision == "mixed" and xl is None:
+        # the metric's accuracy half: FP64 and mixed systems loaded from the
+        # same state (the positions the timed run ended at), one step each
+        from simcore_amd.accuracy import mixed_vs_fp64
+        fil_end = sc.Filaments(fil.site_off, r_end, fil.fil)
+        acc = mixed_vs_fp64(p, fil_end, steps=1, spatial_order=spatial)
+        extra["accuracy_vs_fp64"] = acc
+        log(f"[bench] accuracy: {json.dumps(acc)}")
     if not args.no_fp64_baseline and args.precision != "fp64":
-        d64 = sc.FilamentSystem(p, fil, precision="fp64", use_graph=False)
-        if xl is not None:
-            d64.load_crosslinks(xl[0], xl[1], **xl_params)
-        d64.step(10)
-        w64 = timed_steps(d64, args.fp64_steps)
-        d64.profile(True)
-        d64.step(args.profile_steps)
-        p64 = d64.profile_read()
-        d64.profile(False)
-        extra["plain_fp64_gpu"] = {"value": fil.nb * args.fp64_steps / w64,
-                                   "unit": "segment-steps/s", "steps": args.fp64_steps,
-                                   "note": "one system, FP64 everywhere (bit-compatible with "
-                                           "the CPU path), eager launches",
+        legs = {}
+        for graph in (False, True):
+            d64 = sc.FilamentSystem(p, fil, precision="fp64", use_graph=graph)
+            if xl is not None:
+                d64.load_crosslinks(xl[0], xl[1], **xl_params)
+            d64.step(10)
+            w64 = timed_steps(d64, args.fp64_steps)
+            legs[graph] = fil.nb * args.fp64_steps / w64
+            if not graph:
+                d64.profile(True)
+                d64.step(args.profile_steps)
+                p64 = d64.profile_read()
+                d64.profile(False)
+            d64.close()
+        extra["plain_fp64_gpu"] = {"value": legs[False], "unit": "segment-steps/s",
+                                   "steps": args.fp64_steps,
+                                   "note": "one system at a time, FP64 everywhere "
+                                           "(bit-identical to the CPU path), caller order, "
+                                           "eager launches",
                                    "per_step_ms": {k: v[0] for k, v in p64.items()}}
-        log(f"[bench] plain fp64: {extra['plain_fp64_gpu']['value']:.4e}")
-        d64.close()
+        extra["fp64_gpu_graph"] = {"value": legs[True], "unit": "segment-steps/s",
+                                   "steps": args.fp64_steps,
+                                   "note": "the same FP64 path replayed from hipGraphs"}
+        extra["vs_plain_fp64"] = value / legs[False]
+        extra["vs_fp64_graph"] = value / legs[True]
+        log(f"[bench] fp64 eager {legs[False]:.4e}, graph {legs[True]:.4e}; "
+            f"mixed/plain {value / legs[False]:.2f}x")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -463,12 +450,15 @@ def main():
         tcpu = time.perf_counter() - t0
         if rc != 0:
             raise RuntimeError(f"oracle step failed: {rc}")
+        host = host_cpu()
         cpu = {"value": fil.nb * args.cpu_steps / tcpu, "unit": "segment-steps/s",
                "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the same workload after 10 warm-up "
                          f"steps (incl. the initial cell-list build), {tcpu:.2f} s wall, "
-                         f"OpenMP static chunks as in the reference"}
-        log(f"[bench] cpu oracle: {cpu['value']:.4e} on {threads} threads")
+                         f"{threads} OpenMP threads, static chunks as in the reference",
+               "cpu_model": host["model"], "logical_cpus_available": host["logical"],
+               "physical_cores_available": host["physical"]}
+        log(f"[bench] cpu oracle: {cpu['value']:.4e} on {threads} threads ({host})")
         ora.close()
 
     out = {
@@ -480,10 +470,33 @@ def main():
         "data": "synthetic",
         "config": dict(desc, precision=args.precision, graph=True, spatial_order=spatial,
                        launch_shape=tune),
-        "roofline": roofline, "roofline_pair": roofline_pair, "cpu_baseline": cpu,
+        "roofline": roofline, "roofline_integrator": roofline_int, "cpu_baseline": cpu,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def host_cpu():
+    """CPU model and the logical / physical cores this process may run on."""
+    model, phys = "unknown", set()
+    allowed = os.sched_getaffinity(0)
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as fh:
+            for line in fh.read().splitlines() + [""]:
+                if not line.strip():
+                    if "processor" in cur and int(cur["processor"]) in allowed:
+                        phys.add((cur.get("physical id", "0"), cur.get("core id",
+                                                                        cur["processor"])))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if k.strip() == "model name":
+                    model = v.strip()
+    except OSError:
+        pass
+    return {"model": model, "logical": len(allowed), "physical": len(phys) or None}
 
 
 if __name__ == "__main__":

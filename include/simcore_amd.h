@@ -39,10 +39,11 @@ extern "C" {
 #define SC_PREC_FP64 0  /* everything FP64, bit-compatible with the CPU path */
 #define SC_PREC_MIXED 1 /* FP64 pair geometry with the hardware FP32
                            reciprocal / reciprocal square root, 32-byte bond
-                           records (FP32 offset inside the bond's cell, FP32
-                           orientation and length), FP32 sticky forces,
-                           per-bond sums and random forces, FP64 tensions
-                           and positions; at most 1024 cells per side        */
+                           records (32-bit fixed-point offset inside the
+                           bond's cell, octahedral unit vector, two-float
+                           length), FP32 sticky forces, per-bond sums and
+                           random forces, FP64 tensions and positions; at
+                           most 1024 cells per side                          */
 
 typedef struct sc_system *sc_handle;
 
@@ -72,7 +73,11 @@ typedef struct sc_params {
                               candidate list ORDER (hence FP64 summation
                               order) differs from the reference's          */
   uint64_t seed;           /* thermal-noise stream key                     */
-  double capacity_factor;  /* initial candidate capacity / bonds (0: 4)    */
+  double capacity_factor;  /* initial candidate capacity / bonds (0: 4);
+                              < 0: an absolute initial capacity of
+                              -capacity_factor candidates.  The capacity
+                              grows on overflow (the step is re-run, results
+                              unchanged), up to 2^30 - 1 candidates        */
 } sc_params;
 
 /* Per-filament parameters (Filament::SetParameters / SetDiffusion,
@@ -126,8 +131,9 @@ int sc_get_sites(sc_handle h, double *r, double *r_prev, double *f_rand);
 /* Bond state after Bond::ReInit (src/bond.cpp:22-37): centre, scaled
  * position, unit orientation, length.  Any pointer may be NULL.  Mixed
  * precision: the scaled position, orientation and length as the compact
- * record holds them (within 3e-8 of a cell, FP32); the centre is computed
- * from the FP64 sites. */
+ * record holds them (position within 2^-32 of a cell, orientation within
+ * about 1e-9, length to about 1e-14 relative); the centre is computed from
+ * the FP64 sites. */
 int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u,
                  double *len);
 /* Per-bond force/torque after ApplyPairInteractions of the last step
@@ -154,6 +160,20 @@ int sc_get_counters(sc_handle h, int64_t *i_step, int64_t *n_rebuilds,
 /* Overwrite site positions (host-side perturbation, e.g. to exercise the
  * unphysical-dr error path).  Bond state is re-initialised from them. */
 int sc_set_sites(sc_handle h, const double *r);
+
+/* Thermal noise.  The reference draws xi = RandomUniform() - 0.5 three
+ * times per site on every midstep from each filament's GSL generator
+ * (Filament::ConstructUnprojectedRandomForces, src/filament.cpp:518-548).
+ * SC_NOISE_COUNTER (the default): xi from a counter-based stream keyed by
+ * (seed, caller site index, midstep index); `seed` replaces the key, xi and
+ * n_xi are ignored.  SC_NOISE_BUFFER: the caller's pre-drawn values for the
+ * next n_xi / (3 * n_sites) midsteps, 3 per site (x, y, z), sites in the
+ * caller's order, midstep-major; stepping into a midstep the buffer does
+ * not hold returns SC_ERR_STATE.  The buffer is copied; it is not part of a
+ * checkpoint, and the autotuner's trials draw from the counter stream. */
+#define SC_NOISE_COUNTER 0
+#define SC_NOISE_BUFFER 1
+int sc_set_noise(sc_handle h, int mode, uint64_t seed, const double *xi, size_t n_xi);
 
 /* On-device analyses of the current state.  sc_filament_stats: per filament
  * the end-to-end vector head - tail (FilamentSpecies::RunMse2eAnalysis,
@@ -200,8 +220,12 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
  *                       bonds, incidences dealt evenly over its lanes)
  *   pair_xcd            1: consecutive bond ranges on one XCD
  *   integrate_waves     integrator waves per block, 1/2/4
- *   integrate_layout    filaments of up to 64 sites: one (0) or two (1,
- *                       default) sites per lane
+ *   integrate_layout    filaments of up to 64 sites: one (0) or two (1)
+ *                       sites per lane in lane groups of a fixed width per
+ *                       length bin; 2 (mixed-precision default): two sites
+ *                       per lane, every filament of up to 128 sites in a
+ *                       lane segment of its own width, packed into shared
+ *                       wavefronts (same results as 1)
  *   integrate_streams   0-3: the integrator's length bins are spread over the
  *                       system's stream and this many side streams (forked
  *                       and joined inside the step) and run concurrently */

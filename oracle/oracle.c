@@ -66,6 +66,10 @@ struct orc_sys {
   double *cf[3], *ct1[3], *ct2[3];
   /* state */
   int64_t rebuilds, i_step, n_midsteps;
+  /* caller-supplied noise: xi of midsteps [xi_m0, xi_m0 + xi_nm), 3 per
+   * site, site-major (NULL: the counter-based stream) */
+  double *xi_buf;
+  int64_t xi_m0, xi_nm;
   int i_update;
   int midstep;        /* FilamentSpecies::midstep_ (next update's flag) */
   int last_midstep;   /* Mesh::midstep_ (flag of the last update)       */
@@ -627,6 +631,7 @@ void orc_destroy(orc_sys *s) {
   free(s->sig_par);
   free(s->sig_perp);
   free(s->dr_tot);
+  free(s->xi_buf);
   free(s->norm_switch);
   free(s->n_normalize);
   for (int i = 0; i < 3; ++i) {
@@ -755,7 +760,13 @@ static void update_filament(orc_sys *s, int f, int midstep, double *wk) {
       const double sp_ = s->sig_perp[f], spa = s->sig_par[f];
       for (int i = 0; i < n; ++i) {
         double xi[3], xt[3];
-        orc_noise(s->p.seed, s0 + i, s->n_midsteps, xi);
+        if (s->xi_buf) {
+          const int64_t j = s->n_midsteps - s->xi_m0;
+          for (int d = 0; d < 3; ++d)
+            xi[d] = (j >= 0 && j < s->xi_nm) ? s->xi_buf[(j * s->ns + s0 + i) * 3 + d] : 0.0;
+        } else {
+          orc_noise(s->p.seed, s0 + i, s->n_midsteps, xi);
+        }
         double t0 = TN(i, 0), t1 = TN(i, 1), t2 = TN(i, 2);
         xt[0] = t0 * t0 * xi[0] + t0 * t1 * xi[1] + t0 * t2 * xi[2];
         xt[1] = t1 * t1 * xi[1] + t0 * t1 * xi[0] + t1 * t2 * xi[2];
@@ -1261,6 +1272,21 @@ void orc_get_dr(const orc_sys *s, double *acc, double *dr_max_per_fil) {
   if (acc) memcpy(acc, s->acc, sizeof(double) * (size_t)s->ns);
   if (dr_max_per_fil) memcpy(dr_max_per_fil, s->dr_tot, sizeof(double) * (size_t)s->nf);
 }
+/* Injects a pre-drawn noise stream: xi for the next n_mid midsteps, 3 per
+ * site in site order (the draws RandomUniform() - 0.5 of
+ * src/filament.cpp:527-547); n_mid = 0 returns to the counter stream. */
+void orc_set_noise_buffer(orc_sys *s, const double *xi, int64_t n_mid) {
+  free(s->xi_buf);
+  s->xi_buf = NULL;
+  s->xi_nm = 0;
+  if (n_mid <= 0 || !xi) return;
+  const size_t n = (size_t)n_mid * (size_t)s->ns * 3;
+  s->xi_buf = malloc(sizeof(double) * n);
+  memcpy(s->xi_buf, xi, sizeof(double) * n);
+  s->xi_m0 = s->n_midsteps;
+  s->xi_nm = n_mid;
+}
+
 void orc_set_sites(orc_sys *s, const double *r) {
   /* positions replaced from outside: bond state follows (Mesh::
    * UpdateBondPositions, src/mesh.cpp:324-334) */

@@ -50,6 +50,7 @@ def _load():
         "orc_get_tensions": ([V, dp], None),
         "orc_get_dr": ([V, dp, dp], None),
         "orc_set_sites": ([V, dp], None),
+        "orc_set_noise_buffer": ([V, dp, c_int64], None),
         "orc_min_distance": ([dp, dp, dp, c_double, dp, dp, c_double, dp], None),
         "orc_pair_force": ([c_double, c_double, c_double, c_double, dp, c_double, dp, dp, dp],
                            c_int),
@@ -179,6 +180,13 @@ class OracleSystem:
     def set_sites(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         lib.orc_set_sites(self.s, _dp(r))
+
+    def set_noise_buffer(self, xi, n_mid):
+        """Pre-drawn xi for the next n_mid midsteps (3 per site, site-major);
+        n_mid = 0: back to the counter stream."""
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        self._xi = xi
+        lib.orc_set_noise_buffer(self.s, _dp(xi), int(n_mid))
 
     def load_crosslinks(self, bonds, lam, k_spring=10.0, rest_length=1.0, k_compress=-1.0,
                         k_off=0.1, r_bind=2.0, seed=1):

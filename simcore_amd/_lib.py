@@ -20,13 +20,15 @@ SC_ERR_NOMEM = -5
 SC_ERR_PARSE = -6
 SC_PREC_FP64 = 0
 SC_PREC_MIXED = 1
+SC_NOISE_COUNTER = 0
+SC_NOISE_BUFFER = 1
 
 # every symbol include/simcore_amd.h declares
 EXPORTS = [
     "sc_abi_version", "sc_create", "sc_load_filaments", "sc_step",
     "sc_get_sites", "sc_get_bonds", "sc_get_bond_forces", "sc_get_tensions",
     "sc_get_cell_keys", "sc_get_candidates", "sc_get_slot_forces",
-    "sc_get_dr_state", "sc_get_counters", "sc_set_sites", "sc_device_bytes",
+    "sc_get_dr_state", "sc_get_counters", "sc_set_sites", "sc_set_noise", "sc_device_bytes",
     "sc_profile_enable", "sc_profile_read", "sc_last_error", "sc_destroy",
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
     "sc_generate_filaments", "sc_copy_probe", "sc_time_pair", "sc_step_async",
@@ -142,6 +144,7 @@ def _load():
         "sc_get_dr_state": ([H, dp], c_int),
         "sc_get_counters": ([H, lp, lp, lp, ip], c_int),
         "sc_set_sites": ([H, dp], c_int),
+        "sc_set_noise": ([H, c_int, c_uint64, dp, c_size_t], c_int),
         "sc_filament_stats": ([H, dp, dp], c_int),
         "sc_checkpoint_save": ([H, c_void_p, c_size_t, POINTER(c_size_t)], c_int),
         "sc_checkpoint_load": ([H, c_void_p, c_size_t], c_int),

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "sc_internal.h"
+#include "sc_math.h"
 #include "sc_system.h"
 
 using sc::Ctl;
@@ -43,6 +44,11 @@ struct sc_system {
   std::vector<int32_t> perm;
   std::vector<int64_t> site_map, bond_map;
   std::vector<Bin> bins;
+  // packed integrator layout (mixed precision): lane table and per-wave
+  // lane-move class of the filaments of at most 128 sites
+  int32_t *pack_tab = nullptr, *pack_code = nullptr;
+  int pack_waves = 0;
+  int64_t pack_sites = 0;
   sc::Tune tune;
   // graphs: [0] odd step (rebuild stage included), [1] even step (a rebuild
   // is only ever decided after a full step, i.e. at odd steps), [2] the
@@ -72,6 +78,8 @@ struct sc_system {
   sc::Arena *arena = nullptr;
   bool own_stream = true;
   int device = 0;  // HIP device current at creation
+  double *xi_dev = nullptr;  // caller-supplied noise (sc_set_noise)
+  double max_bond_length = 0.0;
   std::vector<void *> fixed_allocs;
   std::vector<void *> cand_allocs;
   size_t fixed_bytes = 0, cand_bytes = 0;
@@ -185,23 +193,46 @@ void free_cand(sc_system *h) {
   h->cand_bytes = 0;
 }
 
+// Candidates are numbered in int32 and each incidence entry holds a
+// candidate index shifted left by one bit.
+constexpr int64_t kMaxCandidates = ((int64_t)1 << 30) - 1;
+
+// Initial candidate capacity: capacity_factor * bonds + 4096 (4 when the
+// factor is 0); a negative factor is an absolute capacity of -factor
+// candidates (tests of capacity growth use it).
+int64_t initial_capacity(const sc_params &p, int64_t nb) {
+  if (p.capacity_factor < 0) return std::max<int64_t>(1, (int64_t)(-p.capacity_factor));
+  const double cf = p.capacity_factor > 0 ? p.capacity_factor : 4.0;
+  return std::min<int64_t>((int64_t)(cf * nb) + 4096, kMaxCandidates);
+}
+
 int alloc_cand(sc_system *h, int64_t cap) {
+  if (cap > kMaxCandidates)
+    return fail(h, SC_ERR_NOMEM, "more than 2^30 - 1 candidate pairs (" + std::to_string(cap) +
+                                     " asked for)");
   free_cand(h);
   drop_graph(h);
   Dev &d = h->d;
-  d.pairs = nullptr;
-  d.slot64 = nullptr;
-  d.slot32 = nullptr;
+  d.inc = nullptr;
+  d.islot64 = nullptr;
+  d.islot32 = nullptr;
   int rc;
-  if ((rc = dalloc(h, &d.pairs, cap, true))) return rc;
-  if (h->p.precision == SC_PREC_FP64) {
-    if ((rc = dalloc(h, &d.slot64, cap, true))) return rc;
-  } else {
-    if ((rc = dalloc(h, &d.slot32, cap, true))) return rc;
-  }
   if ((rc = dalloc(h, &d.inc, 2 * cap, true))) return rc;
+  if (h->p.precision == SC_PREC_FP64) {
+    if ((rc = dalloc(h, &d.islot64, 2 * cap, true))) return rc;
+  } else {
+    if ((rc = dalloc(h, &d.islot32, 2 * cap, true))) return rc;
+  }
   h->g.cap = cap;
   return SC_OK;
+}
+
+// Capacity after an overflow that asked for `need` candidates: doubled, or
+// need + 25% + 1024 if more, at most kMaxCandidates unless need itself is
+// larger (alloc_cand then refuses).
+int64_t grown_capacity(int64_t cap, int64_t need) {
+  const int64_t want = std::max<int64_t>(2 * cap, need + need / 4 + 1024);
+  return need > kMaxCandidates ? need : std::min(want, kMaxCandidates);
 }
 
 // Control-block transfers, ordered on the system's stream.
@@ -216,6 +247,28 @@ int write_ctl(sc_system *h, const Ctl &c) {
   HIPCHK(h, hipMemcpyAsync(h->d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return SC_OK;
+}
+
+// The side streams and join events of the integrator's length bins, made
+// the first time a launch shape asks for them (most systems never do, and
+// an ensemble admits many systems).
+int ensure_side_streams(sc_system *h) {
+  if (h->side[0]) return SC_OK;
+  for (int k = 0; k < sc_system::kSide; ++k)
+    if (hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->join_ev[k], hipEventDisableTiming) != hipSuccess)
+      return fail(h, SC_ERR_HIP, "side stream creation failed");
+  return SC_OK;
+}
+
+// Whether the midsteps among steps from+1 .. from+n (midstep index
+// (k - 1) / 2 of odd step k) all lie in the caller's noise buffer.
+bool midsteps_in_buffer(const sc_system *h, int64_t from, int64_t n) {
+  int64_t k0 = from + 1, k1 = from + n;
+  if (!(k0 & 1)) ++k0;
+  if (!(k1 & 1)) --k1;
+  if (k1 < k0) return true;  // no midstep
+  return (k0 - 1) / 2 >= h->g.xi_m0 && (k1 - 1) / 2 < h->g.xi_m0 + h->g.xi_nm;
 }
 
 // One step.  `odd` says whether the step index is odd: only odd steps can
@@ -241,41 +294,55 @@ void enqueue_step(sc_system *h, bool odd) {
     (void)hipEventRecord(h->ev[1], st);
     (void)hipEventRecord(h->ev[4], st);
   }
-  sc::BinDesc bins[sc::kMaxBins];
+  // the integrator's launches: one per length bin, or, in the packed layout,
+  // one for every filament of at most 128 sites (maxn 0 below) plus the
+  // bin of longer ones
+  sc::BinDesc bins[sc::kMaxBins + 1];
   int nbins = 0;
-  for (const Bin &b : h->bins) bins[nbins++] = {b.list, b.count, b.maxn};
+  const bool packed = h->tune.int_layout == 2 && h->pack_waves > 0;
+  if (packed) bins[nbins++] = {nullptr, h->pack_waves, 0};
+  for (const Bin &b : h->bins)
+    if (!packed || b.maxn > 128) bins[nbins++] = {b.list, b.count, b.maxn};
+  auto load_of = [&](int k) -> int64_t {
+    return bins[k].maxn ? (int64_t)bins[k].count * bins[k].maxn : h->pack_sites;
+  };
+  auto launch_bin = [&](int k, hipStream_t sk) {
+    if (bins[k].maxn == 0)
+      sc::launch_integrate_packed(d, g, h->pack_tab, h->pack_code, h->pack_waves,
+                                  h->tune.int_waves, odd, sk);
+    else
+      sc::launch_integrate(d, g, bins[k].list, bins[k].count, bins[k].maxn, h->p.precision,
+                           h->tune, odd, sk);
+  };
   if (h->tune.int_streams && nbins > 1 && h->side[0]) {
     // bins are independent (disjoint filaments; the displacement maxima
     // combine by max): deal them to the main and side streams, largest
     // first onto the least loaded, then join
     const int lanes = std::min(nbins, 1 + std::min(h->tune.int_streams, sc_system::kSide));
-    int order[sc::kMaxBins];
+    int order[sc::kMaxBins + 1];
     for (int k = 0; k < nbins; ++k) order[k] = k;
-    std::sort(order, order + nbins, [&](int a, int b) {
-      return (int64_t)bins[a].count * bins[a].maxn > (int64_t)bins[b].count * bins[b].maxn;
-    });
+    std::sort(order, order + nbins, [&](int a, int b) { return load_of(a) > load_of(b); });
     int64_t load[1 + sc_system::kSide] = {};
-    int lane_of[sc::kMaxBins];
+    int lane_of[sc::kMaxBins + 1];
     for (int j = 0; j < nbins; ++j) {
       int best = 0;
       for (int l = 1; l < lanes; ++l)
         if (load[l] < load[best]) best = l;
       lane_of[order[j]] = best;
-      load[best] += (int64_t)bins[order[j]].count * bins[order[j]].maxn;
+      load[best] += load_of(order[j]);
     }
     (void)hipEventRecord(h->fork_ev, st);
     for (int l = 1; l < lanes; ++l) (void)hipStreamWaitEvent(h->side[l - 1], h->fork_ev, 0);
     for (int k = 0; k < nbins; ++k) {
       hipStream_t sk = lane_of[k] == 0 ? st : h->side[lane_of[k] - 1];
-      sc::launch_integrate(d, g, bins[k].list, bins[k].count, bins[k].maxn, h->p.precision,
-                           h->tune, odd, sk);
+      launch_bin(k, sk);
     }
     for (int l = 1; l < lanes; ++l) {
       (void)hipEventRecord(h->join_ev[l - 1], h->side[l - 1]);
       (void)hipStreamWaitEvent(st, h->join_ev[l - 1], 0);
     }
   } else {
-    sc::launch_integrate_all(d, g, bins, nbins, h->p.precision, h->tune, odd, st);
+    for (int k = 0; k < nbins; ++k) launch_bin(k, st);
   }
   if (pf) {
     (void)hipEventRecord(h->ev[5], st);
@@ -314,8 +381,7 @@ int initial_rebuild(sc_system *h) {
     HIPCHK(h, hipGetLastError());
     c = read_ctl(h);
     if (c.overflow) {
-      int64_t ncap = std::max<int64_t>(2 * h->g.cap, c.nc_need + c.nc_need / 4 + 1024);
-      if ((rc = alloc_cand(h, ncap))) return rc;
+      if ((rc = alloc_cand(h, grown_capacity(h->g.cap, c.nc_need)))) return rc;
       continue;
     }
     c.rebuild = 0;
@@ -363,7 +429,7 @@ int fetch_mapped(sc_system *h, const double *dev, double *out, int64_t n, int co
 // are contiguous, so one bin's launch does not fetch cache lines that
 // mostly hold another bin's sites, while a bond's partners stay in its block.
 std::vector<int32_t> spatial_permutation(const sc_params &p, int32_t nf, const int32_t *so,
-                                         const double *r) {
+                                         const double *r, std::vector<int32_t> *block) {
   const int64_t ns = so[nf];
   std::vector<uint64_t> key(nf);
   for (int f = 0; f < nf; ++f) {
@@ -386,7 +452,125 @@ std::vector<int32_t> spatial_permutation(const sc_params &p, int32_t nf, const i
   std::vector<int32_t> perm(nf);
   for (int f = 0; f < nf; ++f) perm[f] = f;
   std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  if (block) {
+    block->resize(nf);
+    for (int j = 0; j < nf; ++j) (*block)[j] = (int32_t)(key[perm[j]] >> 33);
+  }
   return perm;
+}
+
+// Packed integrator layout (k_integrate_packed): each filament of at most
+// 128 sites takes w = ceil(n/2) consecutive lanes of a wavefront.  A segment
+// of 33-64 lanes starts a wave at lane 0; one of 17-32 lanes starts at lane
+// 0 or 32 with both of that half's 16-lane rows otherwise empty; one of at
+// most 16 lanes lies inside a single row (best fit over the rows' free
+// tails, widest filaments first).  The lane moves of a wave are those of
+// its widest class (16-, 32- or 64-lane groups), valid for every segment
+// in it because no segment crosses a group boundary of its class.
+struct PackWave {
+  int code = 0;                   // 0: 16-lane rows, 1: 32-lane halves, 2: 64
+  int32_t lane[64];               // filament (caller-side id) per lane, -1 idle
+  int gl[64];                     // lane inside the segment
+  int row_used[4] = {0, 0, 0, 0};  // lanes used from the start of each row
+  PackWave() {
+    for (int l = 0; l < 64; ++l) {
+      lane[l] = -1;
+      gl[l] = 0;
+    }
+  }
+};
+
+void pack_place(PackWave &w, int32_t id, int start, int width) {
+  for (int k = 0; k < width; ++k) {
+    w.lane[start + k] = id;
+    w.gl[start + k] = k;
+  }
+  for (int r = 0; r < 4; ++r) {
+    const int lo = std::max(start, 16 * r), hi = std::min(start + width, 16 * r + 16);
+    if (hi > lo) w.row_used[r] = std::max(w.row_used[r], hi - 16 * r);
+  }
+}
+
+// Packs `ids` (width = lanes each, 1..64) into waves appended to `out`.
+void pack_group(const std::vector<int32_t> &ids, const std::vector<int> &width,
+                std::vector<PackWave> &out) {
+  std::vector<int32_t> order(ids);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return width[a] > width[b]; });
+  const size_t first = out.size();
+  // rows with free lanes, by free count (best fit for the narrow segments)
+  std::vector<std::vector<std::pair<size_t, int>>> rows_by_free(17);
+  std::vector<std::pair<size_t, int>> free_halves;  // (wave, half)
+  auto add_rows = [&](size_t wi) {
+    const PackWave &w = out[wi];
+    for (int r = 0; r < 4; ++r) {
+      const int fr = 16 - w.row_used[r];
+      if (fr > 0 && fr < 16) rows_by_free[fr].push_back({wi, r});
+    }
+  };
+  for (int32_t id : order) {
+    const int wd = width[id];
+    if (wd > 32) {
+      out.emplace_back();
+      out.back().code = 2;
+      pack_place(out.back(), id, 0, wd);
+      const size_t wi = out.size() - 1;
+      add_rows(wi);
+      for (int r = (wd + 15) / 16; r < 4; ++r) rows_by_free[16].push_back({wi, r});
+    } else if (wd > 16) {
+      size_t wi;
+      int half;
+      if (!free_halves.empty()) {
+        wi = free_halves.back().first;
+        half = free_halves.back().second;
+        free_halves.pop_back();
+      } else {
+        out.emplace_back();
+        wi = out.size() - 1;
+        half = 0;
+        free_halves.push_back({wi, 1});
+      }
+      PackWave &w = out[wi];
+      w.code = std::max(w.code, 1);
+      pack_place(w, id, 32 * half, wd);
+      const int r = 2 * half + 1;  // the half's second row, partly used
+      const int fr = 16 - w.row_used[r];
+      if (fr > 0) rows_by_free[fr].push_back({wi, r});
+    } else {
+      // best fit: the row with the fewest free lanes that still holds wd
+      size_t wi = 0;
+      int r = -1;
+      for (int fr = wd; fr <= 16 && r < 0; ++fr) {
+        while (!rows_by_free[fr].empty()) {
+          auto cand = rows_by_free[fr].back();
+          rows_by_free[fr].pop_back();
+          if (16 - out[cand.first].row_used[cand.second] != fr) continue;  // stale
+          wi = cand.first;
+          r = cand.second;
+          break;
+        }
+      }
+      if (r < 0 && !free_halves.empty()) {
+        // an untouched half of a 32-lane wave
+        wi = free_halves.back().first;
+        const int half = free_halves.back().second;
+        free_halves.pop_back();
+        r = 2 * half;
+        rows_by_free[16].push_back({wi, 2 * half + 1});
+      }
+      if (r < 0) {
+        out.emplace_back();
+        wi = out.size() - 1;
+        r = 0;
+        for (int q = 1; q < 4; ++q) rows_by_free[16].push_back({wi, q});
+      }
+      PackWave &w = out[wi];
+      pack_place(w, id, 16 * r + w.row_used[r], wd);
+      const int fr = 16 - w.row_used[r];
+      if (fr > 0) rows_by_free[fr].push_back({wi, r});
+    }
+  }
+  (void)first;
 }
 
 // Enqueues `count` steps after step index `from` (the next step is
@@ -509,12 +693,6 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
     sc_destroy(h);
     return SC_ERR_HIP;
   }
-  for (int k = 0; k < sc_system::kSide; ++k)
-    if (hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&h->join_ev[k], hipEventDisableTiming) != hipSuccess) {
-      sc_destroy(h);
-      return SC_ERR_HIP;
-    }
   h->ctl_host = static_cast<Ctl *>(pin);
   *h->ctl_host = Ctl{};
   *out = h;
@@ -533,20 +711,100 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   const int64_t rec_bytes = params->precision == SC_PREC_MIXED ? 32 : 64;
   b += R(rec_bytes * nb) + 2 * R(24 * nb) + R(8 * nb);    // record, F, T (at most), tension
   b += R(4 * nb) + R(4 * n_cells) + R(4 * (n_cells + 1));  // keys, cell count/start
-  b += 3 * R(4 * nb) + R(4 * nb) + R(4 * (nb + 1));  // sorted, skey, ccount, deg, inc_start
-  b += R(4 * nb) + R(4 * ((nb + sc::kScanBlock - 1) / sc::kScanBlock + 1));  // sorted_o, tsum
+  b += 2 * R(4 * nb) + R(4 * nb) + R(4 * (nb + 1));  // sorted, skey, deg, inc_start
+  b += R(4 * nb);                                      // sorted_o
   b += R((n_cells + 7) / 8 + 8);                                               // occupancy
   const int64_t tmp = std::max(scan_tmp_elems(n_cells), scan_tmp_elems(nb));
   b += R(4 * tmp);
   b += R(4 * (int64_t)nf) + (size_t)n_bins * Arena::kAlign;  // bin lists
-  const double cf = params->capacity_factor > 0 ? params->capacity_factor : 4.0;
-  const int64_t cap = (int64_t)(cf * nb) + 4096;
+  if (params->precision == SC_PREC_MIXED)  // packed layout: at most one wave per filament
+    b += R(4 * 64 * (int64_t)nf) + R(4 * (int64_t)nf);
+  const int64_t cap = initial_capacity(*params, nb);
   const int64_t slot = params->precision == SC_PREC_FP64 ? 32 : 16;
-  b += R(8 * cap) + R(slot * cap) + R(16 * cap);
+  b += R(16 * cap) + R(2 * slot * cap);
   return b;
 }
 
 }  // namespace sc
+
+namespace {
+
+// q-th cell of the half shell of cell k (CellList::AssignCellNeighbors
+// order, src/cell_list.cpp:206-262), host side.
+int32_t host_shell_key(int n, int32_t k, int q) {
+  const int x = k / (n * n), y = (k / n) % n, z = k % n;
+  int dx = 1, dy = 0, dz = 0;
+  if (q >= 1 && q < 4) {
+    dx = q - 2;
+    dy = 1;
+  } else if (q >= 4) {
+    dx = (q - 4) % 3 - 1;
+    dy = (q - 4) / 3 - 1;
+    dz = 1;
+  }
+  const int xp = (x + dx + n) % n, yp = (y + dy + n) % n, zp = (z + dz) % n;
+  return ((int32_t)xp * n + yp) * n + zp;
+}
+
+struct HostCand {
+  int32_t key, phase, rank1, rank2, b1, b2;
+  int64_t q;  // the role-0 incidence (the candidate's slot copy)
+};
+
+// The candidate list in the reference's order, recovered from the
+// incidence lists: each role-0 incidence of bond b with partner p is the
+// candidate (b, p), and CellList::MakePairs (src/cell_list.cpp:127-137,
+// src/cell.cpp:14-70) orders candidates by the cell key of obj1, then phase
+// (the cell's self pairs, then its 13 half-shell cells in stencil order),
+// then obj1's rank in its cell, then obj2's rank in its cell, ranks in
+// storage order.  Storage indices.
+int host_candidates(sc_system *h, std::vector<HostCand> *out, std::vector<int2> *inc_out) {
+  const int64_t nb = h->g.nb;
+  const int n = h->g.n1d;
+  const int64_t ni = 2 * read_ctl(h).nc;
+  std::vector<int32_t> keys(nb), start(nb + 1);
+  std::vector<int2> inc(ni);
+  HIPCHK(h, hipMemcpy(keys.data(), h->d.keys, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(start.data(), h->d.inc_start, sizeof(int32_t) * (nb + 1),
+                      hipMemcpyDeviceToHost));
+  if (ni) HIPCHK(h, hipMemcpy(inc.data(), h->d.inc, sizeof(int2) * ni, hipMemcpyDeviceToHost));
+  if (start[nb] != ni) return fail(h, SC_ERR_STATE, "incidence lists and list length disagree");
+  std::vector<int32_t> order(nb), rank(nb);
+  for (int64_t b = 0; b < nb; ++b) order[b] = (int32_t)b;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
+  for (int64_t j = 0; j < nb; ++j)
+    rank[order[j]] = (j > 0 && keys[order[j - 1]] == keys[order[j]]) ? rank[order[j - 1]] + 1 : 0;
+  out->clear();
+  out->reserve(ni / 2);
+  for (int64_t b = 0; b < nb; ++b) {
+    int32_t sh[13];
+    for (int q = 0; q < 13; ++q) sh[q] = host_shell_key(n, keys[b], q);
+    for (int64_t q = start[b]; q < start[b + 1]; ++q) {
+      if (inc[q].y & 1) continue;
+      const int32_t p = inc[q].x;
+      int phase = -1;
+      if (keys[p] == keys[b]) {
+        phase = 0;
+      } else {
+        for (int k = 0; k < 13; ++k)
+          if (sh[k] == keys[p]) phase = 1 + k;
+      }
+      if (phase < 0) return fail(h, SC_ERR_STATE, "incidence outside the half shell");
+      out->push_back({keys[b], phase, rank[b], rank[p], (int32_t)b, p, q});
+    }
+  }
+  std::sort(out->begin(), out->end(), [](const HostCand &a, const HostCand &b) {
+    if (a.key != b.key) return a.key < b.key;
+    if (a.phase != b.phase) return a.phase < b.phase;
+    if (a.rank1 != b.rank1) return a.rank1 < b.rank1;
+    return a.rank2 < b.rank2;
+  });
+  if (inc_out) inc_out->swap(inc);
+  return SC_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -573,9 +831,40 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   std::vector<int32_t> so_st, noise_off(nf);
   std::vector<double> r_st;
   std::vector<sc_filament_params> fil_st;
+  // packed integrator layout (mixed precision, filaments of <= 128 sites):
+  // groups of filaments packed together -- the Morton blocks in spatial
+  // order, everything in caller order -- and, in spatial order, the
+  // storage order follows the packing so each wave reads contiguous sites
+  const bool packing = h->p.precision == SC_PREC_MIXED && nf < (1 << 23);
+  std::vector<PackWave> waves;
+  std::vector<int> width(packing ? nf : 0);
+  for (int f = 0; packing && f < nf; ++f) {
+    const int n = site_off[f + 1] - site_off[f];
+    width[f] = n <= 128 ? (n + 1) / 2 : 0;
+  }
   if (h->p.spatial_order) {
     const int64_t ns_ = g.ns, nb_ = g.nb;
-    h->perm = spatial_permutation(h->p, nf, site_off, r);
+    std::vector<int32_t> block;
+    h->perm = spatial_permutation(h->p, nf, site_off, r, &block);
+    if (packing) {
+      std::vector<int32_t> order;
+      order.reserve(nf);
+      for (int j0 = 0; j0 < nf;) {
+        int j1 = j0;
+        while (j1 < nf && block[j1] == block[j0]) ++j1;
+        std::vector<int32_t> ids, longs;
+        for (int j = j0; j < j1; ++j)
+          (width[h->perm[j]] ? ids : longs).push_back(h->perm[j]);
+        const size_t w0 = waves.size();
+        pack_group(ids, width, waves);
+        for (size_t wi = w0; wi < waves.size(); ++wi)
+          for (int l = 0; l < 64; ++l)
+            if (waves[wi].lane[l] >= 0 && waves[wi].gl[l] == 0) order.push_back(waves[wi].lane[l]);
+        order.insert(order.end(), longs.begin(), longs.end());
+        j0 = j1;
+      }
+      h->perm = order;
+    }
     h->reordered = true;
     so_st.assign(nf + 1, 0);
     for (int f = 0; f < nf; ++f) {
@@ -602,6 +891,12 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     fil = fil_st.data();
   } else {
     for (int f = 0; f < nf; ++f) noise_off[f] = site_off[f];
+    if (packing) {
+      std::vector<int32_t> ids;
+      for (int f = 0; f < nf; ++f)
+        if (width[f]) ids.push_back(f);
+      pack_group(ids, width, waves);
+    }
   }
   h->site_off.assign(site_off, site_off + nf + 1);
   const int64_t ns = g.ns, nb = g.nb;
@@ -635,9 +930,6 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = dalloc(h, &d.skey, nb, false))) return rc;
   if ((rc = dalloc(h, &d.sorted_o, nb, false))) return rc;
   if ((rc = dalloc(h, &d.cell_occ, (size_t)(g.n_cells + 7) / 8 + 8, false))) return rc;
-  if ((rc = dalloc(h, &d.ccount, nb, false))) return rc;
-  if ((rc = dalloc(h, &d.tsum, (nb + sc::kScanBlock - 1) / sc::kScanBlock + 1, false)))
-    return rc;
   if ((rc = dalloc(h, &d.deg, nb, false))) return rc;
   if ((rc = dalloc(h, &d.inc_start, nb + 1, false))) return rc;
   int64_t tmp = std::max(sc::scan_tmp_elems(g.n_cells), sc::scan_tmp_elems(nb));
@@ -651,6 +943,7 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     const sc_filament_params &q = fil[f];
     double *fp = &fpar[8 * (size_t)f];
     fp[0] = q.bond_length;
+    h->max_bond_length = std::max(h->max_bond_length, q.bond_length);
     fp[1] = q.persistence_length;
     fp[2] = q.gamma_par;
     fp[3] = q.gamma_perp;
@@ -711,9 +1004,33 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
                              hipMemcpyHostToDevice, h->stream));
     h->bins.push_back(b);
   }
-  double cf = h->p.capacity_factor > 0 ? h->p.capacity_factor : 4.0;
-  int64_t cap = (int64_t)(cf * nb) + 4096;
-  if ((rc = alloc_cand(h, cap))) return rc;
+  // (host arrays of the packed layout live until the copies below have run)
+  std::vector<int32_t> tab, code;
+  if (packing && !waves.empty()) {
+    // caller-side ids -> storage indices
+    std::vector<int32_t> st_of(nf);
+    for (int f = 0; f < nf; ++f) st_of[h->reordered ? h->perm[f] : f] = f;
+    const size_t nw = waves.size();
+    tab.assign(64 * nw, -1);
+    code.assign(nw, 0);
+    for (size_t wi = 0; wi < nw; ++wi) {
+      code[wi] = waves[wi].code;
+      for (int l = 0; l < 64; ++l) {
+        const int32_t id = waves[wi].lane[l];
+        tab[64 * wi + l] = id < 0 ? -1 : ((st_of[id] << 8) | (waves[wi].code << 6) | waves[wi].gl[l]);
+        if (id >= 0 && waves[wi].gl[l] == 0) h->pack_sites += site_off[st_of[id] + 1] - site_off[st_of[id]];
+      }
+    }
+    if ((rc = dalloc(h, &h->pack_tab, tab.size(), false))) return rc;
+    if ((rc = dalloc(h, &h->pack_code, nw, false))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->pack_tab, tab.data(), sizeof(int32_t) * tab.size(),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->pack_code, code.data(), sizeof(int32_t) * nw,
+                             hipMemcpyHostToDevice, h->stream));
+    h->pack_waves = (int)nw;
+    h->tune.int_layout = 2;
+  }
+  if ((rc = alloc_cand(h, initial_capacity(h->p, nb)))) return rc;
   sc::launch_bond_init(d, g, h->p.precision, h->stream);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipGetLastError());
@@ -776,12 +1093,7 @@ int sc_load_crosslinks(sc_handle h, int64_t n, const int32_t *bonds, const doubl
   g.xpoff = p->k_off > 0 ? 1.0 - std::exp(-p->k_off * h->p.delta) : 0.0;
   g.xrb2 = p->r_bind * p->r_bind;
   g.xseed = p->seed;
-  double lmax = 0.0;
-  for (int f = 0; f < g.nf; ++f) {
-    double bl = 0;
-    HIPCHK(h, hipMemcpy(&bl, d.fpar + 8 * f, sizeof(double), hipMemcpyDeviceToHost));
-    lmax = std::max(lmax, bl);
-  }
+  const double lmax = h->max_bond_length;
   const double cell_len = 2.0 * h->p.system_radius / g.n1d;
   g.xreach = (int)std::ceil((p->r_bind + 0.5 * lmax) / cell_len) + 1;
   // build the per-bond lists once
@@ -840,6 +1152,9 @@ int sc_step_async(sc_handle h, int32_t n_steps) {
   }
   if (n_steps == 0) return SC_OK;
   const int64_t from = h->target;
+  if (h->g.xi_buf && !midsteps_in_buffer(h, from, n_steps))
+    return fail(h, SC_ERR_STATE,
+                "the caller's noise buffer does not hold these midsteps (sc_set_noise)");
   h->target += n_steps;
   return enqueue_range(h, from, n_steps);
 }
@@ -869,8 +1184,7 @@ int sc_step_poll(sc_handle h, int *done) {
     // the capacity check returned): grow the candidate buffers and run the
     // remaining steps again from there
     ++h->grow_rounds;
-    int64_t ncap = std::max<int64_t>(2 * h->g.cap, c.nc_need + c.nc_need / 4 + 1024);
-    if ((rc = alloc_cand(h, ncap))) {
+    if ((rc = alloc_cand(h, grown_capacity(h->g.cap, c.nc_need)))) {
       h->pending = false;
       return rc;
     }
@@ -916,20 +1230,25 @@ int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u, double *len)
   const int64_t nb = h->g.nb, ns = h->g.ns;
   std::vector<double> sp(3 * (size_t)nb), uu(3 * (size_t)nb), ln(nb);
   if (h->g.compact) {
-    // mixed precision: the 32-byte record {cell offset x, y, z, length},
-    // {unit vector, packed cell coordinates}; scaled position rebuilt as the
-    // kernels rebuild it
-    std::vector<float> rec(8 * (size_t)nb);
-    HIPCHK(h, hipMemcpy(rec.data(), h->d.br, sizeof(float) * rec.size(), hipMemcpyDeviceToHost));
+    // mixed precision: the 32-byte record {cell offset x, y, z (2^-32
+    // cell), length high}, {octahedral unit vector, packed cell
+    // coordinates, length low}, decoded as the kernels decode it
+    std::vector<uint32_t> rec(8 * (size_t)nb);
+    HIPCHK(h, hipMemcpy(rec.data(), h->d.br, sizeof(uint32_t) * rec.size(), hipMemcpyDeviceToHost));
     for (int64_t b = 0; b < nb; ++b) {
-      int32_t packed;
-      std::memcpy(&packed, &rec[8 * b + 7], sizeof(packed));
+      const uint32_t *w = &rec[8 * b];
+      const int packed = (int)w[6];
       const int c[3] = {packed & 1023, (packed >> 10) & 1023, (packed >> 20) & 1023};
+      double u[3];
+      sc::oct_decode(w[4], w[5], u);
       for (int k = 0; k < 3; ++k) {
-        sp[k * nb + b] = ((double)c[k] + (double)rec[8 * b + k]) * h->g.inv_n1d - 0.5;
-        uu[k * nb + b] = rec[8 * b + 4 + k];
+        sp[k * nb + b] = ((double)c[k] + (double)w[k] * (1.0 / 4294967296.0)) * h->g.inv_n1d - 0.5;
+        uu[k * nb + b] = u[k];
       }
-      ln[b] = rec[8 * b + 3];
+      float lh, ll;
+      std::memcpy(&lh, &w[3], 4);
+      std::memcpy(&ll, &w[7], 4);
+      ln[b] = (double)lh + (double)ll;
     }
   } else {
     std::vector<double> rec(8 * (size_t)nb);
@@ -1014,12 +1333,11 @@ int sc_get_candidates(sc_handle h, int32_t *b1, int32_t *b2, size_t cap, size_t 
   if (n) *n = (size_t)c.nc;
   if (!b1 && !b2) return SC_OK;
   if (cap < (size_t)c.nc) return fail(h, SC_ERR_ARG, "output capacity too small");
-  std::vector<int2> pr((size_t)c.nc);
-  if (c.nc)
-    HIPCHK(h, hipMemcpy(pr.data(), h->d.pairs, sizeof(int2) * c.nc, hipMemcpyDeviceToHost));
-  for (int64_t i = 0; i < c.nc; ++i) {
-    const int64_t x = h->reordered ? h->bond_map[pr[i].x] : pr[i].x;
-    const int64_t y = h->reordered ? h->bond_map[pr[i].y] : pr[i].y;
+  std::vector<HostCand> cand;
+  if ((rc = host_candidates(h, &cand, nullptr))) return rc;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    const int64_t x = h->reordered ? h->bond_map[cand[i].b1] : cand[i].b1;
+    const int64_t y = h->reordered ? h->bond_map[cand[i].b2] : cand[i].b2;
     if (b1) b1[i] = (int32_t)x;
     if (b2) b2[i] = (int32_t)y;
   }
@@ -1032,23 +1350,35 @@ int sc_get_slot_forces(sc_handle h, double *f, size_t cap) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   Ctl c = read_ctl(h);
   if (!f || cap < (size_t)c.nc) return fail(h, SC_ERR_ARG, "output capacity too small");
-  const int64_t nc = c.nc;
+  const int64_t nc = c.nc, ni = 2 * nc;
+  std::vector<HostCand> cand;
+  std::vector<int2> inc;
+  if ((rc = host_candidates(h, &cand, &inc))) return rc;
+  // a candidate's slot: the copy under its obj1 (both copies are equal)
+  std::vector<double> v(3 * (size_t)ni, 0.0);
   if (h->p.precision == SC_PREC_FP64) {
-    std::vector<double4> v(nc);
-    if (nc) HIPCHK(h, hipMemcpy(v.data(), h->d.slot64, sizeof(double4) * nc, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < nc; ++i) {
-      f[i] = v[i].x;
-      f[nc + i] = v[i].y;
-      f[2 * nc + i] = v[i].z;
+    std::vector<double4> s(ni);
+    if (ni) HIPCHK(h, hipMemcpy(s.data(), h->d.islot64, sizeof(double4) * ni, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < ni; ++q) {
+      v[3 * q] = s[q].x;
+      v[3 * q + 1] = s[q].y;
+      v[3 * q + 2] = s[q].z;
     }
   } else {
-    std::vector<float4> v(nc);
-    if (nc) HIPCHK(h, hipMemcpy(v.data(), h->d.slot32, sizeof(float4) * nc, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < nc; ++i) {
-      f[i] = v[i].x;
-      f[nc + i] = v[i].y;
-      f[2 * nc + i] = v[i].z;
+    std::vector<float4> s(ni);
+    if (ni) HIPCHK(h, hipMemcpy(s.data(), h->d.islot32, sizeof(float4) * ni, hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < ni; ++q) {
+      v[3 * q] = s[q].x;
+      v[3 * q + 1] = s[q].y;
+      v[3 * q + 2] = s[q].z;
     }
+  }
+  for (int64_t k = 0; k < nc; ++k) {
+    const int64_t q = cand[k].q;
+    const bool written = (inc[q].y & 2) != 0;  // not written since the rebuild: 0
+    f[k] = written ? v[3 * q] : 0.0;
+    f[nc + k] = written ? v[3 * q + 1] : 0.0;
+    f[2 * nc + k] = written ? v[3 * q + 2] : 0.0;
   }
   return SC_OK;
 }
@@ -1102,6 +1432,39 @@ int sc_set_sites(sc_handle h, const double *r) {
   sc::launch_decide(h->d, h->g, h->stream);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipGetLastError());
+  return SC_OK;
+}
+
+int sc_set_noise(sc_handle h, int mode, uint64_t seed, const double *xi, size_t n_xi) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
+  if (mode != SC_NOISE_COUNTER && mode != SC_NOISE_BUFFER)
+    return fail(h, SC_ERR_ARG, "unknown noise mode");
+  const size_t per_mid = 3 * (size_t)h->g.ns;
+  if (mode == SC_NOISE_BUFFER && (!xi || n_xi == 0 || n_xi % per_mid != 0))
+    return fail(h, SC_ERR_ARG, "noise buffer: a positive multiple of 3 * n_sites values");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->xi_dev) {
+    (void)hipFree(h->xi_dev);
+    h->xi_dev = nullptr;
+  }
+  h->g.xi_buf = nullptr;
+  h->g.xi_m0 = 0;
+  h->g.xi_nm = 0;
+  if (mode == SC_NOISE_COUNTER) {
+    h->g.seed = seed;
+  } else {
+    double *buf = nullptr;
+    if (hipMalloc(&buf, sizeof(double) * n_xi) != hipSuccess)
+      return fail(h, SC_ERR_NOMEM, "noise buffer allocation failed");
+    h->xi_dev = buf;
+    HIPCHK(h, hipMemcpy(buf, xi, sizeof(double) * n_xi, hipMemcpyHostToDevice));
+    h->g.xi_buf = buf;
+    h->g.xi_m0 = (h->i_step_host + 1) / 2;  // the next midstep's index
+    h->g.xi_nm = (int64_t)(n_xi / per_mid);
+  }
+  drop_graph(h);  // kernels take the system constants by value
   return SC_OK;
 }
 
@@ -1192,7 +1555,7 @@ struct CkHeader {
   Ctl ctl;
 };
 
-constexpr char kCkMagic[8] = {'S', 'C', 'C', 'K', 'P', 'T', '0', '1'};
+constexpr char kCkMagic[8] = {'S', 'C', 'C', 'K', 'P', 'T', '0', '2'};
 
 // One field of the checkpoint: `n` elements of `T` at device address `dev`.
 struct CkField {
@@ -1204,7 +1567,7 @@ std::vector<CkField> ck_fields(sc_system *h, int64_t nc) {
   const Dev &d = h->d;
   const int64_t ns = h->g.ns, nb = h->g.nb, nf = h->g.nf;
   const size_t slot = h->p.precision == SC_PREC_FP64 ? sizeof(double4) : sizeof(float4);
-  void *slots = h->p.precision == SC_PREC_FP64 ? (void *)d.slot64 : (void *)d.slot32;
+  void *slots = h->p.precision == SC_PREC_FP64 ? (void *)d.islot64 : (void *)d.islot32;
   return {{d.r, 24 * (size_t)ns},        {d.rp, 24 * (size_t)ns},
           {d.fr, (h->g.compact ? 12 : 24) * (size_t)ns}, {d.dr0, 24 * (size_t)ns},
           {d.acc, 8 * (size_t)ns},       {d.acc_prev, 8 * (size_t)ns},
@@ -1213,7 +1576,7 @@ std::vector<CkField> ck_fields(sc_system *h, int64_t nc) {
           {d.bt ? (void *)d.bt : (void *)d.bt32, (d.bt ? 24 : 12) * (size_t)nb},
           {d.tension, 8 * (size_t)nb},
           {d.keys, 4 * (size_t)nb},      {d.norm, 8 * (size_t)nf},
-          {d.pairs, 8 * (size_t)nc},     {slots, slot * (size_t)nc},
+          {slots, 2 * slot * (size_t)nc},
           {d.inc_start, 4 * (size_t)(nb + 1)}, {d.inc, 16 * (size_t)nc},
           {d.xb, 8 * (size_t)h->g.nx}, {d.xlam, 16 * (size_t)h->g.nx},
           {d.xf, 72 * (size_t)h->g.nx}, {d.xinc, 8 * (size_t)h->g.nx},
@@ -1243,7 +1606,7 @@ int sc_checkpoint_save(sc_handle h, void *buf, size_t cap, size_t *size) {
   char *p = static_cast<char *>(buf);
   CkHeader hd{};
   std::memcpy(hd.magic, kCkMagic, 8);
-  hd.version = 1;
+  hd.version = 2;
   hd.precision = h->p.precision;
   hd.spatial_order = h->reordered ? 1 : 0;
   hd.nf = h->g.nf;
@@ -1276,7 +1639,7 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size) {
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   CkHeader hd;
   std::memcpy(&hd, buf, sizeof(hd));
-  if (std::memcmp(hd.magic, kCkMagic, 8) != 0 || hd.version != 1)
+  if (std::memcmp(hd.magic, kCkMagic, 8) != 0 || hd.version != 2)
     return fail(h, SC_ERR_ARG, "not a checkpoint of this library");
   if (hd.precision != h->p.precision || hd.spatial_order != (h->reordered ? 1 : 0) ||
       hd.nf != h->g.nf || hd.ns != h->g.ns || hd.nb != h->g.nb || hd.n_cells != h->g.n_cells ||
@@ -1319,6 +1682,8 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   if (steps < 2) steps = 2;
   steps += steps & 1;  // whole midstep/full-step pairs
+  // the side-stream shapes are tried whenever there are several bins
+  if (h->bins.size() > 1 && (rc = ensure_side_streams(h))) return rc;
   size_t n = 0;
   if ((rc = sc_checkpoint_save(h, nullptr, 0, &n))) return rc;
   std::vector<char> blob(n);
@@ -1328,6 +1693,10 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   HIPCHK(h, hipEventCreate(&e1));
   const bool prof = h->prof;
   h->prof = false;
+  // trials step past the caller's noise buffer: they draw from the counter
+  // stream instead (their trajectories are discarded)
+  const double *xi_keep = h->g.xi_buf;
+  h->g.xi_buf = nullptr;
   // time `steps` steps of configuration t from the saved state
   auto trial = [&](const sc::Tune &t, double *ms) -> int {
     h->tune = t;
@@ -1385,7 +1754,12 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   // mixed path and is not tuned)
   const sc::Tune pair_best = best;
   const int waves[] = {1, 2, 4};
-  const int n_str = h->bins.size() > 1 ? 1 + sc_system::kSide : 1;  // 0..3 side streams
+  // side streams only matter with several integrator launches (0..3 of them)
+  int launches = 0;
+  for (const Bin &b : h->bins)
+    if (!(h->tune.int_layout == 2 && h->pack_waves > 0) || b.maxn > 128) ++launches;
+  if (h->tune.int_layout == 2 && h->pack_waves > 0) ++launches;
+  const int n_str = std::min(launches, 1 + sc_system::kSide);
   for (int w : waves)
     for (int sm = 0; sm < n_str; ++sm) {
       sc::Tune t = pair_best;
@@ -1396,6 +1770,7 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   // leave the system where it was, with the chosen shapes
   h->tune = rc ? start : best;
   drop_graph(h);
+  h->g.xi_buf = xi_keep;
   const int rc2 = sc_checkpoint_load(h, blob.data(), blob.size());
   h->prof = prof;
   (void)hipEventDestroy(e0);
@@ -1420,7 +1795,7 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
        v.pair_block != 512) ||
       (v.pair_xcd != 0 && v.pair_xcd != 1) ||
       (v.integrate_waves != 1 && v.integrate_waves != 2 && v.integrate_waves != 4) ||
-      (v.integrate_layout != 0 && v.integrate_layout != 1) ||
+      (v.integrate_layout < 0 || v.integrate_layout > 2) ||
       (v.integrate_streams < 0 || v.integrate_streams > 3))
     return fail(h, SC_ERR_ARG, "unsupported launch shape");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
@@ -1431,6 +1806,7 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
   h->tune.int_layout = v.integrate_layout;
   h->tune.int_streams = v.integrate_streams;
   drop_graph(h);
+  if (v.integrate_streams > 0) return ensure_side_streams(h);
   return SC_OK;
 }
 
@@ -1559,6 +1935,7 @@ int sc_destroy(sc_handle h) {
     if (h->join_ev[k]) (void)hipEventDestroy(h->join_ev[k]);
   }
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+  if (h->xi_dev) (void)hipFree(h->xi_dev);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return SC_OK;

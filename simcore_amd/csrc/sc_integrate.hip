@@ -302,7 +302,7 @@ __global__ __launch_bounds__(64) void k_integrate(Dev d, Geo g,
         const int i = gl + G * k;
         if (i >= n) continue;
         double xi[3];
-        noise_xi(g.seed, (int64_t)d.noise_off[f] + i, mstep, xi);
+        draw_xi(g, (int64_t)d.noise_off[f] + i, mstep, xi);
         const double t0 = TN[0][i], t1 = TN[1][i], t2 = TN[2][i];
         double xt[3];
         if (EXACT) {
@@ -701,14 +701,6 @@ void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
     default: SC_LAUNCH(64, 4); break;
   }
 #undef SC_LAUNCH
-}
-
-void launch_integrate_all(const Dev &d, const Geo &g, const BinDesc *bins, int nbins,
-                          int precision, const Tune &t, bool mid, hipStream_t st) {
-  // one launch per bin: a single launch over all bins (the waves of each
-  // bin running their own variant) measured markedly slower than this
-  for (int k = 0; k < nbins; ++k)
-    launch_integrate(d, g, bins[k].list, bins[k].count, bins[k].maxn, precision, t, mid, st);
 }
 
 }  // namespace sc

@@ -27,11 +27,13 @@ using namespace lanes;
 
 namespace {
 
-// o[k] = value at site gl*S + k + DELTA of the group (fb outside its G*S
-// sites); the shuffles run on every lane
-template <int G, int S, int DELTA>
+// o[k] = value at site gl*S + k + DELTA of the filament's lane segment
+// (fb outside its Gf lanes); the shuffles run on every lane.  W is the
+// width of the aligned lane group the segment lies in (it selects the lane
+// moves), Gf <= W the segment's own width.
+template <int W, int S, int DELTA>
 __device__ __forceinline__ void site_at(const double (&v)[S], double (&o)[S], int lane, int gl,
-                                        double fb) {
+                                        int Gf, double fb) {
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     const int t = k + DELTA;
@@ -40,32 +42,104 @@ __device__ __forceinline__ void site_at(const double (&v)[S], double (&o)[S], in
     if (q == 0) {
       o[k] = v[ks];
     } else {
-      const bool ok = gl + q >= 0 && gl + q < G;
-      const double y = lane_shift_c<G>(v[ks], lane, q);
+      const bool ok = gl + q >= 0 && gl + q < Gf;
+      const double y = lane_shift_c<W>(v[ks], lane, q);
       o[k] = ok ? y : fb;
     }
   }
 }
 
-template <int G, int S, int DELTA>
+template <int W, int S, int DELTA>
 __device__ __forceinline__ void site_at3(const double (&v)[S][3], double (&o)[S][3], int lane,
-                                         int gl) {
+                                         int gl, int Gf) {
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     double a[S], b[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) a[k] = v[k][q];
-    site_at<G, S, DELTA>(a, b, lane, gl, 0.0);
+    site_at<W, S, DELTA>(a, b, lane, gl, Gf, 0.0);
 #pragma unroll
     for (int k = 0; k < S; ++k) o[k][q] = b[k];
   }
 }
 
+// value of segment lane gl+delta; `fb` outside [0, Gf)
+template <int W>
+__device__ __forceinline__ double seg_from(double v, int lane, int gl, int Gf, int delta,
+                                           double fb) {
+  const int src = gl + delta;
+  const bool ok = src >= 0 && src < Gf;
+  const double x = lane_shift_c<W>(v, lane, delta);
+  return ok ? x : fb;
+}
+
+// Parallel cyclic reduction over the segment's lanes (row gl in lane gl;
+// rows >= m inert).  Levels beyond the segment's width leave every row
+// unchanged (both couplings are zero by then), so W bounds the loop.
+template <int W>
+__device__ __forceinline__ double pcr_seg(double a, double b, double c, double d, int m, int lane,
+                                          int gl, int Gf) {
+  if (gl >= m) {
+    a = 0.0;
+    b = 1.0;
+    c = 0.0;
+    d = 0.0;
+  }
+#pragma unroll
+  for (int s = 1; s < W; s <<= 1) {
+    const double rb = rcp_fast(b);
+    const double am = seg_from<W>(a, lane, gl, Gf, -s, 0.0);
+    const double cm = seg_from<W>(c, lane, gl, Gf, -s, 0.0);
+    const double dm = seg_from<W>(d, lane, gl, Gf, -s, 0.0);
+    const double rm = seg_from<W>(rb, lane, gl, Gf, -s, 0.0);
+    const double ap = seg_from<W>(a, lane, gl, Gf, s, 0.0);
+    const double cp = seg_from<W>(c, lane, gl, Gf, s, 0.0);
+    const double dp = seg_from<W>(d, lane, gl, Gf, s, 0.0);
+    const double rp = seg_from<W>(rb, lane, gl, Gf, s, 0.0);
+    const bool lo = gl - s >= 0, hi = gl + s < m;
+    const double al = lo ? -a * rm : 0.0;
+    const double ga = hi ? -c * rp : 0.0;
+    const double na = al * am;
+    const double nc = ga * cp;
+    b = b + al * cm + ga * ap;
+    d = d + al * dm + ga * dp;
+    a = na;
+    c = nc;
+  }
+  return d * rcp_fast(b);
+}
+
+// inclusive prefix (dir -1, newest on the left) / suffix (dir +1) product
+// of 2x2 matrices over the segment; identity outside it
+template <int W>
+__device__ __forceinline__ void mat_scan_seg(double *P, int lane, int gl, int Gf, int dir) {
+#pragma unroll
+  for (int s = 1; s < W; s <<= 1) {
+    double Q[4];
+    const double id[4] = {1.0, 0.0, 0.0, 1.0};
+    for (int k = 0; k < 4; ++k) Q[k] = seg_from<W>(P[k], lane, gl, Gf, dir * s, id[k]);
+    double R[4];
+    R[0] = P[0] * Q[0] + P[1] * Q[2];
+    R[1] = P[0] * Q[1] + P[1] * Q[3];
+    R[2] = P[2] * Q[0] + P[3] * Q[2];
+    R[3] = P[2] * Q[1] + P[3] * Q[3];
+    for (int k = 0; k < 4; ++k) P[k] = R[k];
+  }
+}
+
+// inclusive prefix (dir -1) / suffix (dir +1) sum over the segment
+template <int W>
+__device__ __forceinline__ double sum_scan_seg(double v, int lane, int gl, int Gf, int dir) {
+#pragma unroll
+  for (int s = 1; s < W; s <<= 1) v += seg_from<W>(v, lane, gl, Gf, dir * s, 0.0);
+  return v;
+}
+
 // Solves rows i = lS + k < m (lower a: x[i-1], diagonal b, upper c:
 // x[i+1], right-hand side d); rows >= m are inert.  Result in x.
-template <int G, int S>
+template <int W, int S>
 __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], double (&c)[S],
-                                              double (&d)[S], int m, int lane, int gl,
+                                              double (&d)[S], int m, int lane, int gl, int Gf,
                                               double (&x)[S]) {
 #pragma unroll
   for (int k = 0; k < S; ++k)
@@ -84,8 +158,8 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
     // shuffles on every lane (a lane-dependent condition around them would
     // switch off the lanes they read from), then the boundary select
     const bool first = gl == 0;
-    const double ya = lane_shift<G, -1>(a[S - st], lane), yb = lane_shift<G, -1>(b[S - st], lane);
-    const double yc = lane_shift<G, -1>(c[S - st], lane), yd = lane_shift<G, -1>(d[S - st], lane);
+    const double ya = lane_shift<W, -1>(a[S - st], lane), yb = lane_shift<W, -1>(b[S - st], lane);
+    const double yc = lane_shift<W, -1>(c[S - st], lane), yd = lane_shift<W, -1>(d[S - st], lane);
     const double pa = first ? 0.0 : ya, pb = first ? 1.0 : yb;
     const double pc = first ? 0.0 : yc, pd = first ? 0.0 : yd;
 #pragma unroll
@@ -102,11 +176,11 @@ __device__ __forceinline__ void solve_blocked(double (&a)[S], double (&b)[S], do
   }
   // one row per lane left: parallel cyclic reduction across the lanes
   const int m_red = (m + S - 1) / S;
-  x[0] = pcr<G>(a[0], b[0], c[0], d[0], m_red, lane, gl);
+  x[0] = pcr_seg<W>(a[0], b[0], c[0], d[0], m_red, lane, gl, Gf);
 #pragma unroll
   for (int st = S / 2; st >= 1; st >>= 1) {
-    const bool last = gl == G - 1;
-    const double nx = lane_shift<G, 1>(x[0], lane);
+    const bool last = gl == Gf - 1;
+    const double nx = lane_shift<W, 1>(x[0], lane);
 #pragma unroll
     for (int k = st; k < S; k += 2 * st) {
       const double xl = x[k - st];
@@ -125,21 +199,21 @@ __device__ __forceinline__ void mat_mul(const double *P, const double *Q, double
 
 // Prefix (forward) products of per-site 2x2 matrices M (newest on the
 // left): out[k] = M_{lS+k} ... M_0 applied to [2, 1], first component.
-template <int G, int S>
+template <int W, int S>
 __device__ __forceinline__ void det_forward(const double (&M)[S][4], double (&out)[S],
-                                            int lane, int gl) {
+                                            int lane, int gl, int Gf) {
   double L[S][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) L[0][q] = M[0][q];
 #pragma unroll
   for (int k = 1; k < S; ++k) mat_mul(M[k], L[k - 1], L[k]);
   double A[4] = {L[S - 1][0], L[S - 1][1], L[S - 1][2], L[S - 1][3]};
-  mat_scan<G>(A, lane, gl, -1);  // A_l ... A_0
+  mat_scan_seg<W>(A, lane, gl, Gf, -1);  // A_l ... A_0
   double E[4];
   const double id[4] = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const double y = lane_shift<G, -1>(A[q], lane);
+    const double y = lane_shift<W, -1>(A[q], lane);
     E[q] = gl > 0 ? y : id[q];
   }
 #pragma unroll
@@ -151,22 +225,22 @@ __device__ __forceinline__ void det_forward(const double (&M)[S][4], double (&ou
 }
 
 // Suffix products: out[k] = N_{lS+k} ... N_last applied to [2, 1].
-template <int G, int S>
+template <int W, int S>
 __device__ __forceinline__ void det_backward(const double (&N)[S][4], double (&out)[S],
-                                             int lane, int gl) {
+                                             int lane, int gl, int Gf) {
   double R[S][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) R[S - 1][q] = N[S - 1][q];
 #pragma unroll
   for (int k = S - 2; k >= 0; --k) mat_mul(N[k], R[k + 1], R[k]);
   double B[4] = {R[0][0], R[0][1], R[0][2], R[0][3]};
-  mat_scan<G>(B, lane, gl, +1);  // B_l ... B_last
+  mat_scan_seg<W>(B, lane, gl, Gf, +1);  // B_l ... B_last
   double F[4];
   const double id[4] = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const double y = lane_shift<G, 1>(B[q], lane);
-    F[q] = gl < G - 1 ? y : id[q];
+    const double y = lane_shift<W, 1>(B[q], lane);
+    F[q] = gl < Gf - 1 ? y : id[q];
   }
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -178,22 +252,19 @@ __device__ __forceinline__ void det_backward(const double (&N)[S][4], double (&o
 
 }  // namespace
 
-// One wavefront's filaments: wave `wave` of a bin holds list entries
-// wave*64/G ... (wave+1)*64/G - 1.  Everything stays in registers; there is
-// no LDS and no barrier.
+// One filament's lane segment: lanes [lane - gl, lane - gl + Gf) of the
+// wavefront hold its sites, two (S) per lane; the segment lies inside an
+// aligned group of W lanes (16, 32 or 64), which selects the lane moves.
+// Lanes with `factive` false take part in every lane move and write
+// nothing.  Everything stays in registers; there is no LDS and no barrier.
 // MID: the step is a midstep (compiled separately from the full step, so
 // each gets its own register allocation; the midstep's random-force solve
-// sets the larger one).
-template <int G, int S, bool MID>
-__device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int32_t *list,
-                                          int count, int wave) {
-  constexpr int NG = 64 / G;
+// sets the larger one).  `slot`: the displacement-maximum slot of the wave.
+template <int W, int S, bool MID>
+__device__ __forceinline__ void long_body(const Dev &d, const Geo &g, bool factive, int f,
+                                          int gl, int Gf, int slot) {
   const Ctl *ctl = d.ctl;
   const int lane = threadIdx.x & 63;
-  const int grp = lane / G, gl = lane % G;
-  const int fidx = wave * NG + grp;
-  const bool factive = fidx < count;
-  const int f = factive ? list[fidx] : 0;
   const int64_t s0 = factive ? d.site_off[f] : 0;
   const int n = factive ? (int)(d.site_off[f + 1] - s0) : 0;
   const int64_t b0 = s0 - f;
@@ -233,7 +304,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       const int i = base + k;
       for (int q = 0; q < 3; ++q) rc[k][q] = i < n ? rsrc[q * ns + s0 + i] : 0.0;
     }
-    site_at3<G, S, 1>(rc, rnx, lane, gl);
+    site_at3<W, S, 1>(rc, rnx, lane, gl, Gf);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       double o[3], l = 0.0;
@@ -244,7 +315,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       const double il = rcp_fast(sqrt(l));
       for (int q = 0; q < 3; ++q) ub[k][q] = o[q] * il;
     }
-    site_at3<G, S, -1>(ub, ubm, lane, gl);
+    site_at3<W, S, -1>(ub, ubm, lane, gl, Gf);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
@@ -260,8 +331,8 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
 
   // ---- angles and tangents ----------------------------------------------
   double um[S][3], up[S][3], t[S][3], cs[S];
-  site_at3<G, S, -1>(u, um, lane, gl);
-  site_at3<G, S, 1>(u, up, lane, gl);
+  site_at3<W, S, -1>(u, um, lane, gl, Gf);
+  site_at3<W, S, 1>(u, up, lane, gl, Gf);
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     const int i = base + k;
@@ -275,7 +346,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     }
   }
   double csm[S];
-  site_at<G, S, -1>(cs, csm, lane, gl, 0.0);
+  site_at<W, S, -1>(cs, csm, lane, gl, Gf, 0.0);
 
   // ---- random forces (midstep) ------------------------------------------
   if (mid) {
@@ -284,13 +355,13 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       for (int k = 0; k < S; ++k) {
         const int i = base + k;
         double xi[3] = {0.0, 0.0, 0.0};
-        if (i < n) noise_xi(g.seed, noise0 + i, (step - 1) >> 1, xi);
+        if (i < n) draw_xi(g, noise0 + i, (step - 1) >> 1, xi);
         const double txi = dot(t[k], xi);
         for (int q = 0; q < 3; ++q)
           fr[k][q] = sgperp * xi[q] + (sgpar - sgperp) * t[k][q] * txi;
       }
       double frn[S][3], a[S], b[S], c[S], rh[S], Tp[S], Tm[S];
-      site_at3<G, S, 1>(fr, frn, lane, gl);
+      site_at3<W, S, 1>(fr, frn, lane, gl, Gf);
 #pragma unroll
       for (int k = 0; k < S; ++k) {
         const int i = base + k;
@@ -301,8 +372,8 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
         b[k] = 2.0;
         c[k] = i < m - 1 ? -cs[k] : 0.0;
       }
-      solve_blocked<G, S>(a, b, c, rh, m, lane, gl, Tp);
-      site_at<G, S, -1>(Tp, Tm, lane, gl, 0.0);
+      solve_blocked<W, S>(a, b, c, rh, m, lane, gl, Gf, Tp);
+      site_at<W, S, -1>(Tp, Tm, lane, gl, Gf, 0.0);
 #pragma unroll
       for (int k = 0; k < S; ++k) {
         const int i = base + k;
@@ -346,8 +417,8 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       sf[k][q] = 0.5 * Fb[k][q];
     }
   }
-  site_at3<G, S, -1>(sf, sfm, lane, gl);
-  site_at3<G, S, -1>(pt, ptm, lane, gl);
+  site_at3<W, S, -1>(sf, sfm, lane, gl, Gf);
+  site_at3<W, S, -1>(pt, ptm, lane, gl, Gf);
 #pragma unroll
   for (int k = 0; k < S; ++k) {
     const int i = base + k;
@@ -371,7 +442,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   // ---- bending (src/filament.cpp:608-776), reduced form ------------------
   {
     double csmm[S], M[S][4], N[S][4], det_t[S], det_b[S], db3[S], ke[S];
-    site_at<G, S, -1>(csm, csmm, lane, gl, 0.0);
+    site_at<W, S, -1>(csm, csmm, lane, gl, Gf, 0.0);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
@@ -381,7 +452,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       M[k][2] = sm ? 1.0 : 0.0;
       M[k][3] = sm ? 0.0 : 1.0;
     }
-    det_forward<G, S>(M, det_t, lane, gl);
+    det_forward<W, S>(M, det_t, lane, gl, Gf);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
@@ -391,7 +462,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       N[k][2] = sn ? 1.0 : 0.0;
       N[k][3] = sn ? 0.0 : 1.0;
     }
-    det_backward<G, S>(N, det_b, lane, gl);
+    det_backward<W, S>(N, det_b, lane, gl, Gf);
     // det_t at the last site, det_b three sites ahead (det_b[n] = 1)
     const int last = n > 0 ? n - 1 : 0;
     double pick = 0.0;
@@ -399,7 +470,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     for (int k = 0; k < S; ++k)
       if (k == last % S) pick = det_t[k];
     const double det_g = __shfl(pick, lane - gl + last / S, 64);
-    site_at<G, S, 3>(det_b, db3, lane, gl, 1.0);
+    site_at<W, S, 3>(det_b, db3, lane, gl, Gf, 1.0);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
@@ -412,9 +483,9 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       }
     }
     double kem[S], kemm[S], umm[S][3];
-    site_at<G, S, -1>(ke, kem, lane, gl, 0.0);
-    site_at<G, S, -2>(ke, kemm, lane, gl, 0.0);
-    site_at3<G, S, -2>(u, umm, lane, gl);
+    site_at<W, S, -1>(ke, kem, lane, gl, Gf, 0.0);
+    site_at<W, S, -2>(ke, kemm, lane, gl, Gf, 0.0);
+    site_at3<W, S, -2>(u, umm, lane, gl, Gf);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
@@ -441,8 +512,8 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   double T[S], Tm[S];
   {
     double tn[S][3], fn[S][3], hu[S], hun[S], a[S], b[S], c[S], rh[S];
-    site_at3<G, S, 1>(t, tn, lane, gl);
-    site_at3<G, S, 1>(fs, fn, lane, gl);
+    site_at3<W, S, 1>(t, tn, lane, gl, Gf);
+    site_at3<W, S, 1>(fs, fn, lane, gl, Gf);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       double m1[3], m2[3], fd[3];
@@ -454,16 +525,16 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       b[k] = 2.0 * igperp + dg * (a1 * a1 + a2 * a2);
       hu[k] = -igperp * dot(u[k], um[k]) - dg * (dot(t[k], um[k]) * dot(t[k], u[k]));
     }
-    site_at<G, S, 1>(hu, hun, lane, gl, 0.0);
+    site_at<W, S, 1>(hu, hun, lane, gl, Gf, 0.0);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       const int i = base + k;
       a[k] = i >= 1 ? hu[k] : 0.0;
       c[k] = i < m - 1 ? hun[k] : 0.0;
     }
-    solve_blocked<G, S>(a, b, c, rh, m, lane, gl, T);
+    solve_blocked<W, S>(a, b, c, rh, m, lane, gl, Gf, T);
   }
-  site_at<G, S, -1>(T, Tm, lane, gl, 0.0);
+  site_at<W, S, -1>(T, Tm, lane, gl, Gf, 0.0);
 #pragma unroll
   for (int k = 0; k < S; ++k)
     if (base + k < m) d.tension[b0 + base + k] = T[k];
@@ -496,7 +567,7 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
   }
   // new bond lengths mm and unit vectors uo, kept for the bond records
   double mm[S];
-  site_at3<G, S, 1>(rn, rnn, lane, gl);
+  site_at3<W, S, 1>(rn, rnn, lane, gl, Gf);
   int bad = 0;
 #pragma unroll
   for (int k = 0; k < S; ++k) {
@@ -512,10 +583,16 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     }
   }
   const unsigned long long bal = __ballot(bad);
-  const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << (grp * G));
-  if ((bal & gmask) != 0) {
+  const unsigned long long gmask =
+      (Gf >= 64) ? ~0ull : (((1ull << Gf) - 1ull) << (lane - gl));
+  const bool need = (bal & gmask) != 0;
+  if (bal != 0) {
     // bond renormalisation (src/filament.cpp:904-924): cumulative sums of
-    // l u along the chain, tail to head or head to tail by the switch
+    // l u along the chain, tail to head or head to tail by the switch.  The
+    // branch is taken by the whole wavefront and both directions are summed
+    // on every lane: the segments of a wavefront share lane groups, whose
+    // wide moves pass values through other segments' lanes, so every lane
+    // must take part; each segment keeps the result it needs.
     int32_t sw = factive ? d.norm[2 * f] : 1;
     sw = !sw;
     double st3[S][3];
@@ -525,46 +602,44 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
     const int last = n > 0 ? n - 1 : 0;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      double loc[S], tot;
-      if (sw) {
-        // r_i = r_0 + sum_{j < i} l u_j
-        double run = 0.0;
+      // tail to head: r_i = r_0 + sum_{j < i} l u_j
+      double locp[S], run = 0.0;
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-          loc[k] = run;  // exclusive inside the lane
-          run += st3[k][q];
+      for (int k = 0; k < S; ++k) {
+        locp[k] = run;  // exclusive inside the lane
+        run += st3[k][q];
+      }
+      const double inclp = sum_scan_seg<W>(run, lane, gl, Gf, -1);
+      const double yp = lane_shift<W, -1>(inclp, lane);
+      const double before = gl > 0 ? yp : 0.0;
+      const double r0 = __shfl(rn[0][q], lane - gl, 64);
+      // head to tail: r_i = r_{n-1} - sum_{j >= i} l u_j
+      double locs[S];
+      run = 0.0;
+#pragma unroll
+      for (int k = S - 1; k >= 0; --k) {
+        run += st3[k][q];
+        locs[k] = run;  // inclusive suffix inside the lane
+      }
+      const double incls = sum_scan_seg<W>(run, lane, gl, Gf, +1);
+      const double ys = lane_shift<W, 1>(incls, lane);
+      const double after = gl < Gf - 1 ? ys : 0.0;
+      double pick = 0.0;
+#pragma unroll
+      for (int k = 0; k < S; ++k)
+        if (k == last % S) pick = rn[k][q];
+      const double rl = __shfl(pick, lane - gl + last / S, 64);
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        if (!need) continue;
+        if (sw) {
+          if (base + k >= 1) rn[k][q] = r0 + (before + locp[k]);
+        } else {
+          if (base + k < n - 1) rn[k][q] = rl - (after + locs[k]);
         }
-        tot = run;
-        const double incl = sum_scan<G>(tot, lane, gl, -1);
-        const double y = lane_shift<G, -1>(incl, lane);
-        const double before = gl > 0 ? y : 0.0;
-        const double r0 = __shfl(rn[0][q], lane - gl, 64);
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-          if (base + k >= 1) rn[k][q] = r0 + (before + loc[k]);
-      } else {
-        // r_i = r_{n-1} - sum_{j >= i} l u_j
-        double run = 0.0;
-#pragma unroll
-        for (int k = S - 1; k >= 0; --k) {
-          run += st3[k][q];
-          loc[k] = run;  // inclusive suffix inside the lane
-        }
-        tot = run;
-        const double incl = sum_scan<G>(tot, lane, gl, +1);
-        const double y = lane_shift<G, 1>(incl, lane);
-        const double after = gl < G - 1 ? y : 0.0;
-        double pick = 0.0;
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-          if (k == last % S) pick = rn[k][q];
-        const double rl = __shfl(pick, lane - gl + last / S, 64);
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-          if (base + k < n - 1) rn[k][q] = rl - (after + loc[k]);
       }
     }
-    site_at3<G, S, 1>(rn, rnn, lane, gl);
+    site_at3<W, S, 1>(rn, rnn, lane, gl, Gf);
 #pragma unroll
     for (int k = 0; k < S; ++k) {
       double rd[3];
@@ -573,13 +648,12 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       const double inv = rcp_fast(mm[k]);
       for (int q = 0; q < 3; ++q) uo[k][q] = rd[q] * inv;
     }
-    if (gl == 0 && factive) {
+    if (need && gl == 0 && factive) {
       d.norm[2 * f] = sw;
       d.norm[2 * f + 1] = 1;
     }
-  } else if (gl == 0 && factive) {
-    d.norm[2 * f + 1] += 1;
   }
+  if (!need && gl == 0 && factive) d.norm[2 * f + 1] += 1;
 
   // ---- write back, bond records, rebuild bookkeeping ---------------------
   double lmax = 0.0;
@@ -614,13 +688,15 @@ __device__ __forceinline__ void long_wave(const Dev &d, const Geo &g, const int3
       lmax = y > lmax ? y : lmax;
     }
     if (lane == 0)
-      atomicMax(&d.dr_slots[wave & (kDrSlots - 1)],
+      atomicMax(&d.dr_slots[slot & (kDrSlots - 1)],
                 (unsigned long long)__double_as_longlong(lmax));
   }
 }
 
 // two sites per lane: at most 128 VGPRs, four waves per SIMD (the midstep
-// needs 129 left to itself); four sites per lane: two waves
+// needs 129 left to itself); four sites per lane: two waves.  Length-binned
+// layout: G lanes per filament, 64/G filaments per wave, wave `wave` of a
+// bin holding list entries wave*64/G ... (wave+1)*64/G - 1.
 template <int G, int S, bool MID>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S == 2 ? 4 : 1)))
 void k_integrate_long(Dev d, Geo g, const int32_t *list,
@@ -634,7 +710,62 @@ void k_integrate_long(Dev d, Geo g, const int32_t *list,
     }
     return;
   }
-  long_wave<G, S, MID>(d, g, list, count, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  constexpr int NG = 64 / G;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int fidx = wave * NG + lane / G;
+  const bool factive = fidx < count;
+  long_body<G, S, MID>(d, g, factive, factive ? list[fidx] : 0, lane % G, G, wave);
+}
+
+// Packed layout: filaments of up to 128 sites placed as lane segments of
+// their own width ceil(n/2) (lane table entry: filament << 8 | W code << 6
+// | lane in segment, -1 for an idle lane), a segment of at most 16 lanes
+// inside one 16-lane row, of 17-32 lanes inside one half, of 33-64 lanes
+// from lane 0; the wave's W code (0: 16, 1: 32, 2: 64) selects the lane
+// moves.  Same arithmetic as the binned layout, far fewer idle lanes.
+template <bool MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void k_integrate_packed(Dev d, Geo g, const int32_t *lane_tab, const int32_t *wave_code,
+                        int n_waves) {
+  Ctl *ctl = d.ctl;
+  if (ctl->abort) return;
+  if ((ctl->midstep != 0) != MID) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl->err = SC_ERR_STATE;
+      ctl->abort = 1;
+    }
+    return;
+  }
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave >= n_waves) return;
+  const int lane = threadIdx.x & 63;
+  const int32_t e = lane_tab[(int64_t)wave * 64 + lane];
+  const bool factive = e >= 0;
+  const int f = factive ? (e >> 8) : 0;
+  const int gl = factive ? (e & 63) : 0;
+  int Gf = 1;
+  if (factive) Gf = (d.site_off[f + 1] - d.site_off[f] + 1) >> 1;
+  const int code = __builtin_amdgcn_readfirstlane(wave_code[wave]);
+  if (code == 0)
+    long_body<16, 2, MID>(d, g, factive, f, gl, Gf, wave);
+  else if (code == 1)
+    long_body<32, 2, MID>(d, g, factive, f, gl, Gf, wave);
+  else
+    long_body<64, 2, MID>(d, g, factive, f, gl, Gf, wave);
+}
+
+void launch_integrate_packed(const Dev &d, const Geo &g, const int32_t *lane_tab,
+                             const int32_t *wave_code, int n_waves, int waves, bool mid,
+                             hipStream_t st) {
+  if (n_waves <= 0) return;
+  dim3 grid((n_waves + waves - 1) / waves), block(64 * waves);
+  if (mid)
+    hipLaunchKernelGGL((k_integrate_packed<true>), grid, block, 0, st, d, g, lane_tab, wave_code,
+                       n_waves);
+  else
+    hipLaunchKernelGGL((k_integrate_packed<false>), grid, block, 0, st, d, g, lane_tab,
+                       wave_code, n_waves);
 }
 
 bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list, int bin_count,

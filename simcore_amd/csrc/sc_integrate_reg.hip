@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
   if (mid) {
     if (g.stoch) {
       double xi[3] = {0, 0, 0};
-      if (act) noise_xi(g.seed, (int64_t)d.noise_off[f] + i, (step - 1) >> 1, xi);
+      if (act) draw_xi(g, (int64_t)d.noise_off[f] + i, (step - 1) >> 1, xi);
       const double txi = dot(t, xi);
       for (int q = 0; q < 3; ++q) fr[q] = sgperp * xi[q] + (sgpar - sgperp) * t[q] * txi;
       double frp[3], frn[3];

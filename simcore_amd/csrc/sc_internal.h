@@ -53,6 +53,10 @@ struct Geo {
   int compact;             // mixed precision: 32-byte bond records (sc_math.h)
   double inv_n1d;
   uint64_t seed;
+  // caller-supplied noise (sc_set_noise): xi of midsteps [xi_m0, xi_m0 +
+  // xi_nm), 3 per caller site, site-major; null: the counter-based stream
+  const double *xi_buf;
+  int64_t xi_m0, xi_nm;
   int32_t nf;
   int64_t ns, nb, n_cells, cap;
   // crosslinks
@@ -89,13 +93,17 @@ struct Dev {
   int32_t *keys, *cell_count, *cell_start, *sorted, *skey;
   int32_t *sorted_o;       // cell runs in bond order
   uint8_t *cell_occ;       // n_cells bits: cell occupied in the last build
-  int32_t *ccount, *tsum;
-  // candidates
-  int2 *pairs;
-  double4 *slot64;         // {Fx,Fy,Fz,-}
-  float4 *slot32;          // {Fx,Fy,Fz,-}
+  // candidates: per bond, its incidences in candidate order (CSR over
+  // inc_start), each with its own copy of the pair's sticky force -- both
+  // owners compute the same bits, so the copies agree, and a bond's slot
+  // traffic follows its list.  The candidate list itself is implicit (the
+  // reference's order is recovered from the lists for the taps).
   int32_t *deg, *inc_start;
-  int2 *inc;               // per incidence {c << 1 | role, partner bond}
+  int2 *inc;               // {partner bond, role | written << 1}: role 1 when
+                           // the bond is obj2; written once the slot holds a
+                           // force (a slot not written since the rebuild is 0)
+  double4 *islot64;        // sticky force per incidence {Fx,Fy,Fz,-}, FP64
+  float4 *islot32;         // the same in FP32 (mixed precision)
   // scan scratch
   int32_t *scan_tmp;
   // crosslinks: anchors (2 per crosslink), stored forces (9 per
@@ -155,9 +163,6 @@ void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t,
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
                       int bin_count, int bin_maxn, int precision, const Tune &t, bool mid,
                       hipStream_t st);
-// all length bins of a step, one launch per bin
-void launch_integrate_all(const Dev &d, const Geo &g, const BinDesc *bins, int nbins,
-                          int precision, const Tune &t, bool mid, hipStream_t st);
 // register-resident fast-path integrator; false when the bin is not one it
 // serves (FP64 precision or more than 64 sites), nothing launched then
 bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,
@@ -168,6 +173,11 @@ bool launch_integrate_reg(const Dev &d, const Geo &g, const int32_t *bin_list,
 bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list,
                            int bin_count, int bin_maxn, int precision, int waves,
                            int layout, bool mid, hipStream_t st);
+// packed layout for the mixed path: filaments of up to 128 sites as lane
+// segments of their own width (sc_integrate_long.hip)
+void launch_integrate_packed(const Dev &d, const Geo &g, const int32_t *lane_tab,
+                             const int32_t *wave_code, int n_waves, int waves, bool mid,
+                             hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
 // crosslinks (sc_crosslink.hip)

@@ -277,7 +277,7 @@ __global__ void k_key_count(Dev d, Geo g) {
     atomicAdd(&d.cell_count[d.keys[b]], 1);
 }
 
-// Counting-sort scatter (order inside a cell arbitrary; k_count orders the
+// Counting-sort scatter (order inside a cell arbitrary; k_deg orders the
 // runs); returns cell_count to all-zero for the next build.
 __global__ void k_scatter(Dev d, Geo g, int guard) {
   if (guard_off(d.ctl, guard)) return;
@@ -288,14 +288,6 @@ __global__ void k_scatter(Dev d, Geo g, int guard) {
     d.sorted[pos] = (int32_t)b;
     d.skey[pos] = k;
   }
-}
-
-__device__ __forceinline__ bool excluded(const int32_t *bond_fil, int32_t a,
-                                         int32_t b) {
-  // Bond::HasNeighbor within one mesh (src/bond.cpp:118-121,
-  // src/interaction_manager.cpp:352-367)
-  int32_t df = a - b;
-  return (df == 1 || df == -1) && bond_fil[a] == bond_fil[b];
 }
 
 // q-th cell of the half-shell stencil of (x, y, z) in the order of
@@ -323,36 +315,6 @@ __device__ __forceinline__ int32_t shell_key(int n, int x, int y, int z, int q) 
   return ((int32_t)xp * n + yp) * n + zp;
 }
 
-// The 13 neighbour runs of home cell k, loaded up front so that the 26
-// lookups are in flight together.
-struct Shell {
-  int32_t o0[13], o1[13];
-};
-
-// Empty cells (most of them) are recognised from the occupancy mask, a
-// small array that stays in L2, so only occupied neighbours read the large
-// cell_start array.
-__device__ __forceinline__ void load_shell(const Dev &d, const Geo &g, int32_t k, Shell *sh) {
-  const int n = g.n1d;
-  const int x = k / (n * n), y = (k / n) % n, z = k % n;
-  int32_t nbk[13];
-  unsigned occ = 0;
-#pragma unroll
-  for (int q = 0; q < 13; ++q) {
-    nbk[q] = shell_key(n, x, y, z, q);
-    occ |= ((d.cell_occ[nbk[q] >> 3] >> (nbk[q] & 7)) & 1u) << q;
-  }
-#pragma unroll
-  for (int q = 0; q < 13; ++q) {
-    sh->o0[q] = 0;
-    sh->o1[q] = 0;
-    if ((occ >> q) & 1u) {
-      sh->o0[q] = d.cell_start[nbk[q]];
-      sh->o1[q] = d.cell_start[nbk[q] + 1];
-    }
-  }
-}
-
 // q-th cell of the mirrored half shell: the 13 cells whose half shell holds
 // (x, y, z), i.e. the offsets of shell_key negated (wrapped).  With at
 // least 3 cells per side the 13 + 13 + 1 cells are distinct.
@@ -378,202 +340,175 @@ __device__ __forceinline__ int32_t back_shell_key(int n, int x, int y, int z, in
   return ((int32_t)xp * n + yp) * n + zp;
 }
 
-// The runs of the 13 mirrored half-shell cells of k (load_shell's
-// counterpart).
-__device__ __forceinline__ void load_back_shell(const Dev &d, const Geo &g, int32_t k,
-                                                Shell *sh) {
+// Run [start, end) of cell c in sorted / sorted_o, empty for an unoccupied
+// cell.  Most cells are empty; they are recognised from the occupancy mask,
+// a small array that stays in L2, so only occupied ones read the large
+// cell_start array.
+__device__ __forceinline__ void cell_run(const Dev &d, int32_t c, int32_t *s, int32_t *e) {
+  *s = 0;
+  *e = 0;
+  if ((d.cell_occ[c >> 3] >> (c & 7)) & 1u) {
+    *s = d.cell_start[c];
+    *e = d.cell_start[c + 1];
+  }
+}
+
+// The 13 half-shell cells (fw) or mirrored half-shell cells (bw) of k with
+// their runs, loaded up front so that the lookups are in flight together.
+struct Shell {
+  int32_t key[13], o0[13], o1[13];
+};
+
+template <bool BACK>
+__device__ __forceinline__ void load_shell(const Dev &d, const Geo &g, int32_t k, Shell *sh) {
   const int n = g.n1d;
   const int x = k / (n * n), y = (k / n) % n, z = k % n;
-  int32_t nbk[13];
-  unsigned occ = 0;
-#pragma unroll
-  for (int q = 0; q < 13; ++q) {
-    nbk[q] = back_shell_key(n, x, y, z, q);
-    occ |= ((d.cell_occ[nbk[q] >> 3] >> (nbk[q] & 7)) & 1u) << q;
-  }
-#pragma unroll
-  for (int q = 0; q < 13; ++q) {
-    sh->o0[q] = 0;
-    sh->o1[q] = 0;
-    if ((occ >> q) & 1u) {
-      sh->o0[q] = d.cell_start[nbk[q]];
-      sh->o1[q] = d.cell_start[nbk[q] + 1];
-    }
-  }
-}
-
-// End of the run of cell k that starts at sorted position p, from the
-// sorted keys (sequential reads; runs are short).
-__device__ __forceinline__ int32_t run_end(const Dev &d, const Geo &g, int64_t p, int32_t k) {
-  int64_t e = p + 1;
-  while (e < g.nb && d.skey[e] == k) ++e;
-  return (int32_t)e;
-}
-
-// Candidate counts (Cell::MakePairs, src/cell.cpp:14-70), excluded pairs
-// already removed (the reference culls them on the rebuild step,
-// src/interaction_manager.cpp:485-497).  One thread per sorted position,
-// i.e. per bond b, with no atomics:
-//  - places b in its cell's run of sorted_o by its rank in bond order
-//    (Cell::AddObj appends in interactor order, src/cell.cpp:5-8);
-//  - counts the candidates that list b first: run members of higher bond
-//    index and the bonds of the 13 half-shell cells; a run's total, the
-//    sum over its positions, is what its first position emits, so the scan
-//    of these counts gives each run's offset into the candidate list;
-//  - counts the candidates that list b second (run members of lower index,
-//    the mirrored half shell) and writes b's degree, the length of its
-//    incidence list;
-// and the tile's sum goes to tsum for the candidate-offset scan.
-__global__ __launch_bounds__(kScanBlock) void k_count(Dev d, Geo g) {
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  __shared__ int wsum[kScanBlock / 64];
-  const int64_t t = blockIdx.x;
-  const int64_t p = t * kScanBlock + threadIdx.x;
-  int32_t cnt = 0;
-  if (p < g.nb) {
-    const int32_t k = d.skey[p];
-    const int32_t b = d.sorted[p];
-    int64_t s0 = p;
-    while (s0 > 0 && d.skey[s0 - 1] == k) --s0;
-    const int32_t e = run_end(d, g, p, k);
-    Shell fw, bw;
-    load_shell(d, g, k, &fw);
-    load_back_shell(d, g, k, &bw);
-    int32_t rank = 0, first = 0, second = 0;
-    for (int32_t i = (int32_t)s0; i < e; ++i) {
-      const int32_t o = d.sorted[i];
-      const int ok = o != b && !excluded(d.bond_fil, b, o);
-      if (o < b) {
-        ++rank;
-        second += ok;
-      } else {
-        first += ok;
-      }
-    }
-    d.sorted_o[s0 + rank] = b;
-#pragma unroll
-    for (int q = 0; q < 13; ++q) {
-      for (int32_t j = fw.o0[q]; j < fw.o1[q]; ++j) first += !excluded(d.bond_fil, b, d.sorted[j]);
-      for (int32_t j = bw.o0[q]; j < bw.o1[q]; ++j)
-        second += !excluded(d.bond_fil, b, d.sorted[j]);
-    }
-    cnt = first;
-    d.ccount[p] = first;
-    d.deg[b] = first + second;
-  }
-  int tot;
-  block_excl_scan256(cnt, &tot, wsum);
-  if (threadIdx.x == 0) d.tsum[t] = tot;
-}
-
-// Writes the candidate list in MakePairs order: home cells ascending; in
-// each, the self pairs (i<j) then, for each of the 13 half-shell
-// neighbours in AssignCellNeighbors order, all home x neighbour pairs; runs
-// read in bond order from sorted_o.
-//
-// One block per tile of kScanBlock sorted positions, the tiles of k_count,
-// so the candidate offsets need no separate scan: the tile sums before the
-// block's tile plus the exclusive scan of its own counts (the sums, a few
-// thousand entries, are read from L2 by every block; their total is the
-// list length).
-__global__ __launch_bounds__(kScanBlock) void k_emit(Dev d, Geo g, int prec) {
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  __shared__ int wsum[kScanBlock / 64];
-  const int64_t t = blockIdx.x;
-  const int64_t ntiles = (g.nb + kScanBlock - 1) / kScanBlock;
-  int before = 0, all = 0;
-  for (int64_t i = threadIdx.x; i < ntiles; i += kScanBlock) {
-    const int v = d.tsum[i];
-    all += v;
-    before += i < t ? v : 0;
-  }
-  int carry, total;
-  block_excl_scan256(before, &carry, wsum);
-  block_excl_scan256(all, &total, wsum);
-  const int64_t p = t * kScanBlock + threadIdx.x;
-  int tile_tot;
-  const int32_t off = carry + block_excl_scan256(p < g.nb ? d.ccount[p] : 0, &tile_tot, wsum);
-  // capacity check (one thread records it): an overflowing list aborts the
-  // step, the host grows the buffers and re-runs it
-  const int64_t need = total;
-  if (t == 0 && threadIdx.x == 0) {
-    d.ctl->nc_need = need;
-    if (need > g.cap) {
-      d.ctl->abort = 1;
-      d.ctl->overflow = 1;
-    } else {
-      d.ctl->nc = need;
-    }
-  }
-  if (need > g.cap) return;
-  if (p >= g.nb) return;
-  const int32_t k = d.skey[p];
-  if (p > 0 && d.skey[p - 1] == k) return;  // not the first of its run
-  const int32_t e = run_end(d, g, p, k);
-  Shell sh;
-  load_shell(d, g, k, &sh);
-  int32_t w = off;
-  auto put = [&](int32_t b1, int32_t b2) {
-    if (excluded(d.bond_fil, b1, b2)) return;
-    d.pairs[w] = make_int2(b1, b2);
-    if (prec == SC_PREC_FP64) {
-      d.slot64[w] = make_double4(0.0, 0.0, 0.0, 0.0);
-    } else {
-      d.slot32[w] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    ++w;
-  };
-  for (int32_t i = (int32_t)p; i < e; ++i)
-    for (int32_t j = i + 1; j < e; ++j) put(d.sorted_o[i], d.sorted_o[j]);
 #pragma unroll
   for (int q = 0; q < 13; ++q)
-    for (int32_t i = (int32_t)p; i < e; ++i) {
-      const int32_t bi = d.sorted_o[i];
-      for (int32_t j = sh.o0[q]; j < sh.o1[q]; ++j) put(bi, d.sorted_o[j]);
-    }
+    sh->key[q] = BACK ? back_shell_key(n, x, y, z, q) : shell_key(n, x, y, z, q);
+#pragma unroll
+  for (int q = 0; q < 13; ++q) cell_run(d, sh->key[q], &sh->o0[q], &sh->o1[q]);
 }
 
-// Incidence CSR: every candidate appears under obj1 (role 0) and obj2
-// (role 1); entries are {c << 1 | role, partner bond}.
-__global__ void k_inc_fill(Dev d, Geo g) {
+// The bonds b-1 and b+1 when they share a site with b (Bond::HasNeighbor
+// within one mesh, src/bond.cpp:118-121): the pairs the reference culls on
+// the rebuild step (src/interaction_manager.cpp:485-497), never listed here.
+struct Excl {
+  int32_t lo, hi;  // b-1 / b+1, or -1
+};
+
+__device__ __forceinline__ Excl excl_of(const Dev &d, const Geo &g, int32_t b) {
+  const int32_t f = d.bond_fil[b];
+  Excl x;
+  x.lo = (b > 0 && d.bond_fil[b - 1] == f) ? b - 1 : -1;
+  x.hi = (b + 1 < g.nb && d.bond_fil[b + 1] == f) ? b + 1 : -1;
+  return x;
+}
+
+// Degree of every bond: the candidates that list it (Cell::MakePairs,
+// src/cell.cpp:14-70): the other members of its cell, the members of its 13
+// half-shell cells and of its 13 mirrored half-shell cells, less the
+// site-sharing neighbours among them.  One thread per sorted position, no
+// atomics; it also places its bond in its cell's run of sorted_o by its
+// rank in bond order (Cell::AddObj appends in interactor order,
+// src/cell.cpp:5-8), the order the pair lists are written in.
+__global__ __launch_bounds__(256) void k_deg(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  int64_t nc = d.ctl->nc;
-  int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += st) {
-    int2 pr = d.pairs[c];
-    int32_t p1 = d.inc_start[pr.x] + atomicSub(&d.deg[pr.x], 1) - 1;
-    int32_t p2 = d.inc_start[pr.y] + atomicSub(&d.deg[pr.y], 1) - 1;
-    d.inc[p1] = make_int2((int32_t)(c << 1), pr.y);
-    d.inc[p2] = make_int2((int32_t)((c << 1) | 1), pr.x);
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.nb) return;
+  const int32_t k = d.skey[p];
+  const int32_t b = d.sorted[p];
+  const int32_t s0 = d.cell_start[k], e = d.cell_start[k + 1];
+  const Excl x = excl_of(d, g, b);
+  const int32_t klo = x.lo >= 0 ? d.keys[x.lo] : -1;
+  const int32_t khi = x.hi >= 0 ? d.keys[x.hi] : -1;
+  Shell fw, bw;
+  load_shell<false>(d, g, k, &fw);
+  load_shell<true>(d, g, k, &bw);
+  int32_t rank = 0;
+  for (int32_t i = s0; i < e; ++i) rank += d.sorted[i] < b;
+  d.sorted_o[s0 + rank] = b;
+  int32_t deg = (e - s0 - 1) - (klo == k) - (khi == k);
+#pragma unroll
+  for (int q = 0; q < 13; ++q) {
+    deg += (fw.o1[q] - fw.o0[q]) - (klo == fw.key[q]) - (khi == fw.key[q]);
+    deg += (bw.o1[q] - bw.o0[q]) - (klo == bw.key[q]) - (khi == bw.key[q]);
   }
+  d.deg[b] = deg;
 }
 
-// Per-bond incidence lists in candidate order, so the reduction adds the
-// contributions in the order ApplyPairInteractions does; the same launch
-// restarts the displacement bookkeeping, InteractionManager::ZeroDrTot ->
-// Mesh/Object::ZeroDrTot (src/interaction_manager.cpp:316-321,
-// src/mesh.cpp:546-551).
-__global__ void k_inc_sort(Dev d, Geo g) {
+// Incidence lists in candidate order, one thread per bond, no atomics and
+// no sort.  The list order of CellList::MakePairs (src/cell_list.cpp:
+// 127-137, src/cell.cpp:14-70) is: home cells in key order; inside a home
+// cell its self pairs (i < j, in member order), then for each of its 13
+// half-shell cells in stencil order all (home member, neighbour member)
+// pairs, home member outer.  So bond b's candidates come from its own cell
+// (as obj2 of the self pairs with earlier members, as obj1 of those with
+// later members, then as obj1 of its rows of the 13 half-shell blocks) and
+// from each occupied mirrored half-shell cell m (as obj2 of m's block
+// toward b's cell, one pair per member of m, in member order), the groups
+// taken in home-key order.  Entries {partner, role}; the slot copies start
+// unwritten (zero force).  The launch also checks the capacity, commits the
+// list length and restarts the displacement bookkeeping
+// (InteractionManager::ZeroDrTot -> Mesh/Object::ZeroDrTot,
+// src/interaction_manager.cpp:316-321, src/mesh.cpp:546-551).
+__global__ __launch_bounds__(256) void k_pull(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t n = g.ns > g.nb ? g.ns : g.nb;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    if (i == 0) d.ctl->n_rebuilds += 1;
-    if (i < g.ns) {
-      for (int q = 0; q < 3; ++q) d.dr0[q * g.ns + i] = d.r[q * g.ns + i];
-      d.acc[i] = 0.0;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = d.inc_start[g.nb];  // two incidences per candidate
+  const bool fits = total >= 0 && total <= 2 * g.cap;
+  if (i == 0) {
+    d.ctl->nc_need = total >= 0 ? total / 2 : 2 * g.cap + 1;
+    if (fits) {
+      d.ctl->nc = total / 2;
+      d.ctl->n_rebuilds += 1;
+    } else {
+      d.ctl->abort = 1;
+      d.ctl->overflow = 1;
     }
-    if (i >= g.nb) continue;
-    const int32_t s = d.inc_start[i], e = d.inc_start[i + 1];
-    for (int32_t a = s + 1; a < e; ++a) {
-      const int2 v = d.inc[a];
-      int32_t j = a - 1;
-      while (j >= s && d.inc[j].x > v.x) {
-        d.inc[j + 1] = d.inc[j];
-        --j;
+  }
+  if (!fits) return;
+  if (i < g.ns) {
+    for (int q = 0; q < 3; ++q) d.dr0[q * g.ns + i] = d.r[q * g.ns + i];
+    d.acc[i] = 0.0;
+  }
+  if (i >= g.nb) return;
+  const int32_t b = (int32_t)i;
+  int32_t w = d.inc_start[b];
+  const int32_t wend = d.inc_start[b + 1];
+  if (w < 0 || wend < w || wend > total) {  // a wrapped count: refuse, never write
+    d.ctl->abort = 1;
+    d.ctl->overflow = 1;
+    return;
+  }
+  const int32_t k = d.keys[b];
+  const Excl x = excl_of(d, g, b);
+  int32_t s0, e;
+  cell_run(d, k, &s0, &e);
+  Shell fw, bw;
+  load_shell<false>(d, g, k, &fw);
+  load_shell<true>(d, g, k, &bw);
+  auto put = [&](int32_t a, int role) {
+    if (a == x.lo || a == x.hi) return;
+    if (w < wend) d.inc[w] = make_int2(a, role);
+    ++w;
+  };
+  // home cells in key order: b's own cell k and the occupied mirrored
+  // cells, by repeated selection of the next larger key (13 is small)
+  int32_t last = -1;
+  for (int step = 0; step < 14; ++step) {
+    int32_t next = 0x7fffffff;
+    int which = -2;  // -1: own cell, q: mirrored cell q
+    if (k > last) {
+      next = k;
+      which = -1;
+    }
+#pragma unroll
+    for (int q = 0; q < 13; ++q)
+      if (bw.o1[q] > bw.o0[q] && bw.key[q] > last && bw.key[q] < next) {
+        next = bw.key[q];
+        which = q;
       }
-      d.inc[j + 1] = v;
+    if (which == -2) break;
+    last = next;
+    if (which == -1) {
+      int32_t r = s0;
+      while (r < e && d.sorted_o[r] != b) ++r;
+      for (int32_t j = s0; j < r; ++j) put(d.sorted_o[j], 1);
+      for (int32_t j = r + 1; j < e; ++j) put(d.sorted_o[j], 0);
+#pragma unroll
+      for (int q = 0; q < 13; ++q)
+        for (int32_t j = fw.o0[q]; j < fw.o1[q]; ++j) put(d.sorted_o[j], 0);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 13; ++q)
+        if (q == which)
+          for (int32_t j = bw.o0[q]; j < bw.o1[q]; ++j) put(d.sorted_o[j], 1);
     }
+  }
+  if (w != wend) {  // degree and list disagree: never expected; fail the step
+    d.ctl->abort = 1;
+    d.ctl->err = SC_ERR_STATE;
   }
 }
 
@@ -587,24 +522,19 @@ static inline unsigned sgrid(int64_t n) {
   return b < (unsigned)kStreamBlocks ? b : (unsigned)kStreamBlocks;
 }
 
-// keys -> cell scan (2) -> scatter -> count (+ run order, degrees, tile
-// sums) -> emit (+ candidate offsets) -> degree scan (2) -> incidence fill
-// -> incidence order (+ displacement restart): 10 launches.  (Running the
-// degree scan on a forked side stream beside the emit measured slower in
-// graph replay than the launches in a row.)
+// keys (+ cell populations) -> cell scan (2) -> scatter -> degrees (+ run
+// order) -> degree scan (2) -> incidence lists (+ capacity check,
+// displacement restart): 8 launches.
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
   const unsigned gb = nblocks(g.nb, kScanBlock);
   hipLaunchKernelGGL(k_keys, dim3(gb), dim3(kScanBlock), 0, st, d, g);
   launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, d.ctl,
               kGuardRebuild, st, d.cell_occ);
   hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kScanBlock), 0, st, d, g, (int)kGuardRebuild);
-  hipLaunchKernelGGL(k_count, dim3(gb), dim3(kScanBlock), 0, st, d, g);
-  hipLaunchKernelGGL(k_emit, dim3(gb), dim3(kScanBlock), 0, st, d, g,
-                     d.slot64 ? (int)SC_PREC_FP64 : (int)SC_PREC_MIXED);
+  hipLaunchKernelGGL(k_deg, dim3(nblocks(g.nb, 256)), dim3(256), 0, st, d, g);
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
-  hipLaunchKernelGGL(k_inc_fill, dim3(2048), dim3(kScanBlock), 0, st, d, g);
-  hipLaunchKernelGGL(k_inc_sort, dim3(nblocks(g.ns > g.nb ? g.ns : g.nb, kScanBlock)),
-                     dim3(kScanBlock), 0, st, d, g);
+  hipLaunchKernelGGL(k_pull, dim3(nblocks(g.ns > g.nb ? g.ns : g.nb, 256)), dim3(256), 0, st,
+                     d, g);
 }
 
 // Cell membership (cell_start, sorted) of the last rebuild from the stored

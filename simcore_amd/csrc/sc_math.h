@@ -51,14 +51,56 @@ __device__ __forceinline__ int32_t cell_key(int n1d, double sx, double sy,
   return ((int32_t)x * n1d + y) * n1d + z;
 }
 
-// Mixed-precision bond record, 32 bytes (4 doubles of storage per bond):
-// float4 {offset of the scaled position inside its cell, in cell units,
-// x, y, z; length} and float4 {unit vector; cell coordinates x | y << 10 |
-// z << 20 as bits}.  The scaled position comes back in FP64 as
-// (c + offset) / n1d - 1/2, within about 6e-8 of a cell (7e-8 sigma for
-// cells of 1.13 sigma); the cell coordinates, hence the cell key, are those
-// cell_key computes from the FP64 scaled position.  At most 1024 cells per
-// side (checked at creation).
+// Mixed-precision bond record, 32 bytes (4 doubles of storage per bond),
+// two uint4: {offset of the scaled position inside its cell, x, y, z, in
+// units of 2^-32 cell; length, FP32 high part} and {unit vector in
+// octahedral coordinates, two 32-bit fixed-point values over [-1, 1]; cell
+// coordinates x | y << 10 | z << 20; length, FP32 low part}.  Decoded in
+// FP64: the scaled position (c + offset) / n1d - 1/2 within 2^-32 of a cell
+// (3e-10 sigma for cells of 1.13 sigma), the unit vector within about 1e-9,
+// the length to about 1e-14 relative; the cell coordinates, hence the cell
+// key, are those cell_key computes from the FP64 scaled position.  At most
+// 1024 cells per side (checked at creation).
+__host__ __device__ __forceinline__ uint32_t fix32(double f, double scale) {
+  double v = floor(f * scale + 0.5);
+  v = v < 0.0 ? 0.0 : (v > 4294967295.0 ? 4294967295.0 : v);
+  return (uint32_t)v;
+}
+
+// octahedral map of a unit vector to [-1, 1]^2 (the lower hemisphere folded
+// over the diagonals) and back
+__host__ __device__ __forceinline__ void oct_encode(const double *u, uint32_t *ox,
+                                                    uint32_t *oy) {
+  const double a = fabs(u[0]) + fabs(u[1]) + fabs(u[2]);
+  double px = u[0] / a, py = u[1] / a;
+  if (u[2] < 0.0) {
+    const double qx = (1.0 - fabs(py)) * (px >= 0.0 ? 1.0 : -1.0);
+    const double qy = (1.0 - fabs(px)) * (py >= 0.0 ? 1.0 : -1.0);
+    px = qx;
+    py = qy;
+  }
+  *ox = fix32(px + 1.0, 2147483648.0);
+  *oy = fix32(py + 1.0, 2147483648.0);
+}
+
+__host__ __device__ __forceinline__ void oct_decode(uint32_t ox, uint32_t oy, double *u) {
+  const double px = (double)ox * (1.0 / 2147483648.0) - 1.0;
+  const double py = (double)oy * (1.0 / 2147483648.0) - 1.0;
+  const double z = 1.0 - fabs(px) - fabs(py);
+  const double x = z < 0.0 ? (1.0 - fabs(py)) * (px >= 0.0 ? 1.0 : -1.0) : px;
+  const double y = z < 0.0 ? (1.0 - fabs(px)) * (py >= 0.0 ? 1.0 : -1.0) : py;
+  const double n2 = x * x + y * y + z * z;  // in [1/3, 1]
+#ifdef __HIP_DEVICE_COMPILE__
+  double r = __builtin_amdgcn_rsq(n2);
+  r = r * (1.5 - 0.5 * n2 * r * r);
+#else
+  const double r = 1.0 / sqrt(n2);
+#endif
+  u[0] = x * r;
+  u[1] = y * r;
+  u[2] = z * r;
+}
+
 template <class GeoT>
 __device__ __forceinline__ void rec_write_compact(double *br, const GeoT &g, int64_t b,
                                                   const double *sp, const double *u,
@@ -66,19 +108,23 @@ __device__ __forceinline__ void rec_write_compact(double *br, const GeoT &g, int
   double t[3];
   int c[3];
   for (int k = 0; k < 3; ++k) c[k] = cell_coord(g.n1d, sp[k], &t[k]);
-  float4 *rec = reinterpret_cast<float4 *>(br + 4 * b);
-  rec[0] = make_float4((float)(t[0] - c[0]), (float)(t[1] - c[1]), (float)(t[2] - c[2]),
-                       (float)l);
-  rec[1] = make_float4((float)u[0], (float)u[1], (float)u[2],
-                       __int_as_float(c[0] | (c[1] << 10) | (c[2] << 20)));
+  uint32_t ox, oy;
+  oct_encode(u, &ox, &oy);
+  const float lh = (float)l;
+  const float ll = (float)(l - (double)lh);
+  uint4 *rec = reinterpret_cast<uint4 *>(br + 4 * b);
+  rec[0] = make_uint4(fix32(t[0] - c[0], 4294967296.0), fix32(t[1] - c[1], 4294967296.0),
+                      fix32(t[2] - c[2], 4294967296.0), __float_as_uint(lh));
+  rec[1] = make_uint4(ox, oy, (uint32_t)(c[0] | (c[1] << 10) | (c[2] << 20)),
+                      __float_as_uint(ll));
 }
 
 // cell key of a compact record (the key cell_key gives its scaled position)
 template <class GeoT>
 __device__ __forceinline__ int32_t rec_key_compact(const double *br, const GeoT &g,
                                                    int64_t b) {
-  const float4 v = reinterpret_cast<const float4 *>(br + 4 * b)[1];
-  const int packed = __float_as_int(v.w);
+  const uint4 v = reinterpret_cast<const uint4 *>(br + 4 * b)[1];
+  const int packed = (int)v.z;
   const int x = packed & 1023, y = (packed >> 10) & 1023, z = (packed >> 20) & 1023;
   return ((int32_t)x * g.n1d + y) * g.n1d + z;
 }
@@ -86,16 +132,15 @@ __device__ __forceinline__ int32_t rec_key_compact(const double *br, const GeoT 
 // scaled position s, unit vector u and length l of bond b: compact record
 __device__ __forceinline__ void rec_read_compact(const double *br, double inv_n1d, int64_t b,
                                                  double *s, double *u, double *l) {
-  const float4 *rec = reinterpret_cast<const float4 *>(br + 4 * b);
-  const float4 a = rec[0], v = rec[1];
-  const int packed = __float_as_int(v.w);
-  s[0] = ((double)(packed & 1023) + (double)a.x) * inv_n1d - 0.5;
-  s[1] = ((double)((packed >> 10) & 1023) + (double)a.y) * inv_n1d - 0.5;
-  s[2] = ((double)((packed >> 20) & 1023) + (double)a.z) * inv_n1d - 0.5;
-  u[0] = v.x;
-  u[1] = v.y;
-  u[2] = v.z;
-  *l = a.w;
+  const uint4 *rec = reinterpret_cast<const uint4 *>(br + 4 * b);
+  const uint4 a = rec[0], v = rec[1];
+  const int packed = (int)v.z;
+  constexpr double k32 = 1.0 / 4294967296.0;
+  s[0] = ((double)(packed & 1023) + (double)a.x * k32) * inv_n1d - 0.5;
+  s[1] = ((double)((packed >> 10) & 1023) + (double)a.y * k32) * inv_n1d - 0.5;
+  s[2] = ((double)((packed >> 20) & 1023) + (double)a.z * k32) * inv_n1d - 0.5;
+  oct_decode(v.x, v.y, u);
+  *l = (double)__uint_as_float(a.w) + (double)__uint_as_float(v.w);
 }
 
 // the same from a 64-byte FP64 record
@@ -372,6 +417,21 @@ __device__ __forceinline__ void noise_xi(uint64_t seed, int64_t site,
   philox4x32(c, (uint32_t)seed, (uint32_t)(seed >> 32));
   for (int k = 0; k < 3; ++k)
     xi[k] = ((double)c[k] + 0.5) * 2.3283064365386962890625e-10 - 0.5;
+}
+
+// The three draws of `site` (caller numbering) at midstep m: from the
+// caller's buffer when one is set (zero outside the midsteps it holds; the
+// host never steps past them), else from the counter-based stream.
+template <class GeoT>
+__device__ __forceinline__ void draw_xi(const GeoT &g, int64_t site, int64_t m, double *xi) {
+  if (g.xi_buf) {
+    const int64_t j = m - g.xi_m0;
+    const bool ok = j >= 0 && j < g.xi_nm;
+    const double *p = g.xi_buf + (ok ? (j * g.ns + site) * 3 : 0);
+    for (int k = 0; k < 3; ++k) xi[k] = ok ? p[k] : 0.0;
+  } else {
+    noise_xi(g.seed, site, m, xi);
+  }
 }
 
 // Bond::ReInit + Object::UpdatePeriodic (src/bond.cpp:22-37,

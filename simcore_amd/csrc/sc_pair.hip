@@ -146,37 +146,40 @@ __device__ __forceinline__ int64_t logical_block(int xcd) {
   return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
-// Contribution of one incidence {candidate << 1 | role, partner} to its
-// bond ("own"): the pair computed with the candidate's own (obj1, obj2)
-// orientation, the sticky slot updated when in range, then +F and
-// contact1 x F for obj1, -F and -(contact2 x F) for obj2
-// (src/interaction_manager.cpp:448-517).  Both owners of a candidate run
-// the same instructions on the same inputs and obtain the same bits.
+// Contribution of incidence q {partner, role | written << 1} to its bond
+// ("own"): the pair computed with the candidate's own (obj1, obj2)
+// orientation, the incidence's slot copy updated when in range (and marked
+// written), then +F and contact1 x F for obj1, -F and -(contact2 x F) for
+// obj2 (src/interaction_manager.cpp:448-517).  Both owners of a candidate
+// run the same instructions on the same inputs and obtain the same bits, so
+// their slot copies stay equal; out of range the copy's force is used, 0
+// if it was not written since the rebuild (the reference's fresh pair
+// record).
 template <bool MIXED>
 __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const BondRec &own,
-                                             int2 ent, const BondRec &oth, double *cf,
-                                             double *ct) {
-  const int32_t c = ent.x >> 1;
-  const bool second = (ent.x & 1) != 0;  // this bond is obj2 of candidate c
+                                             int32_t q, int2 ent, const BondRec &oth,
+                                             double *cf, double *ct) {
+  const bool second = (ent.y & 1) != 0;  // this bond is obj2 of the candidate
+  const bool written = (ent.y & 2) != 0;
   const BondRec &r1 = second ? oth : own;
   const BondRec &r2 = second ? own : oth;
-  double fc[3], lam, mu;
+  double fc[3] = {0.0, 0.0, 0.0}, lam, mu;
+  bool wrote = false;
   if (!MIXED) {
     double dr[3], dr2;
     sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
-    bool wrote = false;
     if (!(dr2 > g.rcut2)) {
       double f[3];
       if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
         fc[0] = f[0];
         fc[1] = f[1];
         fc[2] = f[2];
-        d.slot64[c] = make_double4(f[0], f[1], f[2], 0.0);
+        d.islot64[q] = make_double4(f[0], f[1], f[2], 0.0);
         wrote = true;
       }
     }
-    if (!wrote) {
-      const double4 o = d.slot64[c];
+    if (!wrote && written) {
+      const double4 o = d.islot64[q];
       fc[0] = o.x;
       fc[1] = o.y;
       fc[2] = o.z;
@@ -190,7 +193,6 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
       sep[k] = g.h[4 * k] * ds;
     }
     sphero_mixed(sep, r1.u, 0.5 * r1.len, r2.u, 0.5 * r2.len, dr, &dr2, &lam, &mu);
-    bool wrote = false;
     if (!(dr2 > g.rcut2)) {
       float4 f4;
       if (dr2 < 1.0e-12) {
@@ -204,19 +206,20 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
         wrote = true;
       }
       if (wrote) {
-        d.slot32[c] = f4;
+        d.islot32[q] = f4;
         fc[0] = f4.x;
         fc[1] = f4.y;
         fc[2] = f4.z;
       }
     }
-    if (!wrote) {
-      const float4 o = d.slot32[c];
+    if (!wrote && written) {
+      const float4 o = d.islot32[q];
       fc[0] = o.x;
       fc[1] = o.y;
       fc[2] = o.z;
     }
   }
+  if (wrote && !written) reinterpret_cast<int32_t *>(d.inc)[2 * (int64_t)q + 1] = ent.y | 2;
   // contact of this bond and the torque cross product
   // (src/interaction_manager.cpp:469-470, src/minimum_distance.cpp:339-342)
   double con[3], t[3];
@@ -276,17 +279,17 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   BondRec oth_n{};
   if (q0 < q1) {
     ent_n = d.inc[q0];
-    oth_n = load_rec<MIXED>(d.br, g, ent_n.y);
+    oth_n = load_rec<MIXED>(d.br, g, ent_n.x);
   }
   for (int32_t q = q0; q < q1; ++q) {
     const int2 ent = ent_n;
     const BondRec oth = oth_n;
     if (q + 1 < q1) {
       ent_n = d.inc[q + 1];
-      oth_n = load_rec<MIXED>(d.br, g, ent_n.y);
+      oth_n = load_rec<MIXED>(d.br, g, ent_n.x);
     }
     double cf[3], ct[3];
-    pair_contrib<MIXED>(d, g, own, ent, oth, cf, ct);
+    pair_contrib<MIXED>(d, g, own, q, ent, oth, cf, ct);
     for (int k = 0; k < 3; ++k) F[k] += cf[k];
     for (int k = 0; k < 3; ++k) T[k] += ct[k];
   }
@@ -344,9 +347,9 @@ __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
         }
         const BondRec own = load_rec<MIXED>(d.br, g, b0 + lo);
         const int2 ent = d.inc[q];
-        const BondRec oth = load_rec<MIXED>(d.br, g, ent.y);
+        const BondRec oth = load_rec<MIXED>(d.br, g, ent.x);
         double cf[3], ct[3];
-        pair_contrib<MIXED>(d, g, own, ent, oth, cf, ct);
+        pair_contrib<MIXED>(d, g, own, q, ent, oth, cf, ct);
         for (int k = 0; k < 3; ++k) {
           cF[q - cs][k] = cf[k];
           cT[q - cs][k] = ct[k];

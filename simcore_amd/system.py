@@ -328,6 +328,18 @@ class FilamentSystem:
                self.h, "sc_theta_histogram")
         return hist
 
+    def set_noise(self, xi=None, seed=None):
+        """Caller-supplied thermal noise (sc_set_noise): `xi` holds the draws
+        of the next len(xi) / (3 n_sites) midsteps, 3 per site, sites in
+        caller order, midstep-major; without `xi` the counter stream is used,
+        keyed by `seed` (default: the key the system was created with)."""
+        if xi is None:
+            key = self.params.seed if seed is None else int(seed)
+            _check(lib.sc_set_noise(self.h, 0, key, None, 0), self.h, "sc_set_noise")
+            return
+        xi = np.ascontiguousarray(xi, dtype=np.float64).ravel()
+        _check(lib.sc_set_noise(self.h, 1, 0, _dp(xi), xi.size), self.h, "sc_set_noise")
+
     def set_sites(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         _check(lib.sc_set_sites(self.h, _dp(r)), self.h, "sc_set_sites")

@@ -38,12 +38,12 @@ def test_initial_cell_keys_and_candidates_bit_exact():
 
 
 def test_mixed_compact_bond_records_at_load():
-    """Mixed path: 32-byte bond records (FP32 offset inside the bond's cell,
-    packed cell coordinates, FP32 orientation and length).  Against the FP64
-    records of the CPU path: scaled positions within 4e-8 of a cell (1/35
-    of the box per cell here), orientations and lengths to FP32 rounding,
-    identical cell keys and candidates, centres from the FP64 sites
-    bit-identical."""
+    """Mixed path: 32-byte bond records (32-bit fixed-point offset inside the
+    bond's cell, packed cell coordinates, octahedral orientation, two-float
+    length).  Against the FP64 records of the CPU path: scaled positions
+    within 1e-9 of a cell (1/35 of the box per cell here), orientations
+    within 5e-9, lengths to 1e-12 relative, identical cell keys and
+    candidates, centres from the FP64 sites bit-identical."""
     print("case: C0 64x20 seed 7 mixed, load-time records", flush=True)
     p, fil, dev, ora = _pair(precision="mixed")
     dpos, dsp, du, dln = dev.bonds()
@@ -52,10 +52,11 @@ def test_mixed_compact_bond_records_at_load():
     d = np.abs(dsp - osp)
     d = np.minimum(d, 1.0 - d)  # periodic in scaled units
     n1d = dev.counters()["n_cells_1d"]
-    print("max scaled-position error in cells", d.max() * n1d)
-    assert d.max() * n1d <= 4e-8
-    assert np.abs(du - ou).max() <= 1e-7
-    np.testing.assert_allclose(dln, oln, rtol=1e-7)
+    print("max scaled-position error in cells", d.max() * n1d,
+          "orientation", np.abs(du - ou).max(), "length", np.abs(dln / oln - 1).max())
+    assert d.max() * n1d <= 1e-9
+    assert np.abs(du - ou).max() <= 5e-9
+    np.testing.assert_allclose(dln, oln, rtol=1e-12)
     np.testing.assert_array_equal(dev.cell_keys(), ora.cell_keys())
     ob1, ob2 = culled(*ora.candidates(), fil.site_off)
     db1, db2 = dev.candidates()
@@ -114,13 +115,41 @@ def test_graph_replay_matches_eager():
     assert eager.counters() == graph.counters()
 
 
-def test_capacity_growth_is_exact():
-    print("case: C0 fp64 tiny candidate capacity", flush=True)
-    _, _, ref, _ = _pair()
-    _, _, small, _ = _pair(capacity_factor=0.01)
-    ref.step(40)
-    small.step(40)
+@pytest.mark.parametrize("precision,spatial", [("fp64", False), ("mixed", True)])
+def test_capacity_growth_mid_run_is_exact(precision, spatial):
+    """A candidate capacity that holds the load-time list exactly overflows
+    at the first rebuild after the box's contents are compressed by 0.75
+    (about 2.4x the candidates): the step aborts on the device, the host
+    grows the buffers, recaptures the graphs and re-runs the step.  Sites,
+    tensions, counters and candidates stay bit-identical to a run that
+    never overflowed, in graph mode, and the growth happened mid-run."""
+    print(f"case: C0 {precision} capacity = load-time list, compressed after step 4",
+          flush=True)
+    p, fil = c0_system()
+    ref = sc.FilamentSystem(p, fil, precision=precision, use_graph=True, spatial_order=spatial)
+    n0 = ref.counters()["n_candidates"]
+    small = sc.FilamentSystem(p, fil, precision=precision, use_graph=True,
+                              spatial_order=spatial, capacity_factor=-float(n0))
+    b0 = small.device_bytes()
+    assert b0 < ref.device_bytes()
+    ref.step(4)
+    small.step(4)
+    assert small.device_bytes() == b0
+    r = ref.sites()[0] * 0.75
+    ref.set_sites(r)
+    small.set_sites(r)
+    ref.step(1)
+    small.step(1)
+    c = small.counters()
+    assert c["n_rebuilds"] == 2 and c["n_candidates"] > 2 * n0
+    assert small.device_bytes() > b0, "the in-step growth path did not run"
+    ref.step(35)
+    small.step(35)
     for a, b in zip(ref.sites(), small.sites()):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(ref.tensions(), small.tensions())
+    assert ref.counters() == small.counters()
+    for a, b in zip(ref.candidates(), small.candidates()):
         np.testing.assert_array_equal(a, b)
 
 
@@ -347,8 +376,11 @@ def test_autotune_leaves_trajectory_unchanged(precision):
     b.step(31)
     rep = b.autotune(steps=6)
     print(rep)
-    # 8 pair shapes, then 3 waves per block x 0-3 side streams
-    assert rep["n_trials"] == 20 and rep["step_ms_best"] <= rep["step_ms_start"]
+    # 8 pair shapes, then 3 waves per block x side streams; the mixed path's
+    # packed integrator is one launch here (no filament above 128 sites), so
+    # side streams are not tried
+    expect = 20 if precision == "fp64" else 11
+    assert rep["n_trials"] == expect and rep["step_ms_best"] <= rep["step_ms_start"]
     assert b.counters()["i_step"] == 31
     a.step(40)
     b.step(40)
@@ -356,7 +388,7 @@ def test_autotune_leaves_trajectory_unchanged(precision):
         np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 @pytest.mark.parametrize("kind", ["short", "long"])
 def test_integrator_layouts_within_tolerance(kind, layout):
     """Both integrator layouts for filaments of up to 64 sites (one site per
@@ -381,6 +413,32 @@ def test_integrator_layouts_within_tolerance(kind, layout):
     d100 = np.abs(dev.sites()[0] - ora.sites()[0]).max()
     print("max |dr| after 100 steps", d100)
     assert d100 <= 1e-5
+
+
+@pytest.mark.parametrize("spatial", [False, True])
+@pytest.mark.parametrize("kind", ["short", "long", "c0"])
+def test_packed_integrator_matches_length_bins(kind, spatial):
+    """The packed layout (every filament of up to 128 sites in a lane segment
+    of its own width, several lengths per wavefront, lane moves chosen per
+    wavefront) runs the length-binned layout's arithmetic: sites, tensions
+    and counters bit-identical over 41 steps, with and without side streams."""
+    print(f"case: {kind} mixed spatial={spatial}, packed vs binned integrator, 41 steps",
+          flush=True)
+    from tests.helpers import long_system, uneven_system
+    p, fil = {"c0": c0_system, "short": uneven_system, "long": long_system}[kind]()
+    runs = []
+    for layout, streams in ((1, 0), (2, 0), (2, 1)):
+        s = sc.FilamentSystem(p, fil, precision="mixed", use_graph=True, spatial_order=spatial)
+        s.set_launch_shape(pair_block=0, integrate_waves=2, integrate_layout=layout,
+                           integrate_streams=streams)
+        s.step(41)
+        runs.append((s.sites(), s.tensions(), s.counters()))
+        s.close()
+    for j in (1, 2):
+        for x, y in zip(runs[0][0], runs[j][0]):
+            np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(runs[0][1], runs[j][1])
+        assert runs[0][2] == runs[j][2]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

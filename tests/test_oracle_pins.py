@@ -452,3 +452,23 @@ def test_candidate_order_is_the_reference_walk(radius, n1d):
     walk = S.reference_pair_walk(keys, n1d)
     assert len(walk) > 1000
     assert list(zip(b1.tolist(), b2.tolist())) == walk
+
+
+def test_noise_buffer_of_zeros_is_the_noiseless_step():
+    """A caller buffer of zero draws gives zero random forces: the same
+    trajectory, bit for bit, as stoch_flag off (src/filament.cpp:518-548 with
+    xi = 0), and a buffer equal to the counter draws the default one."""
+    p, fil = c0_system(num=12, seed=4)
+    quiet = sc.params_init(p.system_radius, delta=p.delta, min_cell_length=p.min_cell_length,
+                           stoch_flag=0, seed=p.seed)
+    a = O.OracleSystem(quiet, fil, n_threads=1)
+    b = O.OracleSystem(p, fil, n_threads=1)
+    b.set_noise_buffer(np.zeros(15 * 3 * fil.ns), 15)
+    assert a.step(30) == 0 and b.step(30) == 0
+    np.testing.assert_array_equal(a.sites()[0], b.sites()[0])
+    c = O.OracleSystem(p, fil, n_threads=1)
+    d = O.OracleSystem(p, fil, n_threads=1)
+    xi = np.array([O.noise(p.seed, s, m) for m in range(15) for s in range(fil.ns)]).ravel()
+    d.set_noise_buffer(xi, 15)
+    assert c.step(30) == 0 and d.step(30) == 0
+    np.testing.assert_array_equal(c.sites()[0], d.sites()[0])
