@@ -161,6 +161,25 @@ int sc_get_counters(sc_handle h, int64_t *i_step, int64_t *n_rebuilds,
  * unphysical-dr error path).  Bond state is re-initialised from them. */
 int sc_set_sites(sc_handle h, const double *r);
 
+/* Thermodynamics (the thermo_flag output of the reference).  While enabled,
+ * every pair pass also sums each live candidate's stress -dr_i F_j
+ * (WCAPotential / MaxForcePotential::CalcPotential, include/simcore/
+ * wca_potential.hpp:28-31, include/simcore/max_force_potential.hpp:35-38;
+ * sticky like the force) into InteractionManager::stress_ and each bond's
+ * pair potential energy (Object::AddPotential of both bonds,
+ * src/interaction_manager.cpp:500-531).  sc_thermo_enable starts a fresh
+ * stress sum.  sc_calculate_pressure is InteractionManager::
+ * CalculatePressure (src/interaction_manager.cpp:534-559): pressure tensor
+ * delta_ij n_bonds / V + stress_ij / (V n_thermo), pressure its trace / 3,
+ * V the unit cell's volume; it resets the sum.  sc_get_bond_energies: each
+ * bond's energy of the last step.  Both return SC_ERR_STATE while disabled.
+ * Mixed precision sums the stress from FP32 slot values (about 1e-7
+ * relative). */
+int sc_thermo_enable(sc_handle h, int on);
+int sc_calculate_pressure(sc_handle h, int32_t n_thermo, double *pressure_tensor,
+                          double *pressure, double *volume);
+int sc_get_bond_energies(sc_handle h, double *pote);
+
 /* Thermal noise.  The reference draws xi = RandomUniform() - 0.5 three
  * times per site on every midstep from each filament's GSL generator
  * (Filament::ConstructUnprojectedRandomForces, src/filament.cpp:518-548).
@@ -436,6 +455,16 @@ int sc_spec_frame(const char *path, int32_t n_filaments, const int32_t *site_off
 int sc_posit_frame(const char *path, const sc_params *params,
                    int32_t n_filaments, const int32_t *site_off,
                    const double *r, const sc_filament_params *fil);
+
+/* The reference's .thermo file (OutputManagerBase::InitThermo /
+ * WriteThermo, include/simcore/output_manager.hpp:156-183): header int
+ * n_steps, int n_thermo, double delta, int n_dim; per frame double[9] unit
+ * cell, double[9] pressure tensor, double pressure, double volume (pair
+ * with sc_calculate_pressure). */
+int sc_thermo_header(const char *path, int32_t n_steps, int32_t n_thermo, double delta,
+                     int32_t n_dim);
+int sc_thermo_frame(const char *path, const double *unit_cell, const double *pressure_tensor,
+                    double pressure, double volume);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,11 @@ struct orc_sys {
   int32_t *c1, *c2;
   unsigned char *no_ix;
   double *cf[3], *ct1[3], *ct2[3];
+  /* Interaction::stress / pote of each pair (sticky like the force), the
+   * InteractionManager's stress_ sum and each bond's p_energy_ */
+  double *cst[9], *cpote;
+  double stress_acc[9];
+  double *bpote;
   /* state */
   int64_t rebuilds, i_step, n_midsteps;
   /* caller-supplied noise: xi of midsteps [xi_m0, xi_m0 + xi_nm), 3 per
@@ -447,6 +452,8 @@ static int ensure_cap(orc_sys *s, int64_t need) {
     s->ct1[i] = realloc(s->ct1[i], sizeof(double) * nc);
     s->ct2[i] = realloc(s->ct2[i], sizeof(double) * nc);
   }
+  for (int i = 0; i < 9; ++i) s->cst[i] = realloc(s->cst[i], sizeof(double) * nc);
+  s->cpote = realloc(s->cpote, sizeof(double) * nc);
   s->cap = nc;
   return 0;
 }
@@ -513,6 +520,8 @@ static void rebuild(orc_sys *s) {
     memset(s->ct1[i], 0, sizeof(double) * (size_t)s->nc);
     memset(s->ct2[i], 0, sizeof(double) * (size_t)s->nc);
   }
+  for (int i = 0; i < 9; ++i) memset(s->cst[i], 0, sizeof(double) * (size_t)s->nc);
+  memset(s->cpote, 0, sizeof(double) * (size_t)s->nc);
   memset(s->no_ix, 0, (size_t)s->nc);
   s->i_update = 0;
   s->rebuilds++;
@@ -558,6 +567,7 @@ orc_sys *orc_create(const orc_params *p, int nf, const int32_t *site_off,
   }
   int64_t ns = s->ns, nb = s->nb;
   s->bond_fil = xcalloc((size_t)nb, sizeof(int32_t));
+  s->bpote = xcalloc((size_t)nb, sizeof(double));
   s->bond_idx = xcalloc((size_t)nb, sizeof(int32_t));
   double **fp[] = {&s->bl, &s->kappa, &s->gpar, &s->gperp, &s->diam,
                    &s->sig_par, &s->sig_perp, &s->dr_tot};
@@ -657,6 +667,9 @@ void orc_destroy(orc_sys *s) {
   free(s->c1);
   free(s->c2);
   free(s->no_ix);
+  for (int k = 0; k < 9; ++k) free(s->cst[k]);
+  free(s->cpote);
+  free(s->bpote);
   free(s->xb);
   free(s->xlam);
   free(s->xf);
@@ -690,7 +703,22 @@ static void process_pair(orc_sys *s, int64_t c) {
     sphero(s->p.h, s1, u1, s->len[b1], s2, u2, s->len[b2], dr, &dr_mag2, c1v,
            c2v);
     if (!(dr_mag2 > s->rcut2)) {
-      pair_force(s->c12, s->c6, s->p.f_cutoff, dr, dr_mag2, c1v, c2v, force);
+      if (pair_force(s->c12, s->c6, s->p.f_cutoff, dr, dr_mag2, c1v, c2v, force)) {
+        /* stress_ij = -dr_i F_j; pote of the WCA branch, 0 for the
+         * max-force branch (include/simcore/wca_potential.hpp:28-31,
+         * include/simcore/max_force_potential.hpp:35-38) */
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) s->cst[3 * i + j][c] = -dr[i] * force[j];
+        double pote = 0.0;
+        if (!(dr_mag2 < 1e-12)) {
+          double rmag = sqrt(dr_mag2);
+          double rinv = 1.0 / (rmag);
+          double rinv2 = rinv * rinv;
+          double r6 = rinv2 * rinv2 * rinv2;
+          pote = r6 * (s->c12 * r6 - s->c6) + s->p.wca_eps;
+        }
+        s->cpote[c] = pote;
+      }
       for (int i = 0; i < 3; ++i) s->cf[i][c] = force[i];
     }
   }
@@ -1191,18 +1219,24 @@ int orc_step(orc_sys *s, int nsteps) {
           s->ct1[d][w] = s->ct1[d][c];
           s->ct2[d][w] = s->ct2[d][c];
         }
+        for (int k = 0; k < 9; ++k) s->cst[k][w] = s->cst[k][c];
+        s->cpote[w] = s->cpote[c];
         w++;
       }
       s->nc = w;
     }
     /* ApplyPairInteractions: serial scatter (src/interaction_manager.cpp:
      * 500-517) */
+    memset(s->bpote, 0, sizeof(double) * (size_t)s->nb);
     for (int64_t c = 0; c < s->nc; ++c) {
       int32_t b1 = s->c1[c], b2 = s->c2[c];
       for (int d = 0; d < 3; ++d) s->bf[d][b1] += s->cf[d][c];
       for (int d = 0; d < 3; ++d) s->bf[d][b2] -= s->cf[d][c];
       for (int d = 0; d < 3; ++d) s->bt[d][b1] += s->ct1[d][c];
       for (int d = 0; d < 3; ++d) s->bt[d][b2] -= s->ct2[d][c];
+      s->bpote[b1] += s->cpote[c];
+      s->bpote[b2] += s->cpote[c];
+      for (int k = 0; k < 9; ++k) s->stress_acc[k] += s->cst[k][c];
     }
     s->i_update++;
     /* FilamentSpecies::UpdatePositions (src/filament_species.cpp:59-85) */
@@ -1272,6 +1306,20 @@ void orc_get_dr(const orc_sys *s, double *acc, double *dr_max_per_fil) {
   if (acc) memcpy(acc, s->acc, sizeof(double) * (size_t)s->ns);
   if (dr_max_per_fil) memcpy(dr_max_per_fil, s->dr_tot, sizeof(double) * (size_t)s->nf);
 }
+/* The stress_ sum since the last call (InteractionManager::
+ * ApplyPairInteractions adds every pair's stress every step,
+ * src/interaction_manager.cpp:500-517; CalculatePressure reads and resets
+ * it, :534-559), and each bond's potential energy of the last step. */
+void orc_take_stress(orc_sys *s, double *out) {
+  for (int k = 0; k < 9; ++k) {
+    out[k] = s->stress_acc[k];
+    s->stress_acc[k] = 0.0;
+  }
+}
+void orc_get_bond_energies(const orc_sys *s, double *out) {
+  memcpy(out, s->bpote, sizeof(double) * (size_t)s->nb);
+}
+
 /* Injects a pre-drawn noise stream: xi for the next n_mid midsteps, 3 per
  * site in site order (the draws RandomUniform() - 0.5 of
  * src/filament.cpp:527-547); n_mid = 0 returns to the counter stream. */

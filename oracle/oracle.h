@@ -93,6 +93,8 @@ void orc_get_bond_forces(const orc_sys *s, double *f, double *t);
 void orc_get_tensions(const orc_sys *s, double *tension);   /* per bond  */
 void orc_get_dr(const orc_sys *s, double *acc, double *dr_max_per_fil);
 void orc_set_noise_buffer(orc_sys *s, const double *xi, int64_t n_mid);
+void orc_take_stress(orc_sys *s, double *out);
+void orc_get_bond_energies(const orc_sys *s, double *out);
 void orc_set_sites(orc_sys *s, const double *r);  /* for forced-error tests */
 
 /* ---- crosslinks (harmonic springs between two bonds) ----------------------- *

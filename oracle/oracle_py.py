@@ -51,6 +51,8 @@ def _load():
         "orc_get_dr": ([V, dp, dp], None),
         "orc_set_sites": ([V, dp], None),
         "orc_set_noise_buffer": ([V, dp, c_int64], None),
+        "orc_take_stress": ([V, dp], None),
+        "orc_get_bond_energies": ([V, dp], None),
         "orc_min_distance": ([dp, dp, dp, c_double, dp, dp, c_double, dp], None),
         "orc_pair_force": ([c_double, c_double, c_double, c_double, dp, c_double, dp, dp, dp],
                            c_int),
@@ -120,6 +122,7 @@ class OracleSystem:
         self.ns = ns
         self.nf = nf
         self.nb = ns - nf
+        self._h = np.array(params.unit_cell[:9], dtype=np.float64).reshape(3, 3)
 
     def close(self):
         if self.s:
@@ -180,6 +183,46 @@ class OracleSystem:
     def set_sites(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         lib.orc_set_sites(self.s, _dp(r))
+
+    def take_stress(self):
+        """The pair stress summed over pairs and steps since the last call
+        (3x3, row-major), reset."""
+        out = np.empty(9)
+        lib.orc_take_stress(self.s, _dp(out))
+        return out.reshape(3, 3)
+
+    def thermo_enable(self, on=True):
+        """The restatement always sums the stress; enabling starts a fresh
+        sum, as on the device."""
+        self.take_stress()
+
+    def calculate_pressure(self, n_thermo):
+        """InteractionManager::CalculatePressure (src/interaction_manager.cpp:
+        534-559) over the stress summed since the last call: pressure tensor
+        delta_ij n_objs / V + stress_ij / (V n_thermo), n_objs the bond
+        count (Mesh::GetCount, src/mesh.cpp:399), V the periodic box's unit
+        cell volume (Space::UpdateVolume, src/space.cpp:233-237), the
+        cofactor expansion of Space::CalculateUnitCellQuantities
+        (src/space.cpp:104-118); pressure the trace over n_dim; resets the
+        sum."""
+        stress = self.take_stress()
+        u = [float(x) for x in self._h.ravel()]
+        vol = (u[0] * ((u[4] * u[8]) - (u[7] * u[5])) - u[1] * (u[3] * u[8] - u[6] * u[5])
+               + u[2] * (u[3] * u[7] - u[6] * u[4]))
+        inv_v = 1.0 / vol
+        pt = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                if i == j:
+                    pt[i, j] += self.nb * inv_v
+                pt[i, j] += stress[i, j] * inv_v / n_thermo
+        return {"pressure_tensor": pt, "pressure": float(np.trace(pt) / 3), "volume": vol}
+
+    def bond_energies(self):
+        """Each bond's pair potential energy of the last step."""
+        out = np.empty(self.nb)
+        lib.orc_get_bond_energies(self.s, _dp(out))
+        return out
 
     def set_noise_buffer(self, xi, n_mid):
         """Pre-drawn xi for the next n_mid midsteps (3 per site, site-major);

@@ -29,13 +29,14 @@ EXPORTS = [
     "sc_get_sites", "sc_get_bonds", "sc_get_bond_forces", "sc_get_tensions",
     "sc_get_cell_keys", "sc_get_candidates", "sc_get_slot_forces",
     "sc_get_dr_state", "sc_get_counters", "sc_set_sites", "sc_set_noise", "sc_device_bytes",
+    "sc_thermo_enable", "sc_calculate_pressure", "sc_get_bond_energies",
     "sc_profile_enable", "sc_profile_read", "sc_last_error", "sc_destroy",
     "sc_params_from_yaml", "sc_params_init", "sc_filament_layout",
     "sc_generate_filaments", "sc_copy_probe", "sc_time_pair", "sc_step_async",
     "sc_step_poll",
     "sc_filament_stats", "sc_theta_histogram", "sc_theta_histogram_noalloc",
     "sc_output_header", "sc_spec_frame",
-    "sc_posit_frame", "sc_checkpoint_save", "sc_checkpoint_load", "sc_autotune",
+    "sc_posit_frame", "sc_thermo_header", "sc_thermo_frame", "sc_checkpoint_save", "sc_checkpoint_load", "sc_autotune",
     "sc_set_launch_shape", "sc_get_launch_shape",
     "sc_load_crosslinks", "sc_get_crosslinks", "sc_generate_crosslinks",
     "sc_ensemble_create", "sc_ensemble_submit", "sc_ensemble_progress",
@@ -145,6 +146,9 @@ def _load():
         "sc_get_counters": ([H, lp, lp, lp, ip], c_int),
         "sc_set_sites": ([H, dp], c_int),
         "sc_set_noise": ([H, c_int, c_uint64, dp, c_size_t], c_int),
+        "sc_thermo_enable": ([H, c_int], c_int),
+        "sc_calculate_pressure": ([H, c_int32, dp, dp, dp], c_int),
+        "sc_get_bond_energies": ([H, dp], c_int),
         "sc_filament_stats": ([H, dp, dp], c_int),
         "sc_checkpoint_save": ([H, c_void_p, c_size_t, POINTER(c_size_t)], c_int),
         "sc_checkpoint_load": ([H, c_void_p, c_size_t], c_int),
@@ -178,6 +182,8 @@ def _load():
         "sc_params_from_yaml": ([c_char_p, POINTER(ScParams), dp], c_int),
         "sc_params_init": ([POINTER(ScParams), c_double], c_int),
         "sc_output_header": ([c_char_p, c_int32, c_int32, c_double], c_int),
+        "sc_thermo_header": ([c_char_p, c_int32, c_int32, c_double, c_int32], c_int),
+        "sc_thermo_frame": ([c_char_p, dp, dp, c_double, c_double], c_int),
         "sc_spec_frame": ([c_char_p, c_int32, ip, dp, POINTER(ScFilamentParams)], c_int),
         "sc_posit_frame": ([c_char_p, POINTER(ScParams), c_int32, ip, dp,
                             POINTER(ScFilamentParams)], c_int),

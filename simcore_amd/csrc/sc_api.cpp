@@ -285,7 +285,7 @@ void enqueue_step(sc_system *h, bool odd) {
     (void)hipEventRecord(h->ev[0], st);
   }
   if (!odd) sc::launch_xlink_forces(d, g, st);
-  sc::launch_pair(d, g, h->p.precision, h->tune, st);
+  sc::launch_pair(d, g, h->p.precision, h->tune, st, g.thermo != 0);
   if (!odd) {
     sc::launch_xlink_events(d, g, st);
     if (g.xpoff > 0) sc::launch_xlink_lists(d, g, st);
@@ -414,6 +414,10 @@ void to_caller(const sc_system *h, const std::vector<int64_t> &map, const T *src
 }
 
 // device SoA site/bond array -> caller-order host array
+// Blocks of the pair pass at its smallest block size (64 bonds): each owns
+// one stress slot, whatever launch shape is set while the sum accumulates.
+int64_t stress_slots(int64_t nb) { return nb / 64 + 2; }
+
 int fetch_mapped(sc_system *h, const double *dev, double *out, int64_t n, int comp,
                  bool sites) {
   std::vector<double> tmp((size_t)comp * n);
@@ -665,6 +669,7 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
   g.rcut2 = rcut * rcut;
   g.c12 = 4.0 * p.wca_eps * std::pow(p.wca_sig, 12.0);
   g.c6 = 4.0 * p.wca_eps * std::pow(p.wca_sig, 6.0);
+  g.eps = p.wca_eps;
   g.fcut = p.f_cutoff;
   g.delta = p.delta;
   const int n1d = cells_per_side(p);
@@ -923,6 +928,8 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     if ((rc = dalloc(h, &d.bt32, 3 * (size_t)nb, false))) return rc;
   }
   if ((rc = dalloc(h, &d.tension, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.bpote, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.stress_acc, 6 * (size_t)stress_slots(nb), false))) return rc;
   if ((rc = dalloc(h, &d.keys, nb, false))) return rc;
   if ((rc = dalloc(h, &d.cell_count, g.n_cells, false))) return rc;
   if ((rc = dalloc(h, &d.cell_start, g.n_cells + 1, false))) return rc;
@@ -977,6 +984,8 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     HIPCHK(h, hipMemsetAsync(d.bt32, 0, sizeof(float) * 3 * nb, h->stream));
   }
   HIPCHK(h, hipMemsetAsync(d.tension, 0, sizeof(double) * nb, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.bpote, 0, sizeof(double) * nb, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.stress_acc, 0, sizeof(double) * 6 * stress_slots(nb), h->stream));
   HIPCHK(h, hipMemsetAsync(d.cell_count, 0, sizeof(int32_t) * g.n_cells, h->stream));
   HIPCHK(h, hipMemsetAsync(d.deg, 0, sizeof(int32_t) * nb, h->stream));
   Ctl c{};
@@ -1312,6 +1321,72 @@ int sc_get_tensions(sc_handle h, double *tension) {
   if (!tension) return fail(h, SC_ERR_ARG, "null output");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return fetch_mapped(h, h->d.tension, tension, h->g.nb, 1, false);
+}
+
+int sc_thermo_enable(sc_handle h, int on) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const int want = on ? 1 : 0;
+  if (want && !h->g.thermo)
+    HIPCHK(h, hipMemsetAsync(h->d.stress_acc, 0,
+                             sizeof(double) * 6 * stress_slots(h->g.nb), h->stream));
+  if (want != h->g.thermo) drop_graph(h);  // kernels take the system constants by value
+  h->g.thermo = want;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return SC_OK;
+}
+
+int sc_calculate_pressure(sc_handle h, int32_t n_thermo, double *pressure_tensor,
+                          double *pressure, double *volume) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (n_thermo < 1) return fail(h, SC_ERR_ARG, "need n_thermo >= 1");
+  if (!h->g.thermo) return fail(h, SC_ERR_STATE, "thermodynamics not enabled");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const int64_t slots = stress_slots(h->g.nb);
+  std::vector<double> acc(6 * (size_t)slots);
+  HIPCHK(h, hipMemcpy(acc.data(), h->d.stress_acc, sizeof(double) * acc.size(),
+                      hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemsetAsync(h->d.stress_acc, 0, sizeof(double) * acc.size(), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double s6[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t b = 0; b < slots; ++b)
+    for (int k = 0; k < 6; ++k) s6[k] += acc[6 * b + k];
+  // the pair stress -dr_i F_j is symmetric (F is parallel to dr)
+  const double S[9] = {s6[0], s6[3], s6[4], s6[3], s6[1], s6[5], s6[4], s6[5], s6[2]};
+  // InteractionManager::CalculatePressure (src/interaction_manager.cpp:
+  // 534-559): the bond count over the volume on the diagonal plus the
+  // stress over volume and n_thermo; Space::UpdateVolume of the periodic
+  // box, the unit cell's determinant (src/space.cpp:233-237)
+  const double *hc = h->g.h;
+  const double V = hc[0] * (hc[4] * hc[8] - hc[5] * hc[7]) - hc[1] * (hc[3] * hc[8] - hc[5] * hc[6]) +
+                   hc[2] * (hc[3] * hc[7] - hc[4] * hc[6]);
+  const double inv_V = 1.0 / V;
+  double P[9], p = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double v = 0.0;
+      if (i == j) v += (double)h->g.nb * inv_V;
+      v += S[3 * i + j] * inv_V / n_thermo;
+      P[3 * i + j] = v;
+    }
+  for (int i = 0; i < 3; ++i) p += P[4 * i];
+  p /= 3;
+  if (pressure_tensor) std::memcpy(pressure_tensor, P, sizeof(P));
+  if (pressure) *pressure = p;
+  if (volume) *volume = V;
+  return SC_OK;
+}
+
+int sc_get_bond_energies(sc_handle h, double *pote) {
+  int rc = check_handle(h, true);
+  if (rc) return rc;
+  if (!pote) return fail(h, SC_ERR_ARG, "null output");
+  if (!h->g.thermo) return fail(h, SC_ERR_STATE, "thermodynamics not enabled");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return fetch_mapped(h, h->d.bpote, pote, h->g.nb, 1, false);
 }
 
 int sc_get_cell_keys(sc_handle h, int32_t *keys) {
@@ -1697,6 +1772,9 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   // stream instead (their trajectories are discarded)
   const double *xi_keep = h->g.xi_buf;
   h->g.xi_buf = nullptr;
+  // nor do they add to the stress sum or the bond energies
+  const int thermo_keep = h->g.thermo;
+  h->g.thermo = 0;
   // time `steps` steps of configuration t from the saved state
   auto trial = [&](const sc::Tune &t, double *ms) -> int {
     h->tune = t;
@@ -1771,6 +1849,7 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   h->tune = rc ? start : best;
   drop_graph(h);
   h->g.xi_buf = xi_keep;
+  h->g.thermo = thermo_keep;
   const int rc2 = sc_checkpoint_load(h, blob.data(), blob.size());
   h->prof = prof;
   (void)hipEventDestroy(e0);
@@ -1863,10 +1942,10 @@ int sc_time_pair(sc_handle h, int32_t launches, double *avg_ms) {
   }
   // the pass rewrites the same slot values and sums on an unchanged state,
   // so repeating it leaves the system as it was
-  sc::launch_pair(h->d, h->g, h->p.precision, h->tune, h->stream);  // warm
+  sc::launch_pair(h->d, h->g, h->p.precision, h->tune, h->stream, false);  // warm
   (void)hipEventRecord(e0, h->stream);
   for (int k = 0; k < launches; ++k)
-    sc::launch_pair(h->d, h->g, h->p.precision, h->tune, h->stream);
+    sc::launch_pair(h->d, h->g, h->p.precision, h->tune, h->stream, false);
   (void)hipEventRecord(e1, h->stream);
   const hipError_t e = hipEventSynchronize(e1);
   float ms = 0.f;

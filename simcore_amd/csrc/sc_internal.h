@@ -45,7 +45,8 @@ struct Ctl {
 struct Geo {
   double h[9];
   double hinv[9];
-  double c12, c6, rcut2, fcut;
+  double c12, c6, rcut2, fcut, eps;
+  int thermo;              // pair pass also sums stress and bond energies
   double delta;
   double thr;              // 0.25 * cell_len^2
   int n1d;
@@ -113,6 +114,10 @@ struct Dev {
   double *xlam, *xf;
   int32_t *xinc_start, *xinc, *xdeg;
   int32_t *bond_caller;    // storage bond -> caller bond index
+  // thermodynamics (sc_thermo_enable): per pair-pass block, the sum of the
+  // pair stress tensors xx yy zz xy xz yz over the steps since the last read;
+  // per bond, its pair potential energy of the last step
+  double *stress_acc, *bpote;
 };
 
 // Launch shapes the autotuner chooses among; none changes a result.
@@ -157,8 +162,10 @@ void launch_decide(const Dev &d, const Geo &g, hipStream_t st);
 void launch_commit(const Dev &d, const Geo &g, hipStream_t st);
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st);
 void launch_cells_from_keys(const Dev &d, const Geo &g, hipStream_t st);
+// `thermo`: also sum the pair stress and the bond energies (Geo::thermo)
 void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t,
-                 hipStream_t st);
+                 hipStream_t st, bool thermo);
+int64_t pair_blocks(const Geo &g, const Tune &t);  // blocks of one pair launch
 // `mid`: the step is a midstep (odd step index)
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
                       int bin_count, int bin_maxn, int precision, const Tune &t, bool mid,

@@ -329,11 +329,14 @@ __device__ __forceinline__ void sphero64(const double *h, const double *s_1,
 // 92-103) -> MaxForcePotential when dr^2 < 1e-12
 // (include/simcore/max_force_potential.hpp:11-40), else WCAPotential
 // (include/simcore/wca_potential.hpp:14-32).  Returns 1 if force written.
+// *rs receives the pair's signed distance: +|dr| on the WCA branch (the
+// force is antiparallel to dr, so stress_ij = -dr_i F_j = (r/|F|) F_i F_j),
+// -|dr| on the max-force branch (force parallel to dr).
 __device__ __forceinline__ int pair_force64(double c12, double c6,
                                             double fcut, double *dr,
                                             double dr2, double lam, double mu,
                                             const double *u1, const double *u2,
-                                            double *force) {
+                                            double *force, double *rs) {
   if (dr2 < 1e-12) {
     double c1[3], c2[3];
     for (int i = 0; i < 3; ++i) {
@@ -348,9 +351,11 @@ __device__ __forceinline__ int pair_force64(double c12, double c6,
     if (rmag < 1e-12) return 0;
     double rinv = 1.0 / (rmag);
     for (int i = 0; i < 3; ++i) force[i] = fcut * dr[i] * rinv;
+    *rs = -rmag;
     return 1;
   }
   double rmag = sqrt(dr2);
+  *rs = rmag;
   double rinv = 1.0 / (rmag);
   double rinv2 = rinv * rinv;
   double r6 = rinv2 * rinv2 * rinv2;
@@ -417,6 +422,19 @@ __device__ __forceinline__ void noise_xi(uint64_t seed, int64_t site,
   philox4x32(c, (uint32_t)seed, (uint32_t)(seed >> 32));
   for (int k = 0; k < 3; ++k)
     xi[k] = ((double)c[k] + 0.5) * 2.3283064365386962890625e-10 - 0.5;
+}
+
+// Potential energy of a pair at signed distance rs (pair_force64): the WCA
+// value r6 (c12 r6 - c6) + eps with the reference's operations
+// (include/simcore/wca_potential.hpp:15-31), 0 on the max-force branch
+// (include/simcore/max_force_potential.hpp:38) or for a pair never inside
+// the cutoff (rs = 0).
+__device__ __forceinline__ double pair_pote(double c12, double c6, double eps, double rs) {
+  if (!(rs > 0.0)) return 0.0;
+  double rinv = 1.0 / (rs);
+  double rinv2 = rinv * rinv;
+  double r6 = rinv2 * rinv2 * rinv2;
+  return r6 * (c12 * r6 - c6) + eps;
 }
 
 // The three draws of `site` (caller numbering) at midstep m: from the

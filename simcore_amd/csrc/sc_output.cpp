@@ -17,6 +17,10 @@
 //     into [-1/2, 1/2) (Object::UpdatePeriodic), double[3] normalised sum of
 //     site orientations (Mesh::GetAvgOrientation, src/mesh.cpp:598-608),
 //     double diameter, double length
+//   thermo file (OutputManagerBase::InitThermo / WriteThermo,
+//          include/simcore/output_manager.hpp:156-183): header int n_steps,
+//     int n_thermo, double delta, int n_dim; per frame double[9] unit cell,
+//     double[9] pressure tensor, double pressure, double volume
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -117,6 +121,27 @@ int sc_posit_frame(const char *path, const sc_params *params, int32_t nf,
     for (int k = 0; k < 3 && ok; ++k) ok = o.put(u[k]);
     ok = ok && o.put(fil[f].diameter) && o.put(length);
   }
+  return ok ? SC_OK : SC_ERR_ARG;
+}
+
+int sc_thermo_header(const char *path, int32_t n_steps, int32_t n_thermo, double delta,
+                     int32_t n_dim) {
+  if (!path) return SC_ERR_ARG;
+  File o(path, "wb");
+  if (!o.f) return SC_ERR_ARG;
+  if (!o.put(n_steps) || !o.put(n_thermo) || !o.put(delta) || !o.put(n_dim)) return SC_ERR_ARG;
+  return SC_OK;
+}
+
+int sc_thermo_frame(const char *path, const double *unit_cell, const double *pressure_tensor,
+                    double pressure, double volume) {
+  if (!path || !unit_cell || !pressure_tensor) return SC_ERR_ARG;
+  File o(path, "ab");
+  if (!o.f) return SC_ERR_ARG;
+  bool ok = true;
+  for (int i = 0; i < 9 && ok; ++i) ok = o.put(unit_cell[i]);
+  for (int i = 0; i < 9 && ok; ++i) ok = o.put(pressure_tensor[i]);
+  ok = ok && o.put(pressure) && o.put(volume);
   return ok ? SC_OK : SC_ERR_ARG;
 }
 

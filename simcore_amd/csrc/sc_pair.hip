@@ -129,7 +129,8 @@ __device__ __forceinline__ float4 wca_mixed(const Geo &g, const double *rmin, do
   double ffac = -(12.0 * g.c12 * r6 - 6.0 * g.c6) * r6 * rinv;
   if (dabs(ffac) > g.fcut) ffac = (ffac >= 0.0 ? 1.0 : -1.0) * g.fcut;
   const double sc = ffac * rinv;
-  return make_float4((float)(sc * rmin[0]), (float)(sc * rmin[1]), (float)(sc * rmin[2]), 0.f);
+  return make_float4((float)(sc * rmin[0]), (float)(sc * rmin[1]), (float)(sc * rmin[2]),
+                     (float)(rm2 * rinv));
 }
 
 }  // namespace
@@ -155,26 +156,31 @@ __device__ __forceinline__ int64_t logical_block(int xcd) {
 // their slot copies stay equal; out of range the copy's force is used, 0
 // if it was not written since the rebuild (the reference's fresh pair
 // record).
+// The slot's fourth value is the pair's signed distance (pair_force64):
+// with the force it gives the pair's stress and potential energy, so the
+// sticky record needs nothing else.  `rs_out` receives it (0 for a slot not
+// written since the rebuild), `fc_out` the candidate's force.
 template <bool MIXED>
 __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const BondRec &own,
                                              int32_t q, int2 ent, const BondRec &oth,
-                                             double *cf, double *ct) {
+                                             double *cf, double *ct, double *fc_out,
+                                             double *rs_out) {
   const bool second = (ent.y & 1) != 0;  // this bond is obj2 of the candidate
   const bool written = (ent.y & 2) != 0;
   const BondRec &r1 = second ? oth : own;
   const BondRec &r2 = second ? own : oth;
-  double fc[3] = {0.0, 0.0, 0.0}, lam, mu;
+  double fc[3] = {0.0, 0.0, 0.0}, lam, mu, rs = 0.0;
   bool wrote = false;
   if (!MIXED) {
     double dr[3], dr2;
     sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
     if (!(dr2 > g.rcut2)) {
       double f[3];
-      if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f)) {
+      if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f, &rs)) {
         fc[0] = f[0];
         fc[1] = f[1];
         fc[2] = f[2];
-        d.islot64[q] = make_double4(f[0], f[1], f[2], 0.0);
+        d.islot64[q] = make_double4(f[0], f[1], f[2], rs);
         wrote = true;
       }
     }
@@ -183,6 +189,7 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
       fc[0] = o.x;
       fc[1] = o.y;
       fc[2] = o.z;
+      rs = o.w;
     }
   } else {
     double sep[3], dr[3], dr2;
@@ -199,8 +206,8 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
         // MaxForcePotential branch (include/simcore/max_force_potential.hpp:
         // 11-40), FP64 as the reference; practically never taken
         double f[3];
-        wrote = pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f) != 0;
-        f4 = make_float4((float)f[0], (float)f[1], (float)f[2], 0.f);
+        wrote = pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f, &rs) != 0;
+        f4 = make_float4((float)f[0], (float)f[1], (float)f[2], (float)rs);
       } else {
         f4 = wca_mixed(g, dr, dr2);
         wrote = true;
@@ -210,6 +217,7 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
         fc[0] = f4.x;
         fc[1] = f4.y;
         fc[2] = f4.z;
+        rs = f4.w;
       }
     }
     if (!wrote && written) {
@@ -217,8 +225,13 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
       fc[0] = o.x;
       fc[1] = o.y;
       fc[2] = o.z;
+      rs = o.w;
     }
   }
+  fc_out[0] = fc[0];
+  fc_out[1] = fc[1];
+  fc_out[2] = fc[2];
+  *rs_out = rs;
   if (wrote && !written) reinterpret_cast<int32_t *>(d.inc)[2 * (int64_t)q + 1] = ent.y | 2;
   // contact of this bond and the torque cross product
   // (src/interaction_manager.cpp:469-470, src/minimum_distance.cpp:339-342)
@@ -251,49 +264,93 @@ __device__ __forceinline__ void store_bond_sums(const Dev &d, int64_t nb, int64_
   }
 }
 
-template <bool MIXED>
+// Stress of one candidate, xx yy zz xy xz yz: -dr_i F_j of
+// WCAPotential::CalcPotential (include/simcore/wca_potential.hpp:28-30),
+// from the force and the signed distance (dr = -(rs/|F|) F).
+__device__ __forceinline__ void add_stress(const double *f, double rs, double *S) {
+  const double f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+  if (!(f2 > 0.0)) return;
+  const double k = rs / sqrt(f2);
+  S[0] += k * f[0] * f[0];
+  S[1] += k * f[1] * f[1];
+  S[2] += k * f[2] * f[2];
+  S[3] += k * f[0] * f[1];
+  S[4] += k * f[0] * f[2];
+  S[5] += k * f[1] * f[2];
+}
+
+// A block's stress sum, added to its slot of the accumulator (each block
+// owns one slot, so no atomics); blocks of several wavefronts combine their
+// wavefronts' sums through LDS.
+__device__ __forceinline__ void block_stress(const Dev &d, double *S) {
+  __shared__ double part[8][6];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    for (int o = 32; o > 0; o >>= 1) S[k] += __shfl_xor(S[k], o, 64);
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) part[w][k] = S[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += part[i][threadIdx.x];
+    d.stress_acc[6 * (int64_t)blockIdx.x + threadIdx.x] += t;
+  }
+}
+
+template <bool MIXED, bool THERMO>
 __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
   if (d.ctl->abort) return;
   const int64_t nb = g.nb;
   const int64_t b = logical_block(xcd) * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  const BondRec own = load_rec<MIXED>(d.br, g, b);
-  double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
-  if (g.nx > 0) {
-    // crosslink forces first, in crosslink order (the reference applies
-    // them inside Interact before the pair forces,
-    // src/interaction_manager.cpp:70, src/crosslink_species.cpp:262-266)
-    const int32_t x0 = d.xinc_start[b], x1 = d.xinc_start[b + 1];
-    for (int32_t q = x0; q < x1; ++q) {
-      const int32_t e = d.xinc[q];
-      const double *o = d.xf + 9 * (int64_t)(e >> 1);
-      const int a = e & 1;
-      for (int k = 0; k < 3; ++k) F[k] += a == 0 ? o[k] : -o[k];
-      for (int k = 0; k < 3; ++k) T[k] += o[3 + 3 * a + k];
+  double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (b < nb) {
+    const BondRec own = load_rec<MIXED>(d.br, g, b);
+    double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0}, E = 0.0;
+    if (g.nx > 0) {
+      // crosslink forces first, in crosslink order (the reference applies
+      // them inside Interact before the pair forces,
+      // src/interaction_manager.cpp:70, src/crosslink_species.cpp:262-266)
+      const int32_t x0 = d.xinc_start[b], x1 = d.xinc_start[b + 1];
+      for (int32_t q = x0; q < x1; ++q) {
+        const int32_t e = d.xinc[q];
+        const double *o = d.xf + 9 * (int64_t)(e >> 1);
+        const int a = e & 1;
+        for (int k = 0; k < 3; ++k) F[k] += a == 0 ? o[k] : -o[k];
+        for (int k = 0; k < 3; ++k) T[k] += o[3 + 3 * a + k];
+      }
     }
-  }
-  const int32_t q0 = d.inc_start[b], q1 = d.inc_start[b + 1];
-  // software pipeline: the next incidence's entry and partner record are in
-  // flight while the current pair is computed
-  int2 ent_n = make_int2(0, 0);
-  BondRec oth_n{};
-  if (q0 < q1) {
-    ent_n = d.inc[q0];
-    oth_n = load_rec<MIXED>(d.br, g, ent_n.x);
-  }
-  for (int32_t q = q0; q < q1; ++q) {
-    const int2 ent = ent_n;
-    const BondRec oth = oth_n;
-    if (q + 1 < q1) {
-      ent_n = d.inc[q + 1];
+    const int32_t q0 = d.inc_start[b], q1 = d.inc_start[b + 1];
+    // software pipeline: the next incidence's entry and partner record are
+    // in flight while the current pair is computed
+    int2 ent_n = make_int2(0, 0);
+    BondRec oth_n{};
+    if (q0 < q1) {
+      ent_n = d.inc[q0];
       oth_n = load_rec<MIXED>(d.br, g, ent_n.x);
     }
-    double cf[3], ct[3];
-    pair_contrib<MIXED>(d, g, own, q, ent, oth, cf, ct);
-    for (int k = 0; k < 3; ++k) F[k] += cf[k];
-    for (int k = 0; k < 3; ++k) T[k] += ct[k];
+    for (int32_t q = q0; q < q1; ++q) {
+      const int2 ent = ent_n;
+      const BondRec oth = oth_n;
+      if (q + 1 < q1) {
+        ent_n = d.inc[q + 1];
+        oth_n = load_rec<MIXED>(d.br, g, ent_n.x);
+      }
+      double cf[3], ct[3], fc[3], rs;
+      pair_contrib<MIXED>(d, g, own, q, ent, oth, cf, ct, fc, &rs);
+      for (int k = 0; k < 3; ++k) F[k] += cf[k];
+      for (int k = 0; k < 3; ++k) T[k] += ct[k];
+      if (THERMO) {
+        // Object::AddPotential on both bonds; the candidate's stress once
+        // (src/interaction_manager.cpp:500-517)
+        E += pair_pote(g.c12, g.c6, g.eps, rs);
+        if (!(ent.y & 1)) add_stress(fc, rs, S);
+      }
+    }
+    store_bond_sums<MIXED>(d, nb, b, F, T);
+    if (THERMO) d.bpote[b] = E;
   }
-  store_bond_sums<MIXED>(d, nb, b, F, T);
+  if (THERMO) block_stress(d, S);
 }
 
 // Load-balanced variant: one wavefront per 64 bonds.  A bond's candidate
@@ -305,9 +362,9 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
 // per-bond kernel, so the results are identical bit for bit.
 constexpr int kChunk = 128;
 
-template <bool MIXED>
+template <bool MIXED, bool THERMO>
 __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
-  __shared__ double cF[kChunk][3], cT[kChunk][3];
+  __shared__ double cF[kChunk][3], cT[kChunk][3], cE[THERMO ? kChunk : 1];
   __shared__ int32_t qs[65];
   if (d.ctl->abort) return;
   const int64_t nb = g.nb;
@@ -319,7 +376,8 @@ __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
   if (lane == 0) qs[64] = d.inc_start[b0 + 64 <= nb ? b0 + 64 : nb];
   __syncthreads();
   const int32_t Q0 = qs[0], Q1 = qs[64];
-  double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+  double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0}, E = 0.0;
+  double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (valid && g.nx > 0) {
     const int32_t x0 = d.xinc_start[b], x1 = d.xinc_start[b + 1];
     for (int32_t q = x0; q < x1; ++q) {
@@ -348,11 +406,15 @@ __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
         const BondRec own = load_rec<MIXED>(d.br, g, b0 + lo);
         const int2 ent = d.inc[q];
         const BondRec oth = load_rec<MIXED>(d.br, g, ent.x);
-        double cf[3], ct[3];
-        pair_contrib<MIXED>(d, g, own, q, ent, oth, cf, ct);
+        double cf[3], ct[3], fc[3], rs;
+        pair_contrib<MIXED>(d, g, own, q, ent, oth, cf, ct, fc, &rs);
         for (int k = 0; k < 3; ++k) {
           cF[q - cs][k] = cf[k];
           cT[q - cs][k] = ct[k];
+        }
+        if (THERMO) {
+          cE[q - cs] = pair_pote(g.c12, g.c6, g.eps, rs);
+          if (!(ent.y & 1)) add_stress(fc, rs, S);
         }
       }
     }
@@ -361,29 +423,43 @@ __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
     for (int32_t q = my0 > cs ? my0 : cs; q < e; ++q) {
       for (int k = 0; k < 3; ++k) F[k] += cF[q - cs][k];
       for (int k = 0; k < 3; ++k) T[k] += cT[q - cs][k];
+      if (THERMO) E += cE[q - cs];
     }
     __syncthreads();
   }
   if (valid) store_bond_sums<MIXED>(d, nb, b, F, T);
+  if (THERMO) {
+    if (valid) d.bpote[b] = E;
+    block_stress(d, S);
+  }
 }
 
-void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t, hipStream_t st) {
-  if (t.pair_bal) {
-    const int64_t blocks = (g.nb + 63) / 64;
-    const dim3 grid((unsigned)(blocks > 0 ? blocks : 1)), block(64);
-    if (precision == SC_PREC_FP64)
-      hipLaunchKernelGGL(k_pair_bal<false>, grid, block, 0, st, d, g, t.pair_xcd);
-    else
-      hipLaunchKernelGGL(k_pair_bal<true>, grid, block, 0, st, d, g, t.pair_xcd);
-    return;
-  }
-  const int bs = t.pair_block;
+int64_t pair_blocks(const Geo &g, const Tune &t) {
+  const int bs = t.pair_bal ? 64 : t.pair_block;
   const int64_t blocks = (g.nb + bs - 1) / bs;
-  const dim3 grid((unsigned)(blocks > 0 ? blocks : 1)), block(bs);
-  if (precision == SC_PREC_FP64)
-    hipLaunchKernelGGL(k_pair_fused<false>, grid, block, 0, st, d, g, t.pair_xcd);
+  return blocks > 0 ? blocks : 1;
+}
+
+void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t, hipStream_t st,
+                 bool thermo) {
+  const dim3 grid((unsigned)pair_blocks(g, t));
+  const bool fp64 = precision == SC_PREC_FP64;
+#define SC_PAIR(KERNEL, BS)                                                                   \
+  do {                                                                                        \
+    if (fp64 && thermo)                                                                      \
+      hipLaunchKernelGGL((KERNEL<false, true>), grid, dim3(BS), 0, st, d, g, t.pair_xcd);    \
+    else if (fp64)                                                                            \
+      hipLaunchKernelGGL((KERNEL<false, false>), grid, dim3(BS), 0, st, d, g, t.pair_xcd);   \
+    else if (thermo)                                                                          \
+      hipLaunchKernelGGL((KERNEL<true, true>), grid, dim3(BS), 0, st, d, g, t.pair_xcd);     \
+    else                                                                                      \
+      hipLaunchKernelGGL((KERNEL<true, false>), grid, dim3(BS), 0, st, d, g, t.pair_xcd);    \
+  } while (0)
+  if (t.pair_bal)
+    SC_PAIR(k_pair_bal, 64);
   else
-    hipLaunchKernelGGL(k_pair_fused<true>, grid, block, 0, st, d, g, t.pair_xcd);
+    SC_PAIR(k_pair_fused, t.pair_block);
+#undef SC_PAIR
 }
 
 }  // namespace sc

@@ -1,7 +1,8 @@
 """The reference's binary .spec / .posit trajectory files for a filament
-species: writers over the C ABI (sc_output_header, sc_spec_frame,
-sc_posit_frame) and plain readers of the same layouts (the reference's
-analysis readers, analysis/simcore_tools.py, parse them the same way)."""
+species and its .thermo file: writers over the C ABI (sc_output_header,
+sc_spec_frame, sc_posit_frame, sc_thermo_header, sc_thermo_frame) and plain
+readers of the same layouts (the reference's analysis readers,
+analysis/simcore_tools.py, parse the trajectory files the same way)."""
 import ctypes
 import struct
 from ctypes import byref
@@ -83,4 +84,42 @@ def read_posit(path):
         a = np.frombuffer(buf, dtype=np.float64, count=11 * num, offset=pos).reshape(num, 11)
         pos += 88 * num
         frames.append(a.copy())
+    return head, frames
+
+
+def write_thermo_header(path, n_steps, n_thermo, delta, n_dim=3):
+    """<run>.thermo header (OutputManagerBase::InitThermo,
+    include/simcore/output_manager.hpp:156-168)."""
+    rc = lib.sc_thermo_header(str(path).encode(), int(n_steps), int(n_thermo), float(delta),
+                              int(n_dim))
+    if rc != SC_OK:
+        raise SimcoreError(rc, f"sc_thermo_header({path})")
+
+
+def write_thermo_frame(path, unit_cell, pressure_tensor, pressure, volume):
+    """One frame of OutputManagerBase::WriteThermo
+    (include/simcore/output_manager.hpp:170-183)."""
+    uc = np.ascontiguousarray(unit_cell, dtype=np.float64).ravel()
+    pt = np.ascontiguousarray(pressure_tensor, dtype=np.float64).ravel()
+    if uc.size != 9 or pt.size != 9:
+        raise ValueError("unit cell and pressure tensor are 3 x 3")
+    rc = lib.sc_thermo_frame(str(path).encode(), _dp(uc), _dp(pt), float(pressure),
+                             float(volume))
+    if rc != SC_OK:
+        raise SimcoreError(rc, f"sc_thermo_frame({path})")
+
+
+def read_thermo(path):
+    """(header, frames); header n_steps, n_thermo, delta, n_dim; a frame is
+    a dict of unit_cell (3x3), pressure_tensor (3x3), pressure, volume."""
+    buf = open(path, "rb").read()
+    n_steps, n_thermo, delta, n_dim = struct.unpack_from("=iidi", buf, 0)
+    head = {"n_steps": n_steps, "n_thermo": n_thermo, "delta": delta, "n_dim": n_dim}
+    pos, frames = 20, []
+    while pos < len(buf):
+        a = np.frombuffer(buf, dtype=np.float64, count=20, offset=pos)
+        pos += 160
+        frames.append({"unit_cell": a[:9].reshape(3, 3).copy(),
+                       "pressure_tensor": a[9:18].reshape(3, 3).copy(),
+                       "pressure": float(a[18]), "volume": float(a[19])})
     return head, frames

@@ -328,6 +328,28 @@ class FilamentSystem:
                self.h, "sc_theta_histogram")
         return hist
 
+    def thermo_enable(self, on=True):
+        """Sum the pair stress and the bond energies on every step
+        (sc_thermo_enable); enabling starts a fresh stress sum."""
+        _check(lib.sc_thermo_enable(self.h, 1 if on else 0), self.h, "sc_thermo_enable")
+
+    def calculate_pressure(self, n_thermo):
+        """InteractionManager::CalculatePressure over the stress summed since
+        the last call (sc_calculate_pressure): {"pressure_tensor" (3x3),
+        "pressure", "volume"}; resets the sum."""
+        pt = np.empty(9)
+        pr, vol = c_double(), c_double()
+        _check(lib.sc_calculate_pressure(self.h, int(n_thermo), _dp(pt), byref(pr), byref(vol)),
+               self.h, "sc_calculate_pressure")
+        return {"pressure_tensor": pt.reshape(3, 3), "pressure": pr.value, "volume": vol.value}
+
+    def bond_energies(self):
+        """Each bond's pair potential energy of the last step, caller bond
+        order (sc_get_bond_energies)."""
+        e = np.empty(self.nb)
+        _check(lib.sc_get_bond_energies(self.h, _dp(e)), self.h, "sc_get_bond_energies")
+        return e
+
     def set_noise(self, xi=None, seed=None):
         """Caller-supplied thermal noise (sc_set_noise): `xi` holds the draws
         of the next len(xi) / (3 n_sites) midsteps, 3 per site, sites in

@@ -109,6 +109,50 @@ def run_sticky(sys_, rtol=1e-12, atol=1e-12):
     assert np.all(F == 0.0) and np.all(T == 0.0)
 
 
+def wca_energy(r, eps=1.0, sig=1.0):
+    """U(r) = 4 eps ((sig/r)^12 - (sig/r)^6) + eps, the shifted WCA energy."""
+    x6 = (sig / r) ** 6
+    return 4.0 * eps * (x6 * x6 - x6) + eps
+
+
+def run_thermo(sys_, rtol=1e-12, atol=1e-12):
+    """The thermo output on the sticky set-up.  Each step adds every pair's
+    stored stress -dr_i F_j to InteractionManager::stress_ and its stored
+    energy to both bonds (src/interaction_manager.cpp:500-531; the stress
+    and pote are set where the force is, include/simcore/
+    wca_potential.hpp:28-31, so they are sticky too).  The one pair, A0
+    below B0 at distance h0 along z with the force -f0 z on A0, has stress
+    diag(0, 0, h0 f0) and energy U(h0) on A0 and B0; after n_thermo = 4
+    steps CalculatePressure (:534-559) gives diag(n_b/V) + diag(0, 0,
+    4 h0 f0) / (4 V) with n_b = 4 bonds and V = 20^3; a rebuild clears both."""
+    f0 = wca_force_magnitude(STICKY_H0)
+    u0 = wca_energy(STICKY_H0)
+    nb, vol = 4, 20.0 ** 3
+    sys_.thermo_enable()
+    e_expect = np.array([u0, 0.0, u0, 0.0])
+    sys_.step(1)
+    np.testing.assert_allclose(sys_.bond_energies(), e_expect, rtol=rtol, atol=atol)
+    moved = soa(sticky_sites(STICKY_H1))
+    for k in (2, 3, 4):
+        sys_.set_sites(moved)
+        sys_.step(1)
+        np.testing.assert_allclose(sys_.bond_energies(), e_expect, rtol=rtol, atol=atol,
+                                   err_msg=f"step {k}")
+    out = sys_.calculate_pressure(4)
+    assert out["volume"] == vol
+    pt = np.diag([nb / vol, nb / vol, (nb + STICKY_H0 * f0) / vol])
+    np.testing.assert_allclose(out["pressure_tensor"], pt, rtol=rtol, atol=atol)
+    np.testing.assert_allclose(out["pressure"], np.trace(pt) / 3, rtol=rtol)
+    rebuilds = sys_.counters()["n_rebuilds"]
+    sys_.set_sites(soa(sticky_sites(STICKY_H1, shift_b=(0.5, 0.0, 0.0))))
+    sys_.step(1)
+    assert sys_.counters()["n_rebuilds"] == rebuilds + 1
+    assert np.all(sys_.bond_energies() == 0.0)
+    out = sys_.calculate_pressure(1)
+    np.testing.assert_allclose(out["pressure_tensor"], np.eye(3) * nb / vol, rtol=rtol,
+                               atol=atol)
+
+
 # ------------------------------------------------------------ trigger ----
 TRIGGER_R = 10.0
 

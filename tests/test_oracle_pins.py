@@ -431,6 +431,20 @@ def test_sticky_scenario_rejects_a_fresh_evaluation():
     assert math.sqrt(0.25 + S.STICKY_H0 ** 2) > S.RCUT
 
 
+def test_thermo_stress_energy_and_pressure_closed_form():
+    p, fil = S.sticky_setup()
+    S.run_thermo(O.OracleSystem(p, fil, n_threads=1))
+
+
+def test_thermo_scenario_rejects_a_dropped_or_fresh_term():
+    """The expectations differ from what a stress without the sticky steps
+    (h0 f0 / 4 on zz), a fresh evaluation (0 at h1) or a missing density
+    term would give."""
+    f0 = S.wca_force_magnitude(S.STICKY_H0)
+    assert S.STICKY_H0 * f0 > 1.0 and S.wca_energy(S.STICKY_H0) > 0.1
+    assert S.wca_energy(S.RCUT) == pytest.approx(0.0, abs=1e-12)
+
+
 def test_rebuild_trigger_counts_each_displacement_twice():
     p, fil = S.trigger_setup()
     S.run_trigger(O.OracleSystem(p, fil, n_threads=1), cycles=2)

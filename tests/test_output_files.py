@@ -63,3 +63,30 @@ def test_posit_fields(tmp_path):
         np.testing.assert_allclose(a[f, 6:9], us / np.linalg.norm(us), atol=1e-12)
         assert a[f, 9] == fil.fil[f].diameter
         assert a[f, 10] == fil.fil[f].bond_length * (s1 - s0 - 1)
+
+
+def test_thermo_file_layout(tmp_path):
+    """OutputManagerBase::InitThermo / WriteThermo
+    (include/simcore/output_manager.hpp:156-183): a 20-byte header (int,
+    int, double, int) and 160 bytes per frame (9 + 9 + 1 + 1 doubles)."""
+    import struct
+    path = tmp_path / "run.thermo"
+    output.write_thermo_header(path, 5000, 100, 2e-4)
+    rng = np.random.default_rng(1)
+    frames = []
+    for k in range(3):
+        uc = np.diag([20.0, 20.0, 20.0]) + k
+        pt = rng.normal(size=(3, 3))
+        frames.append((uc, pt, float(np.trace(pt) / 3), 8000.0 + k))
+        output.write_thermo_frame(path, *frames[-1])
+    raw = path.read_bytes()
+    assert len(raw) == 20 + 3 * 160
+    assert struct.unpack_from("=iidi", raw, 0) == (5000, 100, 2e-4, 3)
+    head, got = output.read_thermo(path)
+    assert head == {"n_steps": 5000, "n_thermo": 100, "delta": 2e-4, "n_dim": 3}
+    for (uc, pt, pr, vol), g in zip(frames, got):
+        np.testing.assert_array_equal(g["unit_cell"], uc)
+        np.testing.assert_array_equal(g["pressure_tensor"], pt)
+        assert g["pressure"] == pr and g["volume"] == vol
+    # the second frame's pressure tensor starts 20 + 160 + 72 bytes in
+    assert struct.unpack_from("=d", raw, 20 + 160 + 72)[0] == frames[1][1][0, 0]
