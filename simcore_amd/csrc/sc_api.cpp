@@ -117,6 +117,55 @@ unsigned long long *scratch_ptr(int device) {
   return g_scratch_ptr[device];
 }
 
+// q-th cell of the half shell of cell k (CellList::AssignCellNeighbors
+// order, src/cell_list.cpp:206-262), host side.
+int32_t host_shell_key(int n, int32_t k, int q) {
+  const int x = k / (n * n), y = (k / n) % n, z = k % n;
+  int dx = 1, dy = 0, dz = 0;
+  if (q >= 1 && q < 4) {
+    dx = q - 2;
+    dy = 1;
+  } else if (q >= 4) {
+    dx = (q - 4) % 3 - 1;
+    dy = (q - 4) / 3 - 1;
+    dz = 1;
+  }
+  const int xp = (x + dx + n) % n, yp = (y + dy + n) % n, zp = (z + dz) % n;
+  return ((int32_t)xp * n + yp) * n + zp;
+}
+
+// Geo::home_order: for each class of cell position (on the low face, in
+// the interior or on the high face in x and y; on the low face in z or
+// not), the 14 home cells of a bond -- its own cell (code 13) and the 13
+// cells whose half shell holds it (code q: half-shell offset q negated) --
+// sorted by cell key.  With at least 3 cells per side the order depends on
+// the class only: keys compare as (x, y, z) lexicographically and each
+// coordinate's wrapped offsets keep one order inside a class.
+void build_home_order(sc::Geo &g) {
+  const int n = g.n1d;
+  const int xs[3] = {0, 1, n - 1};
+  for (int cxc = 0; cxc < 3; ++cxc)
+    for (int cyc = 0; cyc < 3; ++cyc)
+      for (int czc = 0; czc < 2; ++czc) {
+        const int x = xs[cxc], y = xs[cyc], z = czc == 0 ? 0 : 1;
+        const int32_t k = ((int32_t)x * n + y) * n + z;
+        std::vector<std::pair<int32_t, int>> cells;
+        cells.push_back({k, 13});
+        for (int q = 0; q < 13; ++q) {
+          // the mirror of half-shell cell q: offsets negated
+          const int32_t fk = host_shell_key(n, k, q);
+          const int fx = fk / (n * n), fy = (fk / n) % n, fz = fk % n;
+          const int mx = (2 * x - fx + 2 * n) % n, my = (2 * y - fy + 2 * n) % n,
+                    mz = (2 * z - fz + 2 * n) % n;
+          cells.push_back({((int32_t)mx * n + my) * n + mz, q});
+        }
+        std::sort(cells.begin(), cells.end());
+        uint64_t code = 0;
+        for (int i = 13; i >= 0; --i) code = (code << 4) | (uint64_t)cells[i].second;
+        g.home_order[cxc * 6 + cyc * 2 + czc] = code;
+      }
+}
+
 // InteractionManager::InitInteractions -> CellList::SetMinCellLength /
 // Init (src/interaction_manager.cpp:46-54, src/cell_list.cpp:9-64)
 int cells_per_side(const sc_params &p) {
@@ -677,6 +726,8 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
   g.n1d = n1d;
   g.thr = 0.25 * cell_len * cell_len;
   g.n_cells = (int64_t)n1d * n1d * n1d;
+  g.occ_words = (n1d + 31) / 32;
+  build_home_order(g);
   g.stoch = p.stoch_flag ? 1 : 0;
   g.seed = p.seed;
   // mixed precision keeps 32-byte bond records whose cell coordinates are
@@ -715,11 +766,12 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   b += 4 * R(24 * ns) + 2 * R(8 * ns);                    // r, rp, fr, dr0, acc, acc_prev
   const int64_t rec_bytes = params->precision == SC_PREC_MIXED ? 32 : 64;
   b += R(rec_bytes * nb) + 2 * R(24 * nb) + R(8 * nb);    // record, F, T (at most), tension
-  b += R(4 * nb) + R(4 * n_cells) + R(4 * (n_cells + 1));  // keys, cell count/start
+  const int64_t ncol = n1d * n1d, occ_words = (n1d + 31) / 32;
+  b += R(4 * nb) + R(4 * ncol) + R(4 * (ncol + 1)) + R(4 * nb);  // keys, columns, col_tmp
+  b += R(8 * n_cells) + R(4 * ncol * occ_words);                 // cell runs, occupancy
   b += 2 * R(4 * nb) + R(4 * nb) + R(4 * (nb + 1));  // sorted, skey, deg, inc_start
-  b += R(4 * nb);                                      // sorted_o
-  b += R((n_cells + 7) / 8 + 8);                                               // occupancy
-  const int64_t tmp = std::max(scan_tmp_elems(n_cells), scan_tmp_elems(nb));
+  b += R(4 * nb);                                      // neighbour masks
+  const int64_t tmp = std::max(scan_tmp_elems(ncol), scan_tmp_elems(nb));
   b += R(4 * tmp);
   b += R(4 * (int64_t)nf) + (size_t)n_bins * Arena::kAlign;  // bin lists
   if (params->precision == SC_PREC_MIXED)  // packed layout: at most one wave per filament
@@ -733,23 +785,6 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
 }  // namespace sc
 
 namespace {
-
-// q-th cell of the half shell of cell k (CellList::AssignCellNeighbors
-// order, src/cell_list.cpp:206-262), host side.
-int32_t host_shell_key(int n, int32_t k, int q) {
-  const int x = k / (n * n), y = (k / n) % n, z = k % n;
-  int dx = 1, dy = 0, dz = 0;
-  if (q >= 1 && q < 4) {
-    dx = q - 2;
-    dy = 1;
-  } else if (q >= 4) {
-    dx = (q - 4) % 3 - 1;
-    dy = (q - 4) / 3 - 1;
-    dz = 1;
-  }
-  const int xp = (x + dx + n) % n, yp = (y + dy + n) % n, zp = (z + dz) % n;
-  return ((int32_t)xp * n + yp) * n + zp;
-}
 
 struct HostCand {
   int32_t key, phase, rank1, rank2, b1, b2;
@@ -931,15 +966,18 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = dalloc(h, &d.bpote, nb, false))) return rc;
   if ((rc = dalloc(h, &d.stress_acc, 6 * (size_t)stress_slots(nb), false))) return rc;
   if ((rc = dalloc(h, &d.keys, nb, false))) return rc;
-  if ((rc = dalloc(h, &d.cell_count, g.n_cells, false))) return rc;
-  if ((rc = dalloc(h, &d.cell_start, g.n_cells + 1, false))) return rc;
+  const int64_t ncol = (int64_t)g.n1d * g.n1d;
+  if ((rc = dalloc(h, &d.col_count, ncol, false))) return rc;
+  if ((rc = dalloc(h, &d.col_start, ncol + 1, false))) return rc;
+  if ((rc = dalloc(h, &d.col_tmp, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.cell_run, g.n_cells, false))) return rc;
   if ((rc = dalloc(h, &d.sorted, nb, false))) return rc;
   if ((rc = dalloc(h, &d.skey, nb, false))) return rc;
-  if ((rc = dalloc(h, &d.sorted_o, nb, false))) return rc;
-  if ((rc = dalloc(h, &d.cell_occ, (size_t)(g.n_cells + 7) / 8 + 8, false))) return rc;
+  if ((rc = dalloc(h, &d.cell_occ, ncol * g.occ_words, false))) return rc;
   if ((rc = dalloc(h, &d.deg, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.nmask, nb, false))) return rc;
   if ((rc = dalloc(h, &d.inc_start, nb + 1, false))) return rc;
-  int64_t tmp = std::max(sc::scan_tmp_elems(g.n_cells), sc::scan_tmp_elems(nb));
+  int64_t tmp = std::max(sc::scan_tmp_elems(ncol), sc::scan_tmp_elems(nb));
   if ((rc = dalloc(h, &d.scan_tmp, tmp, false))) return rc;
 
   // host-side per-filament tables
@@ -986,7 +1024,8 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   HIPCHK(h, hipMemsetAsync(d.tension, 0, sizeof(double) * nb, h->stream));
   HIPCHK(h, hipMemsetAsync(d.bpote, 0, sizeof(double) * nb, h->stream));
   HIPCHK(h, hipMemsetAsync(d.stress_acc, 0, sizeof(double) * 6 * stress_slots(nb), h->stream));
-  HIPCHK(h, hipMemsetAsync(d.cell_count, 0, sizeof(int32_t) * g.n_cells, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.col_count, 0, sizeof(int32_t) * ncol, h->stream));
+  HIPCHK(h, hipMemsetAsync(d.cell_occ, 0, sizeof(uint32_t) * ncol * g.occ_words, h->stream));
   HIPCHK(h, hipMemsetAsync(d.deg, 0, sizeof(int32_t) * nb, h->stream));
   Ctl c{};
   if ((rc = write_ctl(h, c))) return rc;
@@ -1630,7 +1669,7 @@ struct CkHeader {
   Ctl ctl;
 };
 
-constexpr char kCkMagic[8] = {'S', 'C', 'C', 'K', 'P', 'T', '0', '2'};
+constexpr char kCkMagic[8] = {'S', 'C', 'C', 'K', 'P', 'T', '0', '3'};
 
 // One field of the checkpoint: `n` elements of `T` at device address `dev`.
 struct CkField {
@@ -1681,7 +1720,7 @@ int sc_checkpoint_save(sc_handle h, void *buf, size_t cap, size_t *size) {
   char *p = static_cast<char *>(buf);
   CkHeader hd{};
   std::memcpy(hd.magic, kCkMagic, 8);
-  hd.version = 2;
+  hd.version = 3;
   hd.precision = h->p.precision;
   hd.spatial_order = h->reordered ? 1 : 0;
   hd.nf = h->g.nf;
@@ -1714,7 +1753,7 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size) {
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   CkHeader hd;
   std::memcpy(&hd, buf, sizeof(hd));
-  if (std::memcmp(hd.magic, kCkMagic, 8) != 0 || hd.version != 2)
+  if (std::memcmp(hd.magic, kCkMagic, 8) != 0 || hd.version != 3)
     return fail(h, SC_ERR_ARG, "not a checkpoint of this library");
   if (hd.precision != h->p.precision || hd.spatial_order != (h->reordered ? 1 : 0) ||
       hd.nf != h->g.nf || hd.ns != h->g.ns || hd.nb != h->g.nb || hd.n_cells != h->g.n_cells ||

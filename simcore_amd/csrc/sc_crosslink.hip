@@ -147,9 +147,9 @@ __global__ __launch_bounds__(256) void k_xl_events(Dev d, Geo g) {
       for (int dz = 0; dz < span; ++dz) {
         const int X = ((cx + off0 + dx) % n + n) % n, Y = ((cy + off0 + dy) % n + n) % n;
         const int Z = ((cz + off0 + dz) % n + n) % n;
-        const int32_t c = (X * n + Y) * n + Z;
-        if (!((d.cell_occ[c >> 3] >> (c & 7)) & 1)) continue;  // empty cell
-        for (int32_t j = d.cell_start[c]; j < d.cell_start[c + 1]; ++j) {
+        if (!cell_occupied(d, g, X, Y, Z)) continue;  // empty cell
+        const int2 run = d.cell_run[(X * n + Y) * n + Z];
+        for (int32_t j = run.x; j < run.y; ++j) {
           const int32_t b = d.sorted[j];
           if (b == bs) continue;
           double rec[7];

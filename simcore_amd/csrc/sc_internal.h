@@ -50,6 +50,7 @@ struct Geo {
   double delta;
   double thr;              // 0.25 * cell_len^2
   int n1d;
+  int occ_words;           // 32-bit occupancy words per cell column
   int stoch;
   int compact;             // mixed precision: 32-byte bond records (sc_math.h)
   double inv_n1d;
@@ -66,6 +67,11 @@ struct Geo {
   double xpoff, xrb2;      // unbinding probability per full step, range^2
   uint64_t xseed;
   int xreach;              // cells searched on each side when rebinding
+  // order of a bond's 14 home cells (its own cell: code 13; mirrored
+  // half-shell cell q: code q) by cell key, 4 bits per entry, first in the
+  // low bits; indexed by which of the cell's coordinates lie on a periodic
+  // face: (x == 0 ? 0 : x == n-1 ? 2 : 1) * 6 + (y likewise) * 2 + (z > 0)
+  uint64_t home_order[18];
 };
 
 // Device pointers (a plain struct so it can be passed by value).
@@ -91,19 +97,31 @@ struct Dev {
   double *bf, *bt, *tension;  // bf, bt: FP64 precision only
   float *bf32, *bt32;          // per-bond force / torque sums, mixed precision
   // cell list
-  int32_t *keys, *cell_count, *cell_start, *sorted, *skey;
-  int32_t *sorted_o;       // cell runs in bond order
-  uint8_t *cell_occ;       // n_cells bits: cell occupied in the last build
+  // cell list of the last build (CellList::UpdateCellList,
+  // src/cell_list.cpp:183-199): bonds sorted by (cell key, bond index) --
+  // a counting sort by cell column (x, y) followed by a rank sort of each
+  // column in LDS -- with the run of every occupied cell and one
+  // occupancy bit per cell
+  int32_t *keys;           // nb: cell key of each bond
+  int32_t *col_count;      // n1d^2: column populations (zero between builds)
+  int32_t *col_start;      // n1d^2 + 1: column starts in sorted
+  int32_t *col_tmp;        // nb: bonds grouped by column, unordered
+  int32_t *sorted, *skey;  // nb: bonds in (key, bond) order, their keys
+  int2 *cell_run;          // n_cells: [start, end) of an occupied cell
+  uint32_t *cell_occ;      // n1d^2 * occ_words: bit z of column (x, y)
   // candidates: per bond, its incidences in candidate order (CSR over
   // inc_start), each with its own copy of the pair's sticky force -- both
   // owners compute the same bits, so the copies agree, and a bond's slot
   // traffic follows its list.  The candidate list itself is implicit (the
   // reference's order is recovered from the lists for the taps).
   int32_t *deg, *inc_start;
-  int2 *inc;               // {partner bond, role | written << 1}: role 1 when
-                           // the bond is obj2; written once the slot holds a
-                           // force (a slot not written since the rebuild is 0)
-  double4 *islot64;        // sticky force per incidence {Fx,Fy,Fz,-}, FP64
+  uint32_t *nmask;         // per bond, rebuild scratch: occupancy of its 13
+                           // half-shell (bits 0-12) and 13 mirrored cells
+  int2 *inc;               // {partner bond, role | written << 1 | (owner
+                           // mod 64) << 2}: role 1 when the bond is obj2;
+                           // written once the slot holds a force (a slot
+                           // not written since the rebuild is 0)
+  double4 *islot64;        // sticky force per incidence {Fx,Fy,Fz,r}, FP64
   float4 *islot32;         // the same in FP32 (mixed precision)
   // scan scratch
   int32_t *scan_tmp;
@@ -157,6 +175,13 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 
 enum Guard { kGuardStep = 0, kGuardRebuild = 1, kGuardXlink = 2 };
 
+// Cell-list lookups.  Occupancy bit of cell (x, y, z); run of an occupied
+// cell (its members in bond order).
+__device__ __forceinline__ bool cell_occupied(const Dev &d, const Geo &g, int x, int y, int z) {
+  const uint32_t w = d.cell_occ[((int64_t)x * g.n1d + y) * g.occ_words + (z >> 5)];
+  return (w >> (z & 31)) & 1u;
+}
+
 // Launchers (sc_kernels.hip).  All are asynchronous on `st`.
 void launch_decide(const Dev &d, const Geo &g, hipStream_t st);
 void launch_commit(const Dev &d, const Geo &g, hipStream_t st);
@@ -200,7 +225,7 @@ void launch_zero_i32(int32_t *p, int64_t n, hipStream_t st);
 void launch_redo_dr(const Dev &d, const Geo &g, hipStream_t st);
 void launch_copy_probe(const void *src, void *dst, int64_t n32, hipStream_t st);
 void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
-                 const Ctl *ctl, int guard, hipStream_t st, uint8_t *occ = nullptr);
+                 const Ctl *ctl, int guard, hipStream_t st);
 int64_t scan_tmp_elems(int64_t n);
 
 }  // namespace sc

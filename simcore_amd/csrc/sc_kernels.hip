@@ -8,11 +8,14 @@
 //
 // One step, in stream order (all guarded by the device control block,
 // which holds the decision for the step about to run):
-//   [rebuild: keys, scan, scatter, count (+ run order, degrees), emit
-//    (+ offsets, cap), scan, incidence fill, incidence sort, zero dr]
-//   -> pair+scatter -> integrate (one launch per length bin)
+//   [rebuild: keys (+ column populations), column scan, column scatter,
+//    column rank sort (+ cell runs, occupancy), degrees, degree scan,
+//    incidence lists (+ capacity, zero dr)]
+//   -> pair+scatter -> integrate (per length class)
 //   -> commit (and decide the next step)
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "sc_internal.h"
 #include "sc_math.h"
@@ -197,14 +200,12 @@ __device__ void tile_scan(int *tile, int *wsum, int carry_in, int *total) {
 // few thousand, read from L2) and scans itself; the last tile also writes
 // the total.  Two launches per scan; blocks loop over tiles, so any grid
 // size serves.
-// With `occ` set, the pass also writes one bit per entry, set when the
-// entry (a cell's population) is non-zero: the cell-occupancy mask.
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
                                                           int64_t n,
                                                           const int32_t *bsum,
                                                           int32_t *out,
                                                           const Ctl *ctl,
-                                                          int guard, uint8_t *occ) {
+                                                          int guard) {
   if (guard_off(ctl, guard)) return;
   __shared__ int tile[kScanTile];
   __shared__ int wsum[kScanBlock / 64];
@@ -218,14 +219,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
     for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
       tile[j] = (base + j < n) ? in[base + j] : 0;
     __syncthreads();
-    if (occ && base + threadIdx.x * kScanItems < n) {
-      static_assert(kScanItems == 8, "one mask byte per thread");
-      unsigned bits = 0;
-#pragma unroll
-      for (int j = 0; j < kScanItems; ++j)
-        bits |= (tile[threadIdx.x * kScanItems + j] != 0 ? 1u : 0u) << j;
-      occ[(base >> 3) + threadIdx.x] = (uint8_t)bits;
-    }
     int tot;
     tile_scan(tile, wsum, carry, &tot);
     for (int j = threadIdx.x; j < kScanTile; j += kScanBlock)
@@ -238,13 +231,14 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
 int64_t scan_tmp_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
 void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
-                 const Ctl *ctl, int guard, hipStream_t st, uint8_t *occ) {
+                 const Ctl *ctl, int guard, hipStream_t st) {
   int64_t nblk = (n + kScanTile - 1) / kScanTile;
   if (nblk == 0) nblk = 1;
+  if (nblk > kStreamBlocks) nblk = kStreamBlocks;
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblk), dim3(kScanBlock), 0,
                      st, in, n, tmp, ctl, guard);
   hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nblk), dim3(kScanBlock), 0,
-                     st, in, n, tmp, out, ctl, guard, occ);
+                     st, in, n, tmp, out, ctl, guard);
 }
 
 // ---------------------------------------------------------------------------
@@ -252,8 +246,9 @@ void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
 // ---------------------------------------------------------------------------
 
 // CellList::FindCellCoords over every bond (src/cell_list.cpp:139-157) and
-// the per-cell population count.  Streaming passes loop over their range,
-// so any grid size serves.
+// the population of each cell column (x, y) (n1d^2 counters instead of
+// n1d^3, so the scan that follows is small).  Streaming passes loop over
+// their range, so any grid size serves.
 __global__ void k_keys(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -266,34 +261,99 @@ __global__ void k_keys(Dev d, Geo g) {
       k = cell_key(g.n1d, rec[0], rec[1], rec[2]);
     }
     d.keys[b] = k;
-    atomicAdd(&d.cell_count[k], 1);
+    atomicAdd(&d.col_count[k / g.n1d], 1);
   }
 }
 
-// Population count from the stored keys (cell list restore).
+// Column populations from the stored keys (cell list restore).
 __global__ void k_key_count(Dev d, Geo g) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride)
-    atomicAdd(&d.cell_count[d.keys[b]], 1);
+    atomicAdd(&d.col_count[d.keys[b] / g.n1d], 1);
 }
 
-// Counting-sort scatter (order inside a cell arbitrary; k_deg orders the
-// runs); returns cell_count to all-zero for the next build.
+// Groups the bonds by column (order inside a column arbitrary; the rank
+// sort orders it) and returns col_count to all-zero for the next build.
 __global__ void k_scatter(Dev d, Geo g, int guard) {
   if (guard_off(d.ctl, guard)) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
-    const int32_t k = d.keys[b];
-    const int32_t pos = d.cell_start[k] + atomicSub(&d.cell_count[k], 1) - 1;
-    d.sorted[pos] = (int32_t)b;
-    d.skey[pos] = k;
+    const int32_t c = d.keys[b] / g.n1d;
+    const int32_t pos = d.col_start[c] + atomicSub(&d.col_count[c], 1) - 1;
+    d.col_tmp[pos] = (int32_t)b;
   }
 }
 
-// q-th cell of the half-shell stencil of (x, y, z) in the order of
-// CellList::AssignCellNeighbors (src/cell_list.cpp:206-262): (x+1, y, z);
-// (x-1..x+1, y+1, z); then (x-1..x+1, y-1..y+1, z+1), x fastest.
-__device__ __forceinline__ int32_t shell_key(int n, int x, int y, int z, int q) {
+// Each cell column sorted by (cell key, bond index): one wavefront per
+// column; an entry's place is the number of smaller (key, bond) pairs in
+// its column, counted against the column staged in LDS (a column holds a
+// few tens of bonds at the densities simulated; longer ones than the stage
+// count against global memory).  Members of a cell thus appear in bond
+// order, the order Cell::AddObj appends in (src/cell.cpp:5-8).  The pass
+// also writes the run of each occupied cell and the column's occupancy
+// words (every word of the column, so no clearing pass is needed).
+constexpr int kColStage = 512;
+
+__device__ __forceinline__ int64_t kb_pair(const Dev &d, int32_t b) {
+  return ((int64_t)d.keys[b] << 32) | (uint32_t)b;
+}
+
+__global__ __launch_bounds__(64) void k_colsort(Dev d, Geo g, int guard) {
+  if (guard_off(d.ctl, guard)) return;
+  __shared__ int64_t ks[kColStage], kr[kColStage];
+  __shared__ uint32_t occ[64];  // occ_words <= 41 (n1d^3 < 2^31)
+  const int lane = threadIdx.x;
+  const int n = g.n1d, W = g.occ_words;
+  const int64_t ncol = (int64_t)n * n;
+  for (int64_t c = blockIdx.x; c < ncol; c += gridDim.x) {
+    const int32_t s = d.col_start[c], e = d.col_start[c + 1], m = e - s;
+    const bool staged = m <= kColStage;
+    if (lane < W) occ[lane] = 0u;
+    if (staged)
+      for (int i = lane; i < m; i += 64) ks[i] = kb_pair(d, d.col_tmp[s + i]);
+    __syncthreads();
+    for (int i = lane; i < m; i += 64) {
+      const int64_t ki = staged ? ks[i] : kb_pair(d, d.col_tmp[s + i]);
+      int32_t r = 0;
+      if (staged) {
+        for (int j = 0; j < m; ++j) r += ks[j] < ki;
+        kr[r] = ki;
+      } else {
+        for (int j = 0; j < m; ++j) r += kb_pair(d, d.col_tmp[s + j]) < ki;
+      }
+      d.sorted[s + r] = (int32_t)(uint32_t)ki;
+      d.skey[s + r] = (int32_t)(ki >> 32);
+    }
+    // runs: staged columns read the sorted keys from LDS, long ones from
+    // global memory after a device-scope fence
+    if (!staged) __threadfence();
+    __syncthreads();
+    for (int i = lane; i < m; i += 64) {
+      const int32_t k = staged ? (int32_t)(kr[i] >> 32) : d.skey[s + i];
+      const int32_t kp = i == 0 ? -1 : (staged ? (int32_t)(kr[i - 1] >> 32) : d.skey[s + i - 1]);
+      if (k != kp) {
+        int32_t j = i + 1;
+        while (j < m && (staged ? (int32_t)(kr[j] >> 32) : d.skey[s + j]) == k) ++j;
+        d.cell_run[k] = make_int2(s + i, s + j);
+        const int z = k - (int32_t)c * n;
+        atomicOr(&occ[z >> 5], 1u << (z & 31));
+      }
+    }
+    __syncthreads();
+    if (lane < W) d.cell_occ[c * W + lane] = occ[lane];
+    __syncthreads();
+  }
+}
+
+// Coordinates of the q-th cell of the half-shell stencil of (x, y, z) in
+// the order of CellList::AssignCellNeighbors (src/cell_list.cpp:206-262):
+// (x+1, y, z); (x-1..x+1, y+1, z); then (x-1..x+1, y-1..y+1, z+1), x
+// fastest.  BACK: the mirrored half shell, the 13 cells whose half shell
+// holds (x, y, z) (offsets negated).  Periodic wrap; with at least 3 cells
+// per side the 13 + 13 + 1 cells are distinct.
+template <bool BACK>
+__device__ __forceinline__ void shell_cell(int n, int x, int y, int z, int q, int *xo, int *yo,
+                                           int *zo) {
   int dx, dy, dz;
   if (q == 0) {
     dx = 1;
@@ -308,66 +368,45 @@ __device__ __forceinline__ int32_t shell_key(int n, int x, int y, int z, int q) 
     dy = (q - 4) / 3 - 1;
     dz = 1;
   }
-  int xp = x + dx, yp = y + dy, zp = z + dz;
-  xp = xp < 0 ? n - 1 : (xp == n ? 0 : xp);
-  yp = yp < 0 ? n - 1 : (yp == n ? 0 : yp);
-  zp = zp == n ? 0 : zp;
-  return ((int32_t)xp * n + yp) * n + zp;
-}
-
-// q-th cell of the mirrored half shell: the 13 cells whose half shell holds
-// (x, y, z), i.e. the offsets of shell_key negated (wrapped).  With at
-// least 3 cells per side the 13 + 13 + 1 cells are distinct.
-__device__ __forceinline__ int32_t back_shell_key(int n, int x, int y, int z, int q) {
-  int dx, dy, dz;
-  if (q == 0) {
-    dx = -1;
-    dy = 0;
-    dz = 0;
-  } else if (q < 4) {
-    dx = 2 - q;
-    dy = -1;
-    dz = 0;
-  } else {
-    dx = 1 - (q - 4) % 3;
-    dy = 1 - (q - 4) / 3;
-    dz = -1;
+  if (BACK) {
+    dx = -dx;
+    dy = -dy;
+    dz = -dz;
   }
   int xp = x + dx, yp = y + dy, zp = z + dz;
   xp = xp < 0 ? n - 1 : (xp == n ? 0 : xp);
   yp = yp < 0 ? n - 1 : (yp == n ? 0 : yp);
-  zp = zp < 0 ? n - 1 : zp;
-  return ((int32_t)xp * n + yp) * n + zp;
+  zp = zp < 0 ? n - 1 : (zp == n ? 0 : zp);
+  *xo = xp;
+  *yo = yp;
+  *zo = zp;
 }
-
-// Run [start, end) of cell c in sorted / sorted_o, empty for an unoccupied
-// cell.  Most cells are empty; they are recognised from the occupancy mask,
-// a small array that stays in L2, so only occupied ones read the large
-// cell_start array.
-__device__ __forceinline__ void cell_run(const Dev &d, int32_t c, int32_t *s, int32_t *e) {
-  *s = 0;
-  *e = 0;
-  if ((d.cell_occ[c >> 3] >> (c & 7)) & 1u) {
-    *s = d.cell_start[c];
-    *e = d.cell_start[c + 1];
-  }
-}
-
-// The 13 half-shell cells (fw) or mirrored half-shell cells (bw) of k with
-// their runs, loaded up front so that the lookups are in flight together.
-struct Shell {
-  int32_t key[13], o0[13], o1[13];
-};
 
 template <bool BACK>
-__device__ __forceinline__ void load_shell(const Dev &d, const Geo &g, int32_t k, Shell *sh) {
+__device__ __forceinline__ int32_t shell_key(int n, int x, int y, int z, int q) {
+  int xp, yp, zp;
+  shell_cell<BACK>(n, x, y, z, q, &xp, &yp, &zp);
+  return ((int32_t)xp * n + yp) * n + zp;
+}
+
+// Occupancy of the 26 neighbours of cell (cx, cy, cz): bit q the half-shell
+// cell q, bit 13 + q the mirrored half-shell cell q.  The 26 reads are
+// independent and go out together (one round trip to L2, where the mask
+// lives), so the list passes then read the runs of the few occupied cells
+// only.
+__device__ __forceinline__ uint32_t shell_mask(const Dev &d, const Geo &g, int cx, int cy,
+                                               int cz) {
   const int n = g.n1d;
-  const int x = k / (n * n), y = (k / n) % n, z = k % n;
+  uint32_t m = 0;
 #pragma unroll
-  for (int q = 0; q < 13; ++q)
-    sh->key[q] = BACK ? back_shell_key(n, x, y, z, q) : shell_key(n, x, y, z, q);
-#pragma unroll
-  for (int q = 0; q < 13; ++q) cell_run(d, sh->key[q], &sh->o0[q], &sh->o1[q]);
+  for (int q = 0; q < 13; ++q) {
+    int x, y, z;
+    shell_cell<false>(n, cx, cy, cz, q, &x, &y, &z);
+    m |= (uint32_t)cell_occupied(d, g, x, y, z) << q;
+    shell_cell<true>(n, cx, cy, cz, q, &x, &y, &z);
+    m |= (uint32_t)cell_occupied(d, g, x, y, z) << (13 + q);
+  }
+  return m;
 }
 
 // The bonds b-1 and b+1 when they share a site with b (Bond::HasNeighbor
@@ -388,33 +427,40 @@ __device__ __forceinline__ Excl excl_of(const Dev &d, const Geo &g, int32_t b) {
 // Degree of every bond: the candidates that list it (Cell::MakePairs,
 // src/cell.cpp:14-70): the other members of its cell, the members of its 13
 // half-shell cells and of its 13 mirrored half-shell cells, less the
-// site-sharing neighbours among them.  One thread per sorted position, no
-// atomics; it also places its bond in its cell's run of sorted_o by its
-// rank in bond order (Cell::AddObj appends in interactor order,
-// src/cell.cpp:5-8), the order the pair lists are written in.
-__global__ __launch_bounds__(256) void k_deg(Dev d, Geo g) {
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.nb) return;
+// site-sharing neighbours among them.  One thread per sorted position (the
+// cell lookups of neighbouring threads then hit the same lines), no
+// atomics.  The stencil cells are recomputed per entry rather than held,
+// which keeps the register count (and so the occupancy) of this
+// latency-bound pass high.  The neighbour occupancy is kept for k_pull.
+__device__ __forceinline__ void deg_one(const Dev &d, const Geo &g, int64_t p) {
   const int32_t k = d.skey[p];
   const int32_t b = d.sorted[p];
-  const int32_t s0 = d.cell_start[k], e = d.cell_start[k + 1];
+  const int2 own = d.cell_run[k];
   const Excl x = excl_of(d, g, b);
   const int32_t klo = x.lo >= 0 ? d.keys[x.lo] : -1;
   const int32_t khi = x.hi >= 0 ? d.keys[x.hi] : -1;
-  Shell fw, bw;
-  load_shell<false>(d, g, k, &fw);
-  load_shell<true>(d, g, k, &bw);
-  int32_t rank = 0;
-  for (int32_t i = s0; i < e; ++i) rank += d.sorted[i] < b;
-  d.sorted_o[s0 + rank] = b;
-  int32_t deg = (e - s0 - 1) - (klo == k) - (khi == k);
-#pragma unroll
-  for (int q = 0; q < 13; ++q) {
-    deg += (fw.o1[q] - fw.o0[q]) - (klo == fw.key[q]) - (khi == fw.key[q]);
-    deg += (bw.o1[q] - bw.o0[q]) - (klo == bw.key[q]) - (khi == bw.key[q]);
+  int32_t deg = (own.y - own.x - 1) - (klo == k) - (khi == k);
+  const int n = g.n1d;
+  const int cx = k / (n * n), cy = (k / n) % n, cz = k % n;
+  const uint32_t m = shell_mask(d, g, cx, cy, cz);
+  d.nmask[b] = m;
+  for (uint32_t mm = m; mm; mm &= mm - 1) {
+    const int q = __ffs(mm) - 1;
+    const int32_t kc = q < 13 ? shell_key<false>(n, cx, cy, cz, q)
+                              : shell_key<true>(n, cx, cy, cz, q - 13);
+    const int2 r = d.cell_run[kc];
+    deg += (r.y - r.x) - (klo == kc) - (khi == kc);
   }
   d.deg[b] = deg;
+}
+
+// Grid-stride over the sorted positions: a capped grid keeps the launch
+// cheap on the steps that do not rebuild (the guard returns at once).
+__global__ __launch_bounds__(256) void k_deg(Dev d, Geo g) {
+  if (guard_off(d.ctl, kGuardRebuild)) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.nb; p += stride)
+    deg_one(d, g, p);
 }
 
 // Incidence lists in candidate order, one thread per bond, no atomics and
@@ -427,17 +473,78 @@ __global__ __launch_bounds__(256) void k_deg(Dev d, Geo g) {
 // later members, then as obj1 of its rows of the 13 half-shell blocks) and
 // from each occupied mirrored half-shell cell m (as obj2 of m's block
 // toward b's cell, one pair per member of m, in member order), the groups
-// taken in home-key order.  Entries {partner, role}; the slot copies start
-// unwritten (zero force).  The launch also checks the capacity, commits the
-// list length and restarts the displacement bookkeeping
-// (InteractionManager::ZeroDrTot -> Mesh/Object::ZeroDrTot,
-// src/interaction_manager.cpp:316-321, src/mesh.cpp:546-551).
+// taken in home-key order.  That order of the 14 home cells depends only on
+// which of b's cell coordinates sit on a periodic face (Geo::home_order,
+// built on the host), so no per-thread sort is needed.  Entries {partner,
+// role | written << 1 | (b mod 64) << 2}: the owner's place in its 64-bond
+// group lets the balanced pair kernel find an entry's bond without a
+// search; the slot copies start unwritten (zero force).  The launch also
+// checks the capacity, commits the list length and restarts the
+// displacement bookkeeping (InteractionManager::ZeroDrTot -> Mesh/Object::
+// ZeroDrTot, src/interaction_manager.cpp:316-321, src/mesh.cpp:546-551).
+__device__ __forceinline__ void pull_one(const Dev &d, const Geo &g, int32_t b,
+                                         int64_t total, int2 *stage, int32_t base) {
+  int32_t w = d.inc_start[b];
+  const int32_t wend = d.inc_start[b + 1];
+  if (w < 0 || wend < w || wend > total) {  // a wrapped count: refuse, never write
+    d.ctl->abort = 1;
+    d.ctl->overflow = 1;
+    return;
+  }
+  const int32_t k = d.keys[b];
+  const Excl x = excl_of(d, g, b);
+  const int32_t tag = (b & 63) << 2;
+  auto put = [&](int32_t a, int role) {
+    if (a == x.lo || a == x.hi) return;
+    if (w < wend) {
+      if (stage)
+        stage[w - base] = make_int2(a, role | tag);
+      else
+        d.inc[w] = make_int2(a, role | tag);
+    }
+    ++w;
+  };
+  const int n = g.n1d;
+  const int cx = k / (n * n), cy = (k / n) % n, cz = k % n;
+  const int cls = (cx == 0 ? 0 : cx == n - 1 ? 2 : 1) * 6 +
+                  (cy == 0 ? 0 : cy == n - 1 ? 2 : 1) * 2 + (cz == 0 ? 0 : 1);
+  const uint32_t m = d.nmask[b];  // occupied neighbours (k_deg)
+  const int2 own = d.cell_run[k];
+  uint64_t order = g.home_order[cls];
+  for (int step = 0; step < 14; ++step, order >>= 4) {
+    const int c = (int)(order & 15);
+    if (c == 13) {
+      // own cell (members in bond order): self pairs, then the rows of the
+      // half-shell blocks
+      int32_t r = own.x;
+      while (r < own.y && d.sorted[r] != b) ++r;
+      for (int32_t j = own.x; j < r; ++j) put(d.sorted[j], 1);
+      for (int32_t j = r + 1; j < own.y; ++j) put(d.sorted[j], 0);
+      for (uint32_t mm = m & 0x1fffu; mm; mm &= mm - 1) {
+        const int2 rn = d.cell_run[shell_key<false>(n, cx, cy, cz, __ffs(mm) - 1)];
+        for (int32_t j = rn.x; j < rn.y; ++j) put(d.sorted[j], 0);
+      }
+    } else if ((m >> (13 + c)) & 1u) {
+      // mirrored half-shell cell c: b is obj2 of each of its members
+      const int2 rn = d.cell_run[shell_key<true>(n, cx, cy, cz, c)];
+      for (int32_t j = rn.x; j < rn.y; ++j) put(d.sorted[j], 1);
+    }
+  }
+  if (w != wend) {  // degree and list disagree: never expected; fail the step
+    d.ctl->abort = 1;
+    d.ctl->err = SC_ERR_STATE;
+  }
+}
+
+constexpr int kPullStage = 512;
+
 __global__ __launch_bounds__(256) void k_pull(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t total = d.inc_start[g.nb];  // two incidences per candidate
   const bool fits = total >= 0 && total <= 2 * g.cap;
-  if (i == 0) {
+  if (i0 == 0) {
     d.ctl->nc_need = total >= 0 ? total / 2 : 2 * g.cap + 1;
     if (fits) {
       d.ctl->nc = total / 2;
@@ -448,67 +555,29 @@ __global__ __launch_bounds__(256) void k_pull(Dev d, Geo g) {
     }
   }
   if (!fits) return;
-  if (i < g.ns) {
+  for (int64_t i = i0; i < g.ns; i += stride) {
     for (int q = 0; q < 3; ++q) d.dr0[q * g.ns + i] = d.r[q * g.ns + i];
     d.acc[i] = 0.0;
   }
-  if (i >= g.nb) return;
-  const int32_t b = (int32_t)i;
-  int32_t w = d.inc_start[b];
-  const int32_t wend = d.inc_start[b + 1];
-  if (w < 0 || wend < w || wend > total) {  // a wrapped count: refuse, never write
-    d.ctl->abort = 1;
-    d.ctl->overflow = 1;
-    return;
-  }
-  const int32_t k = d.keys[b];
-  const Excl x = excl_of(d, g, b);
-  int32_t s0, e;
-  cell_run(d, k, &s0, &e);
-  Shell fw, bw;
-  load_shell<false>(d, g, k, &fw);
-  load_shell<true>(d, g, k, &bw);
-  auto put = [&](int32_t a, int role) {
-    if (a == x.lo || a == x.hi) return;
-    if (w < wend) d.inc[w] = make_int2(a, role);
-    ++w;
-  };
-  // home cells in key order: b's own cell k and the occupied mirrored
-  // cells, by repeated selection of the next larger key (13 is small)
-  int32_t last = -1;
-  for (int step = 0; step < 14; ++step) {
-    int32_t next = 0x7fffffff;
-    int which = -2;  // -1: own cell, q: mirrored cell q
-    if (k > last) {
-      next = k;
-      which = -1;
+  // each wavefront takes 64 consecutive bonds, whose lists are one
+  // contiguous range: written to LDS entry by entry, then copied out in
+  // coalesced stores (ranges longer than the stage are written directly)
+  __shared__ int2 stage[4][kPullStage];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t ngrp = (g.nb + 255) / 256;
+  for (int64_t grp = blockIdx.x; grp < ngrp; grp += gridDim.x) {
+    const int64_t b0 = grp * 256 + wv * 64, b = b0 + lane;
+    int32_t base = 0, cnt = 0;
+    if (b0 < g.nb) {
+      base = d.inc_start[b0];
+      cnt = d.inc_start[b0 + 64 < g.nb ? b0 + 64 : g.nb] - base;
     }
-#pragma unroll
-    for (int q = 0; q < 13; ++q)
-      if (bw.o1[q] > bw.o0[q] && bw.key[q] > last && bw.key[q] < next) {
-        next = bw.key[q];
-        which = q;
-      }
-    if (which == -2) break;
-    last = next;
-    if (which == -1) {
-      int32_t r = s0;
-      while (r < e && d.sorted_o[r] != b) ++r;
-      for (int32_t j = s0; j < r; ++j) put(d.sorted_o[j], 1);
-      for (int32_t j = r + 1; j < e; ++j) put(d.sorted_o[j], 0);
-#pragma unroll
-      for (int q = 0; q < 13; ++q)
-        for (int32_t j = fw.o0[q]; j < fw.o1[q]; ++j) put(d.sorted_o[j], 0);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 13; ++q)
-        if (q == which)
-          for (int32_t j = bw.o0[q]; j < bw.o1[q]; ++j) put(d.sorted_o[j], 1);
-    }
-  }
-  if (w != wend) {  // degree and list disagree: never expected; fail the step
-    d.ctl->abort = 1;
-    d.ctl->err = SC_ERR_STATE;
+    const bool staged = cnt >= 0 && cnt <= kPullStage;
+    if (b < g.nb) pull_one(d, g, (int32_t)b, total, staged ? stage[wv] : nullptr, base);
+    __syncthreads();
+    if (staged)
+      for (int i = lane; i < cnt; i += 64) d.inc[base + i] = stage[wv][i];
+    __syncthreads();
   }
 }
 
@@ -522,28 +591,40 @@ static inline unsigned sgrid(int64_t n) {
   return b < (unsigned)kStreamBlocks ? b : (unsigned)kStreamBlocks;
 }
 
-// keys (+ cell populations) -> cell scan (2) -> scatter -> degrees (+ run
-// order) -> degree scan (2) -> incidence lists (+ capacity check,
-// displacement restart): 8 launches.
+// keys (+ column populations) -> column scan (2) -> column scatter ->
+// column sort (+ runs, occupancy) -> degrees -> degree scan (2) ->
+// incidence lists (+ capacity check, displacement restart): 9 launches,
+// none over the n1d^3 cells.  Every launch's grid is capped (the kernels
+// loop over their range), so on the odd steps that do not rebuild each
+// launch only reads the guard.  The two per-bond list passes are
+// latency-bound and get 8 waves per SIMD.
+constexpr unsigned kListBlocks = 2048;
+
 void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
-  const unsigned gb = nblocks(g.nb, kScanBlock);
-  hipLaunchKernelGGL(k_keys, dim3(gb), dim3(kScanBlock), 0, st, d, g);
-  launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, d.ctl,
-              kGuardRebuild, st, d.cell_occ);
-  hipLaunchKernelGGL(k_scatter, dim3(gb), dim3(kScanBlock), 0, st, d, g, (int)kGuardRebuild);
-  hipLaunchKernelGGL(k_deg, dim3(nblocks(g.nb, 256)), dim3(256), 0, st, d, g);
+  const unsigned gs = sgrid(g.nb);
+  const int64_t ncol = (int64_t)g.n1d * g.n1d;
+  hipLaunchKernelGGL(k_keys, dim3(gs), dim3(kScanBlock), 0, st, d, g);
+  launch_scan(d.col_count, d.col_start, ncol, d.scan_tmp, d.ctl, kGuardRebuild, st);
+  hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(kScanBlock), 0, st, d, g, (int)kGuardRebuild);
+  hipLaunchKernelGGL(k_colsort, dim3((unsigned)std::min<int64_t>(ncol, 8192)), dim3(64), 0, st,
+                     d, g, (int)kGuardRebuild);
+  const unsigned gl = std::min(nblocks(g.nb, 256), kListBlocks);
+  hipLaunchKernelGGL(k_deg, dim3(gl), dim3(256), 0, st, d, g);
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
-  hipLaunchKernelGGL(k_pull, dim3(nblocks(g.ns > g.nb ? g.ns : g.nb, 256)), dim3(256), 0, st,
-                     d, g);
+  hipLaunchKernelGGL(k_pull, dim3(std::min(nblocks(g.ns > g.nb ? g.ns : g.nb, 256), kListBlocks)),
+                     dim3(256), 0, st, d, g);
 }
 
-// Cell membership (cell_start, sorted) of the last rebuild from the stored
-// keys; after a checkpoint restore the crosslink partner search needs the
-// list the saved state was stepping with.
+// The cell list of the last rebuild (sorted, runs, occupancy) from the
+// stored keys; after a checkpoint restore the crosslink partner search
+// needs the list the saved state was stepping with.
 void launch_cells_from_keys(const Dev &d, const Geo &g, hipStream_t st) {
+  const int64_t ncol = (int64_t)g.n1d * g.n1d;
   hipLaunchKernelGGL(k_key_count, dim3(sgrid(g.nb)), dim3(256), 0, st, d, g);
-  launch_scan(d.cell_count, d.cell_start, g.n_cells, d.scan_tmp, nullptr, 0, st, d.cell_occ);
+  launch_scan(d.col_count, d.col_start, ncol, d.scan_tmp, nullptr, 0, st);
   hipLaunchKernelGGL(k_scatter, dim3(sgrid(g.nb)), dim3(256), 0, st, d, g, 0);
+  hipLaunchKernelGGL(k_colsort, dim3((unsigned)std::min<int64_t>(ncol, 8192)), dim3(64), 0, st,
+                     d, g, 0);
 }
 
 // Replays the displacement bookkeeping of the last full step after the host

@@ -13,14 +13,17 @@
 //
 // Bond state is one record per bond (scaled position, orientation,
 // length): 64 bytes of FP64 on the FP64 path, so each gather touches half a
-// cache line; 32 bytes on the mixed path (FP32 offset inside the bond's
-// cell, orientation and length; the scaled position is rebuilt in FP64).
+// cache line; 32 bytes on the mixed path (32-bit fixed-point offset inside
+// the bond's cell and the packed cell index, octahedral orientation, length
+// as two FP32 values), decoded to a position in cell units in FP64.
 //
 // FP64 path: bit-for-bit the reference's arithmetic.  Mixed path: the same
 // FP64 geometry and branch rules with the divisions and square roots taken
 // from the hardware FP32 reciprocal / reciprocal square root (the latter
 // refined in FP64), and FP32 force slots.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "sc_internal.h"
 #include "sc_math.h"
@@ -29,20 +32,45 @@ namespace sc {
 
 namespace {
 
+// A bond as the pair pass uses it.  FP64 path: the scaled position, as
+// Bond::ReInit stores it.  Mixed path: the position in cell units, cell
+// index plus the record's 32-bit offset (exact in FP64), so that the
+// separation of two bonds is an exact difference and one multiplication.
 struct BondRec {
   double s[3], u[3], len;
 };
 
-// a bond's record: 64 bytes on the FP64 path, 32 on the mixed one (the
-// layout is a template argument here, so each kernel reads one)
 template <bool MIXED>
 __device__ __forceinline__ BondRec load_rec(const double *br, const Geo &g, int64_t b) {
   BondRec r;
-  if (MIXED)
-    rec_read_compact(br, g.inv_n1d, b, r.s, r.u, &r.len);
-  else
+  if (MIXED) {
+    const uint4 *rec = reinterpret_cast<const uint4 *>(br + 4 * b);
+    const uint4 a = rec[0], v = rec[1];
+    const int packed = (int)v.z;
+    constexpr double k32 = 1.0 / 4294967296.0;
+    r.s[0] = (double)(packed & 1023) + (double)a.x * k32;
+    r.s[1] = (double)((packed >> 10) & 1023) + (double)a.y * k32;
+    r.s[2] = (double)((packed >> 20) & 1023) + (double)a.z * k32;
+    oct_decode(v.x, v.y, r.u);
+    r.len = (double)__uint_as_float(a.w) + (double)__uint_as_float(v.w);
+  } else {
     rec_read_full(br, b, r.s, r.u, &r.len);
+  }
   return r;
+}
+
+// Mixed path: separation r2 - r1 of two cell-unit positions with the
+// minimum-image rule of the reference's ds -= NINT(ds) (|ds| >= half the
+// box wraps), times the cell length.
+__device__ __forceinline__ void sep_cells(const Geo &g, const double *p1, const double *p2,
+                                          double *sep) {
+  const double n = (double)g.n1d, half = 0.5 * (double)g.n1d;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double ds = p2[k] - p1[k];
+    const double w = ds >= half ? ds - n : (ds <= -half ? ds + n : ds);
+    sep[k] = w * (g.h[4 * k] * g.inv_n1d);
+  }
 }
 
 // Mixed-precision MinimumDistance::Sphero (src/minimum_distance.cpp:231-343):
@@ -147,36 +175,47 @@ __device__ __forceinline__ int64_t logical_block(int xcd) {
   return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
-// Contribution of incidence q {partner, role | written << 1} to its bond
-// ("own"): the pair computed with the candidate's own (obj1, obj2)
-// orientation, the incidence's slot copy updated when in range (and marked
-// written), then +F and contact1 x F for obj1, -F and -(contact2 x F) for
-// obj2 (src/interaction_manager.cpp:448-517).  Both owners of a candidate
-// run the same instructions on the same inputs and obtain the same bits, so
-// their slot copies stay equal; out of range the copy's force is used, 0
-// if it was not written since the rebuild (the reference's fresh pair
-// record).
-// The slot's fourth value is the pair's signed distance (pair_force64):
-// with the force it gives the pair's stress and potential energy, so the
-// sticky record needs nothing else.  `rs_out` receives it (0 for a slot not
-// written since the rebuild), `fc_out` the candidate's force.
+// The pair of incidence q {partner, role | written << 1 | owner << 2},
+// computed with the candidate's own (obj1, obj2) orientation
+// (src/interaction_manager.cpp:448-498): the incidence's slot copy updated
+// when in range (and marked written), else the copy's force, 0 if it was
+// not written since the rebuild (the reference's fresh pair record).  Both
+// owners of a candidate run the same instructions on the same inputs and
+// obtain the same bits, so their slot copies stay equal.  The slot's fourth
+// value is the pair's signed distance (pair_force64): with the force it
+// gives the pair's stress and potential energy, so the sticky record needs
+// nothing else.  Returns the candidate's force in fc, the owner's contact
+// parameter (lambda for obj1, mu for obj2) in *sc and the signed distance
+// in *rs (0 for a slot not written since the rebuild).
 template <bool MIXED>
-__device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const BondRec &own,
-                                             int32_t q, int2 ent, const BondRec &oth,
-                                             double *cf, double *ct, double *fc_out,
-                                             double *rs_out) {
+__device__ __forceinline__ void pair_slot(const Dev &d, const Geo &g, const BondRec &own,
+                                          int32_t q, int2 ent, const BondRec &oth, double *fc,
+                                          double *sc, double *rs_out) {
   const bool second = (ent.y & 1) != 0;  // this bond is obj2 of the candidate
   const bool written = (ent.y & 2) != 0;
-  const BondRec &r1 = second ? oth : own;
-  const BondRec &r2 = second ? own : oth;
-  double fc[3] = {0.0, 0.0, 0.0}, lam, mu, rs = 0.0;
+  // obj1 and obj2 by selection (no branch on the role)
+  double u1[3], u2[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    u1[k] = second ? oth.u[k] : own.u[k];
+    u2[k] = second ? own.u[k] : oth.u[k];
+  }
+  const double l1 = second ? oth.len : own.len;
+  const double l2 = second ? own.len : oth.len;
+  double lam, mu, rs = 0.0;
+  fc[0] = fc[1] = fc[2] = 0.0;
   bool wrote = false;
   if (!MIXED) {
-    double dr[3], dr2;
-    sphero64(g.h, r1.s, r1.u, r1.len, r2.s, r2.u, r2.len, dr, &dr2, &lam, &mu);
+    double s1[3], s2[3], dr[3], dr2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      s1[k] = second ? oth.s[k] : own.s[k];
+      s2[k] = second ? own.s[k] : oth.s[k];
+    }
+    sphero64(g.h, s1, u1, l1, s2, u2, l2, dr, &dr2, &lam, &mu);
     if (!(dr2 > g.rcut2)) {
       double f[3];
-      if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f, &rs)) {
+      if (pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, u1, u2, f, &rs)) {
         fc[0] = f[0];
         fc[1] = f[1];
         fc[2] = f[2];
@@ -192,21 +231,20 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
       rs = o.w;
     }
   } else {
+    // obj2 - obj1: the owner-to-partner separation, negated for obj2 (the
+    // minimum-image rule is odd, so both owners get the same bits)
     double sep[3], dr[3], dr2;
+    sep_cells(g, own.s, oth.s, sep);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double ds = r2.s[k] - r1.s[k];
-      ds -= (double)nint(ds);
-      sep[k] = g.h[4 * k] * ds;
-    }
-    sphero_mixed(sep, r1.u, 0.5 * r1.len, r2.u, 0.5 * r2.len, dr, &dr2, &lam, &mu);
+    for (int k = 0; k < 3; ++k) sep[k] = second ? -sep[k] : sep[k];
+    sphero_mixed(sep, u1, 0.5 * l1, u2, 0.5 * l2, dr, &dr2, &lam, &mu);
     if (!(dr2 > g.rcut2)) {
       float4 f4;
       if (dr2 < 1.0e-12) {
         // MaxForcePotential branch (include/simcore/max_force_potential.hpp:
         // 11-40), FP64 as the reference; practically never taken
         double f[3];
-        wrote = pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, r1.u, r2.u, f, &rs) != 0;
+        wrote = pair_force64(g.c12, g.c6, g.fcut, dr, dr2, lam, mu, u1, u2, f, &rs) != 0;
         f4 = make_float4((float)f[0], (float)f[1], (float)f[2], (float)rs);
       } else {
         f4 = wca_mixed(g, dr, dr2);
@@ -228,24 +266,20 @@ __device__ __forceinline__ void pair_contrib(const Dev &d, const Geo &g, const B
       rs = o.w;
     }
   }
-  fc_out[0] = fc[0];
-  fc_out[1] = fc[1];
-  fc_out[2] = fc[2];
   *rs_out = rs;
+  *sc = second ? mu : lam;
   if (wrote && !written) reinterpret_cast<int32_t *>(d.inc)[2 * (int64_t)q + 1] = ent.y | 2;
-  // contact of this bond and the torque cross product
-  // (src/interaction_manager.cpp:469-470, src/minimum_distance.cpp:339-342)
-  double con[3], t[3];
-  const double sc = second ? mu : lam;
-  for (int k = 0; k < 3; ++k) con[k] = sc * own.u[k];
-  cross3(con, fc, t);
-  if (!second) {
-    for (int k = 0; k < 3; ++k) cf[k] = fc[k];
-    for (int k = 0; k < 3; ++k) ct[k] = t[k];
-  } else {
-    for (int k = 0; k < 3; ++k) cf[k] = -fc[k];
-    for (int k = 0; k < 3; ++k) ct[k] = -t[k];
-  }
+}
+
+// The owner's share of a pair (src/interaction_manager.cpp:469-470,
+// 500-517; src/minimum_distance.cpp:339-342): +F and contact1 x F for obj1,
+// -F and -(contact2 x F) for obj2.  With f the signed force (-F for obj2)
+// the torque is contact x f, the same bits as -(contact x F).
+__device__ __forceinline__ void pair_share(const double *u, double sc, const double *f,
+                                           double *t) {
+  double con[3];
+  for (int k = 0; k < 3; ++k) con[k] = sc * u[k];
+  cross3(con, f, t);
 }
 
 // Per-bond sums, accumulated in FP64; FP32 storage on the mixed path (one
@@ -336,8 +370,11 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
         ent_n = d.inc[q + 1];
         oth_n = load_rec<MIXED>(d.br, g, ent_n.x);
       }
-      double cf[3], ct[3], fc[3], rs;
-      pair_contrib<MIXED>(d, g, own, q, ent, oth, cf, ct, fc, &rs);
+      double fc[3], sc, rs, cf[3], ct[3];
+      pair_slot<MIXED>(d, g, own, q, ent, oth, fc, &sc, &rs);
+      const double sg = (ent.y & 1) ? -1.0 : 1.0;
+      for (int k = 0; k < 3; ++k) cf[k] = sg * fc[k];
+      pair_share(own.u, sc, cf, ct);
       for (int k = 0; k < 3; ++k) F[k] += cf[k];
       for (int k = 0; k < 3; ++k) T[k] += ct[k];
       if (THERMO) {
@@ -356,15 +393,21 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
 // Load-balanced variant: one wavefront per 64 bonds.  A bond's candidate
 // count varies (0 to ~10), and in the per-bond kernel a wave runs as long
 // as its busiest lane.  Here the wave's incidences (contiguous in the CSR)
-// are dealt out to the lanes in chunks of kChunk, the contributions staged
-// in LDS, and every lane then adds its own bond's contributions in
+// are dealt out to the lanes in chunks of kChunk: each lane evaluates its
+// pairs against the owner's record (decoded once per bond into LDS, the
+// owner found from the entry's owner field) and stages the owner's force
+// and torque shares in LDS; every lane then adds its own bond's shares in
 // incidence order -- the same values summed in the same order as the
-// per-bond kernel, so the results are identical bit for bit.
+// per-bond kernel, so the results are identical bit for bit.  The mixed
+// path's force shares are FP32 values and are staged as such.
 constexpr int kChunk = 128;
 
 template <bool MIXED, bool THERMO>
-__global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
-  __shared__ double cF[kChunk][3], cT[kChunk][3], cE[THERMO ? kChunk : 1];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_bal(Dev d, Geo g, int xcd) {
+  using FT = typename std::conditional<MIXED, float, double>::type;
+  __shared__ BondRec ownr[64];
+  __shared__ FT cF[kChunk][3];
+  __shared__ double cT[kChunk][3], cE[THERMO ? kChunk : 1];
   __shared__ int32_t qs[65];
   if (d.ctl->abort) return;
   const int64_t nb = g.nb;
@@ -374,8 +417,7 @@ __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
   const bool valid = b < nb;
   qs[lane] = d.inc_start[valid ? b : nb];
   if (lane == 0) qs[64] = d.inc_start[b0 + 64 <= nb ? b0 + 64 : nb];
-  __syncthreads();
-  const int32_t Q0 = qs[0], Q1 = qs[64];
+  if (valid) ownr[lane] = load_rec<MIXED>(d.br, g, b);
   double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0}, E = 0.0;
   double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (valid && g.nx > 0) {
@@ -388,40 +430,33 @@ __global__ __launch_bounds__(64) void k_pair_bal(Dev d, Geo g, int xcd) {
       for (int k = 0; k < 3; ++k) T[k] += o[3 + 3 * a + k];
     }
   }
+  __syncthreads();
+  const int32_t Q0 = qs[0], Q1 = qs[64];
   const int32_t my0 = qs[lane], my1 = qs[lane + 1];
   for (int32_t cs = Q0; cs < Q1; cs += kChunk) {
-#pragma unroll
-    for (int kk = 0; kk < kChunk / 64; ++kk) {
-      const int32_t q = cs + kk * 64 + lane;
-      if (q < Q1) {
-        // owning bond: the last l with qs[l] <= q
-        int lo = 0, hi = 64;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (qs[mid] <= q)
-            lo = mid;
-          else
-            hi = mid;
-        }
-        const BondRec own = load_rec<MIXED>(d.br, g, b0 + lo);
-        const int2 ent = d.inc[q];
-        const BondRec oth = load_rec<MIXED>(d.br, g, ent.x);
-        double cf[3], ct[3], fc[3], rs;
-        pair_contrib<MIXED>(d, g, own, q, ent, oth, cf, ct, fc, &rs);
-        for (int k = 0; k < 3; ++k) {
-          cF[q - cs][k] = cf[k];
-          cT[q - cs][k] = ct[k];
-        }
-        if (THERMO) {
-          cE[q - cs] = pair_pote(g.c12, g.c6, g.eps, rs);
-          if (!(ent.y & 1)) add_stress(fc, rs, S);
-        }
+    const int32_t ce = Q1 < cs + kChunk ? Q1 : cs + kChunk;
+    for (int32_t q = cs + lane; q < ce; q += 64) {
+      const int2 ent = d.inc[q];
+      const BondRec oth = load_rec<MIXED>(d.br, g, ent.x);
+      const BondRec own = ownr[(ent.y >> 2) & 63];
+      double fc[3], sc, rs, cf[3], ct[3];
+      pair_slot<MIXED>(d, g, own, q, ent, oth, fc, &sc, &rs);
+      const double sg = (ent.y & 1) ? -1.0 : 1.0;
+      for (int k = 0; k < 3; ++k) cf[k] = sg * fc[k];
+      pair_share(own.u, sc, cf, ct);
+      for (int k = 0; k < 3; ++k) {
+        cF[q - cs][k] = (FT)cf[k];
+        cT[q - cs][k] = ct[k];
+      }
+      if (THERMO) {
+        cE[q - cs] = pair_pote(g.c12, g.c6, g.eps, rs);
+        if (!(ent.y & 1)) add_stress(fc, rs, S);
       }
     }
     __syncthreads();
-    const int32_t e = my1 < cs + kChunk ? my1 : cs + kChunk;
+    const int32_t e = my1 < ce ? my1 : ce;
     for (int32_t q = my0 > cs ? my0 : cs; q < e; ++q) {
-      for (int k = 0; k < 3; ++k) F[k] += cF[q - cs][k];
+      for (int k = 0; k < 3; ++k) F[k] += (double)cF[q - cs][k];
       for (int k = 0; k < 3; ++k) T[k] += cT[q - cs][k];
       if (THERMO) E += cE[q - cs];
     }
