@@ -336,7 +336,7 @@ def main():
     tune = None
     if args.launch_shape:
         keys = ["pair_block", "pair_xcd", "integrate_waves", "integrate_layout",
-                "integrate_streams"]
+                "integrate_streams", "pair_parts"]
         shape = dict(zip(keys, (int(v) for v in args.launch_shape.split(","))))
         dev.set_launch_shape(**shape)
         tune = dict(dev.launch_shape(), set="fixed")

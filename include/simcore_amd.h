@@ -247,9 +247,15 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
  *                       wavefronts (same results as 1)
  *   integrate_streams   0-3: the integrator's length bins are spread over the
  *                       system's stream and this many side streams (forked
- *                       and joined inside the step) and run concurrently */
+ *                       and joined inside the step) and run concurrently
+ *   pair_parts          0/1: the pair pass, then the integrator; 2-8: the
+ *                       bonds are cut into this many ranges, and the
+ *                       integrator of the filaments below a cut runs on a
+ *                       side stream beside the pair pass of the next range
+ *                       (systems without crosslinks; results unchanged) */
 typedef struct sc_launch_shape {
   int32_t pair_block, pair_xcd, integrate_waves, integrate_layout, integrate_streams;
+  int32_t pair_parts;
 } sc_launch_shape;
 int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape);
 int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
@@ -257,7 +263,8 @@ int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
 /* Autotuner (no counterpart in the reference): times the launch shapes
  * that leave every result unchanged -- first the pair kernel's (block size
  * or balanced variant, XCD order), then the integrator's (waves per block,
- * side streams; the layout is kept) -- over `steps` steps each
+ * side streams; the layout is kept), then the overlap of the two over bond
+ * ranges -- over `steps` steps each
  * from a checkpoint of the current state, keeps the fastest and restores
  * the state, so the trajectory is the one without tuning.  The cell size
  * is not tuned: in the reference it sets the rebuild threshold and the

@@ -24,6 +24,7 @@ struct Bin {
   int maxn = 0;
   int count = 0;
   int32_t *list = nullptr;
+  std::vector<int64_t> last;  // last bond of each listed filament (ascending)
 };
 
 constexpr int kFamilies = 4;  // pair+scatter, commit, integrate, rebuild
@@ -64,6 +65,9 @@ struct sc_system {
   // pinned Ctl mirror is refreshed at the end of every enqueued range and
   // `done_ev` marks the end of the last one
   int64_t i_step_host = 0;  // i_step when nothing is pending
+  // the two bond-record buffers: the state after i completed steps is in
+  // rec[i & 1] (each step reads one and writes the other)
+  double *rec[2] = {nullptr, nullptr};
   int64_t target = 0;
   bool pending = false;
   int grow_rounds = 0;
@@ -74,6 +78,9 @@ struct sc_system {
   static constexpr int kSide = 3;
   hipStream_t side[kSide] = {};
   hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
+  // overlapped pair pass / integrator: one event per bond range
+  hipEvent_t part_ev[sc::kMaxParts] = {};
+  std::vector<int64_t> pack_last;  // per packed wave, its filaments' last bond
   // allocations (from `arena` when set, else hipMalloc)
   sc::Arena *arena = nullptr;
   bool own_stream = true;
@@ -298,6 +305,16 @@ int write_ctl(sc_system *h, const Ctl &c) {
   return SC_OK;
 }
 
+// Points d.br / d.br_out at the records of the state after i_step steps.
+// Host code that reads the records or launches kernels on the current
+// state calls it once the stream is idle (ctl_host then holds i_step).
+void refresh_records(sc_system *h) {
+  if (!h->rec[0]) return;
+  double *cur = h->rec[h->ctl_host->i_step & 1];
+  h->d.br = cur;
+  h->d.br_out = cur;
+}
+
 // The side streams and join events of the integrator's length bins, made
 // the first time a launch shape asks for them (most systems never do, and
 // an ensemble admits many systems).
@@ -307,6 +324,9 @@ int ensure_side_streams(sc_system *h) {
     if (hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->join_ev[k], hipEventDisableTiming) != hipSuccess)
       return fail(h, SC_ERR_HIP, "side stream creation failed");
+  for (int k = 0; k < sc::kMaxParts; ++k)
+    if (hipEventCreateWithFlags(&h->part_ev[k], hipEventDisableTiming) != hipSuccess)
+      return fail(h, SC_ERR_HIP, "event creation failed");
   return SC_OK;
 }
 
@@ -323,7 +343,10 @@ bool midsteps_in_buffer(const sc_system *h, int64_t from, int64_t n) {
 // One step.  `odd` says whether the step index is odd: only odd steps can
 // rebuild, so even steps skip the rebuild stage's launches.
 void enqueue_step(sc_system *h, bool odd) {
-  const Dev &d = h->d;
+  // an odd step k starts from the state after k - 1 (even) steps
+  Dev d = h->d;
+  d.br = h->rec[odd ? 0 : 1];
+  d.br_out = h->rec[odd ? 1 : 0];
   const Geo &g = h->g;
   hipStream_t st = h->stream;
   const bool pf = h->prof;
@@ -332,6 +355,60 @@ void enqueue_step(sc_system *h, bool odd) {
   if (pf) {
     (void)hipEventRecord(h->ev[7], st);
     (void)hipEventRecord(h->ev[0], st);
+  }
+  // Overlapped pair pass and integrator (no crosslinks, not profiling): the
+  // bonds are cut into P ranges; the pair pass runs range by range on the
+  // system's stream, and the integrator of the filaments whose bonds all
+  // lie below a cut runs on a side stream as soon as the ranges below it
+  // are done, beside the pair pass of the next range.  Each step reads one
+  // record buffer and writes the other, so the two never touch the same
+  // memory; the results are those of the sequential order.
+  const int P = std::min(h->tune.parts, sc::kMaxParts);
+  if (P > 1 && g.nx == 0 && h->side[0] && !pf) {
+    const int64_t nb = g.nb;
+    int64_t cut[sc::kMaxParts + 1];
+    cut[0] = 0;
+    for (int p = 1; p < P; ++p) cut[p] = std::max(cut[p - 1], nb * p / P / sc::kPairAlign * sc::kPairAlign);
+    cut[P] = nb;
+    hipStream_t si = h->side[0];
+    for (int p = 0; p < P; ++p) {
+      sc::launch_pair_range(d, g, h->p.precision, h->tune, st, g.thermo != 0, cut[p], cut[p + 1]);
+      (void)hipEventRecord(h->part_ev[p], st);
+    }
+    const bool packed = h->tune.int_layout == 2 && h->pack_waves > 0;
+    // first entry whose last bond is at or above bond c
+    auto first_at = [](const std::vector<int64_t> &v, int64_t c) -> int {
+      return (int)(std::lower_bound(v.begin(), v.end(), c) - v.begin());
+    };
+    // packed filaments on one side stream, the longer ones (their own
+    // launches, latency-bound) on a second
+    hipStream_t sl = h->side[1];
+    for (int p = 0; p < P; ++p) {
+      (void)hipStreamWaitEvent(si, h->part_ev[p], 0);
+      (void)hipStreamWaitEvent(sl, h->part_ev[p], 0);
+      const int64_t lo = cut[p], hi = p + 1 < P ? cut[p + 1] : nb + 1;
+      if (packed) {
+        const int w0 = p == 0 ? 0 : first_at(h->pack_last, lo);
+        const int w1 = p + 1 < P ? first_at(h->pack_last, hi) : h->pack_waves;
+        if (w1 > w0)
+          sc::launch_integrate_packed(d, g, h->pack_tab + (int64_t)64 * w0, h->pack_code + w0,
+                                      w1 - w0, h->tune.int_waves, odd, si);
+      }
+      for (const Bin &b : h->bins) {
+        if (packed && b.maxn <= 128) continue;
+        const int i0 = p == 0 ? 0 : first_at(b.last, lo);
+        const int i1 = p + 1 < P ? first_at(b.last, hi) : b.count;
+        if (i1 > i0)
+          sc::launch_integrate(d, g, b.list + i0, i1 - i0, b.maxn, h->p.precision, h->tune,
+                               odd, packed ? sl : si);
+      }
+    }
+    for (int k = 0; k < 2; ++k) {
+      (void)hipEventRecord(h->join_ev[k], h->side[k]);
+      (void)hipStreamWaitEvent(st, h->join_ev[k], 0);
+    }
+    sc::launch_commit(d, g, st);
+    return;
   }
   if (!odd) sc::launch_xlink_forces(d, g, st);
   sc::launch_pair(d, g, h->p.precision, h->tune, st, g.thermo != 0);
@@ -500,7 +577,11 @@ std::vector<int32_t> spatial_permutation(const sc_params &p, int32_t nf, const i
     const int n = so[f + 1] - so[f];
     uint64_t bin = 0;
     while (bin < 5 && n > (8 << bin)) ++bin;  // bins of 8, 16, ..., 256 sites
-    key[f] = ((uint64_t)(code >> 21) << 33) | (bin << 30) | code;
+    // filaments of more than 128 sites first: their bonds then fall in the
+    // first bond range, so with the pair pass and the integrator
+    // overlapped, their (latency-bound) integrator starts earliest
+    const uint64_t late = n > 128 ? 0 : 1;
+    key[f] = (late << 42) | ((uint64_t)(code >> 21) << 33) | (bin << 30) | code;
   }
   std::vector<int32_t> perm(nf);
   for (int f = 0; f < nf; ++f) perm[f] = f;
@@ -954,7 +1035,9 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     if ((rc = dalloc(h, a, 3 * (size_t)ns, false))) return rc;
   if ((rc = dalloc(h, &d.acc, ns, false))) return rc;
   if ((rc = dalloc(h, &d.acc_prev, ns, false))) return rc;
-  if ((rc = dalloc(h, &d.br, (g.compact ? 4 : 8) * (size_t)nb, false))) return rc;
+  for (int k = 0; k < 2; ++k)
+    if ((rc = dalloc(h, &h->rec[k], (g.compact ? 4 : 8) * (size_t)nb, false))) return rc;
+  d.br = d.br_out = h->rec[0];
   if (h->p.precision == SC_PREC_FP64) {
     if ((rc = dalloc(h, &d.bf, 3 * (size_t)nb, false))) return rc;
     if ((rc = dalloc(h, &d.bt, 3 * (size_t)nb, false))) return rc;
@@ -1047,6 +1130,7 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     Bin b;
     b.maxn = bin_max[k];
     b.count = (int)lists[k].size();
+    for (int32_t f : lists[k]) b.last.push_back((int64_t)site_off[f + 1] - f - 2);
     if ((rc = dalloc(h, &b.list, lists[k].size(), false))) return rc;
     HIPCHK(h, hipMemcpyAsync(b.list, lists[k].data(), sizeof(int32_t) * lists[k].size(),
                              hipMemcpyHostToDevice, h->stream));
@@ -1058,6 +1142,28 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     // caller-side ids -> storage indices
     std::vector<int32_t> st_of(nf);
     for (int f = 0; f < nf; ++f) st_of[h->reordered ? h->perm[f] : f] = f;
+    // waves in the order of their filaments' last bond, so that the waves
+    // whose bonds all lie below a cut form a prefix (overlapped steps)
+    {
+      std::vector<int64_t> wl(waves.size(), -1);
+      for (size_t wi = 0; wi < waves.size(); ++wi)
+        for (int l = 0; l < 64; ++l)
+          if (waves[wi].lane[l] >= 0) {
+            const int32_t f = st_of[waves[wi].lane[l]];
+            wl[wi] = std::max<int64_t>(wl[wi], (int64_t)site_off[f + 1] - f - 2);
+          }
+      std::vector<size_t> ord(waves.size());
+      for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+      std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wl[a] < wl[b]; });
+      std::vector<PackWave> sorted_waves;
+      sorted_waves.reserve(waves.size());
+      h->pack_last.clear();
+      for (size_t i : ord) {
+        sorted_waves.push_back(waves[i]);
+        h->pack_last.push_back(wl[i]);
+      }
+      waves.swap(sorted_waves);
+    }
     const size_t nw = waves.size();
     tab.assign(64 * nw, -1);
     code.assign(nw, 0);
@@ -1119,6 +1225,7 @@ int sc_load_crosslinks(sc_handle h, int64_t n, const int32_t *bonds, const doubl
   Dev &d = h->d;
   Geo &g = h->g;
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  refresh_records(h);
   if ((rc = dalloc(h, &d.xb, 2 * n, false))) return rc;
   if ((rc = dalloc(h, &d.xlam, 2 * n, false))) return rc;
   if ((rc = dalloc(h, &d.xf, 9 * n, false))) return rc;
@@ -1222,6 +1329,7 @@ int sc_step_poll(sc_handle h, int *done) {
   HIPCHK(h, hipGetLastError());
   Ctl c = *h->ctl_host;
   h->i_step_host = c.i_step;
+  refresh_records(h);
   if (!c.abort) {
     h->pending = false;
     if (done) *done = 1;
@@ -1275,6 +1383,7 @@ int sc_get_bonds(sc_handle h, double *pos, double *spos, double *u, double *len)
   int rc = check_handle(h, true);
   if (rc) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  refresh_records(h);
   const int64_t nb = h->g.nb, ns = h->g.ns;
   std::vector<double> sp(3 * (size_t)nb), uu(3 * (size_t)nb), ln(nb);
   if (h->g.compact) {
@@ -1524,6 +1633,7 @@ int sc_set_sites(sc_handle h, const double *r) {
   if (!r) return fail(h, SC_ERR_ARG, "null input");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  refresh_records(h);
   std::vector<double> st;
   if (h->reordered) {
     const int64_t ns = h->g.ns;
@@ -1589,6 +1699,8 @@ int sc_filament_stats(sc_handle h, double *e2e, double *com) {
   const int64_t n = 3 * (int64_t)h->g.nf;
   double *buf = nullptr;
   HIPCHK(h, hipMalloc(&buf, sizeof(double) * 2 * n));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  refresh_records(h);
   sc::launch_filament_stats(h->d, h->g, buf, buf + n, h->stream);
   std::vector<double> tmp(2 * (size_t)n);
   hipError_t e = hipMemcpyAsync(tmp.data(), buf, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
@@ -1617,6 +1729,8 @@ int sc_theta_histogram(sc_handle h, int32_t n_bins, int32_t n_rows, int64_t *his
   std::vector<unsigned long long> host(n);
   hipError_t e = hipMemsetAsync(buf, 0, sizeof(unsigned long long) * n, h->stream);
   if (e == hipSuccess) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    refresh_records(h);
     sc::launch_theta_hist(h->d, h->g, n_bins, n_rows, buf, h->stream);
     e = hipMemcpyAsync(host.data(), buf, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost,
                        h->stream);
@@ -1642,6 +1756,8 @@ int sc_theta_histogram_noalloc(sc_handle h, int32_t n_bins, int32_t n_rows, int6
   if (!buf) return fail(h, SC_ERR_STATE, "no scratch address for this device");
   // the array keeps the last call's counts: clear the part this call uses
   // (a fill kernel, not a memset node, so the call may be captured)
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  refresh_records(h);
   sc::launch_zero_i32(reinterpret_cast<int32_t *>(buf), 2 * n, h->stream);
   sc::launch_theta_hist(h->d, h->g, n_bins, n_rows, buf, h->stream);
   HIPCHK(h, hipGetLastError());
@@ -1713,6 +1829,7 @@ int sc_checkpoint_save(sc_handle h, void *buf, size_t cap, size_t *size) {
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const Ctl c = read_ctl(h);
+  h->d.br = h->d.br_out = h->rec[c.i_step & 1];
   const size_t need = ck_size(h, c.nc);
   if (size) *size = need;
   if (!buf) return SC_OK;
@@ -1775,6 +1892,7 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size) {
   if (hd.nc > h->g.cap) {
     if ((rc = alloc_cand(h, hd.nc + hd.nc / 4 + 1024))) return rc;
   }
+  h->d.br = h->d.br_out = h->rec[hd.ctl.i_step & 1];
   for (const CkField &f : ck_fields(h, hd.nc)) {
     if (f.bytes) HIPCHK(h, hipMemcpy(f.dev, p, f.bytes, hipMemcpyHostToDevice));
     p += f.bytes;
@@ -1796,8 +1914,9 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   if (steps < 2) steps = 2;
   steps += steps & 1;  // whole midstep/full-step pairs
-  // the side-stream shapes are tried whenever there are several bins
-  if (h->bins.size() > 1 && (rc = ensure_side_streams(h))) return rc;
+  // the side-stream shapes (several bins, overlapped ranges) need the
+  // side streams
+  if ((rc = ensure_side_streams(h))) return rc;
   size_t n = 0;
   if ((rc = sc_checkpoint_save(h, nullptr, 0, &n))) return rc;
   std::vector<char> blob(n);
@@ -1884,6 +2003,19 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
       t.int_streams = sm;
       consider(t);
     }
+  // overlap of the pair pass and the integrator over bond ranges (systems
+  // without crosslinks); the ranges take the side stream, so the
+  // integrator's own side-stream spreading is off there
+  if (h->g.nx == 0 && h->side[0]) {
+    const sc::Tune int_best = best;
+    const int parts[] = {2, 4, 8};
+    for (int pp : parts) {
+      sc::Tune t = int_best;
+      t.parts = pp;
+      t.int_streams = 0;
+      consider(t);
+    }
+  }
   // leave the system where it was, with the chosen shapes
   h->tune = rc ? start : best;
   drop_graph(h);
@@ -1914,7 +2046,8 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
       (v.pair_xcd != 0 && v.pair_xcd != 1) ||
       (v.integrate_waves != 1 && v.integrate_waves != 2 && v.integrate_waves != 4) ||
       (v.integrate_layout < 0 || v.integrate_layout > 2) ||
-      (v.integrate_streams < 0 || v.integrate_streams > 3))
+      (v.integrate_streams < 0 || v.integrate_streams > 3) ||
+      (v.pair_parts < 0 || v.pair_parts > sc::kMaxParts))
     return fail(h, SC_ERR_ARG, "unsupported launch shape");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   h->tune.pair_bal = v.pair_block == 0 ? 1 : 0;
@@ -1923,8 +2056,9 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
   h->tune.int_waves = v.integrate_waves;
   h->tune.int_layout = v.integrate_layout;
   h->tune.int_streams = v.integrate_streams;
+  h->tune.parts = v.pair_parts > 1 ? v.pair_parts : 1;
   drop_graph(h);
-  if (v.integrate_streams > 0) return ensure_side_streams(h);
+  if (v.integrate_streams > 0 || v.pair_parts > 1) return ensure_side_streams(h);
   return SC_OK;
 }
 
@@ -1938,6 +2072,7 @@ int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape) {
   shape->integrate_waves = t.int_waves;
   shape->integrate_layout = t.int_layout;
   shape->integrate_streams = t.int_streams;
+  shape->pair_parts = t.parts;
   return SC_OK;
 }
 
@@ -1979,6 +2114,12 @@ int sc_time_pair(sc_handle h, int32_t launches, double *avg_ms) {
     (void)hipEventDestroy(e0);
     return fail(h, SC_ERR_HIP, "hipEventCreate");
   }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return fail(h, SC_ERR_HIP, "hipStreamSynchronize");
+  }
+  refresh_records(h);
   // the pass rewrites the same slot values and sums on an unchanged state,
   // so repeating it leaves the system as it was
   sc::launch_pair(h->d, h->g, h->p.precision, h->tune, h->stream, false);  // warm
@@ -2052,6 +2193,8 @@ int sc_destroy(sc_handle h) {
     }
     if (h->join_ev[k]) (void)hipEventDestroy(h->join_ev[k]);
   }
+  for (int k = 0; k < sc::kMaxParts; ++k)
+    if (h->part_ev[k]) (void)hipEventDestroy(h->part_ev[k]);
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->xi_dev) (void)hipFree(h->xi_dev);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);

@@ -724,8 +724,11 @@ void k_integrate_long(Dev d, Geo g, const int32_t *list,
 // inside one 16-lane row, of 17-32 lanes inside one half, of 33-64 lanes
 // from lane 0; the wave's W code (0: 16, 1: 32, 2: 64) selects the lane
 // moves.  Same arithmetic as the binned layout, far fewer idle lanes.
+// Three waves per SIMD: the three lane-group variants in one kernel need
+// about 140 VGPRs, and the spills of a 128-register budget cost more than
+// the fourth wave gains (measured on the uneven-length system).
 template <bool MID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_integrate_packed(Dev d, Geo g, const int32_t *lane_tab, const int32_t *wave_code,
                         int n_waves) {
   Ctl *ctl = d.ctl;

@@ -93,7 +93,12 @@ struct Dev {
   // sites
   double *r, *rp, *fr, *dr0, *acc, *acc_prev;
   // bonds
+  // bond records, two buffers: a step reads the state's records (br) and
+  // its integrator writes the next state's (br_out), so the pair pass and
+  // the integrator of different filaments may run at the same time.
+  // Outside a step both point at the current state's buffer.
   double *br;              // nb*8: {s.x, s.y, s.z, u.x, u.y, u.z, len, 0}
+  double *br_out;
   double *bf, *bt, *tension;  // bf, bt: FP64 precision only
   float *bf32, *bt32;          // per-bond force / torque sums, mixed precision
   // cell list
@@ -147,7 +152,12 @@ struct Tune {
   int int_layout = 1;    // up to 64 sites: 0 one site per lane, 1 two (the
                          // two round differently; not changed by tuning)
   int int_streams = 0;   // length bins spread over this many side streams (0-3)
+  int parts = 1;         // >1: the bonds are cut into this many ranges; the
+                         // integrator of a range's filaments runs on a side
+                         // stream beside the pair pass of the next range
 };
+constexpr int kMaxParts = 8;
+constexpr int64_t kPairAlign = 512;  // range cuts: a multiple of every block size
 
 // Length bins of the integrator, as the step loop hands them over.
 constexpr int kMaxBins = 6;
@@ -190,7 +200,9 @@ void launch_cells_from_keys(const Dev &d, const Geo &g, hipStream_t st);
 // `thermo`: also sum the pair stress and the bond energies (Geo::thermo)
 void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t,
                  hipStream_t st, bool thermo);
-int64_t pair_blocks(const Geo &g, const Tune &t);  // blocks of one pair launch
+int64_t pair_blocks(const Geo &g, const Tune &t);
+void launch_pair_range(const Dev &d, const Geo &g, int precision, const Tune &t, hipStream_t st,
+                       bool thermo, int64_t b_lo, int64_t b_hi);  // blocks of one pair launch
 // `mid`: the step is a midstep (odd step index)
 void launch_integrate(const Dev &d, const Geo &g, const int32_t *bin_list,
                       int bin_count, int bin_maxn, int precision, const Tune &t, bool mid,

@@ -486,10 +486,10 @@ __device__ __forceinline__ void bond_reinit(const DevT &d, const GeoT &g,
     sp[k] = si;
   }
   if (prec == SC_PREC_MIXED) {
-    rec_write_compact(d.br, g, b, sp, o, l);
+    rec_write_compact(d.br_out, g, b, sp, o, l);
     return;
   }
-  double4 *rec = reinterpret_cast<double4 *>(d.br + 8 * b);
+  double4 *rec = reinterpret_cast<double4 *>(d.br_out + 8 * b);
   rec[0] = make_double4(sp[0], sp[1], sp[2], o[0]);
   rec[1] = make_double4(o[1], o[2], l, 0.0);
 }
@@ -510,7 +510,7 @@ __device__ __forceinline__ void bond_store(const DevT &d, const GeoT &g, int64_t
     si -= (double)nint(si);
     sp[k] = si;
   }
-  rec_write_compact(d.br, g, b, sp, u, l);
+  rec_write_compact(d.br_out, g, b, sp, u, l);
 }
 
 }  // namespace sc

@@ -333,10 +333,10 @@ __device__ __forceinline__ void block_stress(const Dev &d, double *S) {
 }
 
 template <bool MIXED, bool THERMO>
-__global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
+__global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd, int64_t blk0) {
   if (d.ctl->abort) return;
   const int64_t nb = g.nb;
-  const int64_t b = logical_block(xcd) * blockDim.x + threadIdx.x;
+  const int64_t b = (blk0 + logical_block(xcd)) * blockDim.x + threadIdx.x;
   double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (b < nb) {
     const BondRec own = load_rec<MIXED>(d.br, g, b);
@@ -403,7 +403,8 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd) {
 constexpr int kChunk = 128;
 
 template <bool MIXED, bool THERMO>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_bal(Dev d, Geo g, int xcd) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_bal(Dev d, Geo g, int xcd,
+                                                                                           int64_t blk0) {
   using FT = typename std::conditional<MIXED, float, double>::type;
   __shared__ BondRec ownr[64];
   __shared__ FT cF[kChunk][3];
@@ -412,7 +413,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
   if (d.ctl->abort) return;
   const int64_t nb = g.nb;
   const int lane = threadIdx.x;
-  const int64_t b0 = logical_block(xcd) * 64;
+  const int64_t b0 = (blk0 + logical_block(xcd)) * 64;
   const int64_t b = b0 + lane;
   const bool valid = b < nb;
   qs[lane] = d.inc_start[valid ? b : nb];
@@ -477,18 +478,29 @@ int64_t pair_blocks(const Geo &g, const Tune &t) {
 
 void launch_pair(const Dev &d, const Geo &g, int precision, const Tune &t, hipStream_t st,
                  bool thermo) {
-  const dim3 grid((unsigned)pair_blocks(g, t));
+  launch_pair_range(d, g, precision, t, st, thermo, 0, g.nb);
+}
+
+// Bonds [b_lo, b_hi): b_lo a multiple of kPairAlign, and so is b_hi unless
+// it is the last bond.
+void launch_pair_range(const Dev &d, const Geo &g, int precision, const Tune &t, hipStream_t st,
+                       bool thermo, int64_t b_lo, int64_t b_hi) {
+  const int bs = t.pair_bal ? 64 : t.pair_block;
+  const int64_t blk0 = b_lo / bs;
+  const int64_t nblk = (b_hi - b_lo + bs - 1) / bs;
+  if (nblk <= 0) return;
+  const dim3 grid((unsigned)nblk);
   const bool fp64 = precision == SC_PREC_FP64;
-#define SC_PAIR(KERNEL, BS)                                                                   \
-  do {                                                                                        \
-    if (fp64 && thermo)                                                                      \
-      hipLaunchKernelGGL((KERNEL<false, true>), grid, dim3(BS), 0, st, d, g, t.pair_xcd);    \
-    else if (fp64)                                                                            \
-      hipLaunchKernelGGL((KERNEL<false, false>), grid, dim3(BS), 0, st, d, g, t.pair_xcd);   \
-    else if (thermo)                                                                          \
-      hipLaunchKernelGGL((KERNEL<true, true>), grid, dim3(BS), 0, st, d, g, t.pair_xcd);     \
-    else                                                                                      \
-      hipLaunchKernelGGL((KERNEL<true, false>), grid, dim3(BS), 0, st, d, g, t.pair_xcd);    \
+#define SC_PAIR(KERNEL, BS)                                                                    \
+  do {                                                                                         \
+    if (fp64 && thermo)                                                                       \
+      hipLaunchKernelGGL((KERNEL<false, true>), grid, dim3(BS), 0, st, d, g, t.pair_xcd, blk0); \
+    else if (fp64)                                                                             \
+      hipLaunchKernelGGL((KERNEL<false, false>), grid, dim3(BS), 0, st, d, g, t.pair_xcd, blk0); \
+    else if (thermo)                                                                           \
+      hipLaunchKernelGGL((KERNEL<true, true>), grid, dim3(BS), 0, st, d, g, t.pair_xcd, blk0);  \
+    else                                                                                       \
+      hipLaunchKernelGGL((KERNEL<true, false>), grid, dim3(BS), 0, st, d, g, t.pair_xcd, blk0); \
   } while (0)
   if (t.pair_bal)
     SC_PAIR(k_pair_bal, 64);

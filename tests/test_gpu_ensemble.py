@@ -94,3 +94,55 @@ def test_ensemble_rejects_request_larger_than_budget():
         assert info["status"] == SC_ERR_NOMEM
         with pytest.raises(sc.SimcoreError):
             ens.result(rid)
+
+
+def test_ensemble_fp64_requests_match_oracle():
+    """FP64 requests run through the ensemble among mixed ones: each
+    result equals the CPU oracle's positions after the same steps, bit for
+    bit (the ensemble runs the same FP64 path as a system alone)."""
+    from oracle.oracle_py import OracleSystem
+    print("case: 2 FP64 requests among 2 mixed, vs the oracle", flush=True)
+    cases = _systems()
+    fp64 = [cases[0], cases[1]]
+    with sc.Ensemble(hbm_budget=2 << 30, max_active=4, steps_per_launch=16) as ens:
+        ids = [ens.submit(p, f, n, precision="fp64") for p, f, n in fp64]
+        ens.submit(*cases[2][:2], cases[2][2], precision="mixed")
+        ens.submit(*cases[3][:2], cases[3][2], precision="mixed")
+        ens.drain(timeout_s=120)
+        for rid, (p, f, n) in zip(ids, fp64):
+            assert ens.request(rid)["state"] == "done"
+            ora = OracleSystem(p, f)
+            assert ora.step(n) == 0
+            np.testing.assert_array_equal(ens.result(rid), ora.sites()[0])
+            ora.close()
+
+
+def test_ensemble_admits_while_others_run():
+    """Requests submitted while earlier ones are resident and stepping are
+    admitted next to them (no waiting for the earlier ones to finish), and
+    every result equals the system stepped alone."""
+    print("case: submissions during a running request", flush=True)
+    cases = _systems()
+    p0, f0 = c0_system(num=64, seed=9, noise_seed=505)
+    long_steps = 600
+    refs = [_alone(p, f, n) for p, f, n in cases]
+    ref0 = _alone(p0, f0, long_steps)
+    with sc.Ensemble(hbm_budget=2 << 30, max_active=8, steps_per_launch=8) as ens:
+        first = ens.submit(p0, f0, long_steps, precision="mixed")
+        for _ in range(4):
+            ens.progress()
+        assert ens.request(first)["state"] == "running"
+        ids = []
+        for p, f, n in cases:
+            ids.append(ens.submit(p, f, n, precision="mixed"))
+            ens.progress()
+        ens.drain(timeout_s=120)
+        info0 = ens.request(first)
+        for rid, ref in zip(ids, refs):
+            info = ens.request(rid)
+            assert info["state"] == "done"
+            # admitted before the first request completed
+            assert info["t_admit"] < info0["t_done"]
+            np.testing.assert_array_equal(ens.result(rid), ref)
+        np.testing.assert_array_equal(ens.result(first), ref0)
+        assert ens.stats()["peak_active"] >= 2

@@ -376,10 +376,11 @@ def test_autotune_leaves_trajectory_unchanged(precision):
     b.step(31)
     rep = b.autotune(steps=6)
     print(rep)
-    # 8 pair shapes, then 3 waves per block x side streams; the mixed path's
-    # packed integrator is one launch here (no filament above 128 sites), so
-    # side streams are not tried
-    expect = 20 if precision == "fp64" else 11
+    # 8 pair shapes, then 3 waves per block x side streams, then 3 counts of
+    # overlapped bond ranges; the mixed path's packed integrator is one
+    # launch here (no filament above 128 sites), so side streams are not
+    # tried
+    expect = 23 if precision == "fp64" else 14
     assert rep["n_trials"] == expect and rep["step_ms_best"] <= rep["step_ms_start"]
     assert b.counters()["i_step"] == 31
     a.step(40)
@@ -484,6 +485,45 @@ def test_length_bins_on_side_streams_are_bit_identical(precision):
             np.testing.assert_array_equal(x, y)
         np.testing.assert_array_equal(runs[0][1], runs[j][1])
         assert runs[0][2] == runs[j][2]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind", ["uneven2000", "long"])
+def test_overlapped_ranges_are_bit_identical(precision, kind):
+    """The pair pass cut into bond ranges with the integrator of finished
+    ranges on a side stream (graph and eager; 2, 4 and 8 ranges) gives the
+    results of the sequential order: sites, tensions, counters and bond
+    forces identical over 41 steps with rebuilds; FP64 equal to the CPU
+    path."""
+    from simcore_amd import workloads as W
+    from tests.helpers import long_system
+    if kind == "uneven2000":
+        p, fil = W.uneven(nf=2000)[:2]
+    else:
+        p, fil = long_system()
+    print(f"case: {kind} {precision}, overlapped pair/integrator ranges, 41 steps", flush=True)
+    runs = []
+    for parts, graph in ((1, True), (2, True), (4, True), (8, True), (4, False)):
+        s = sc.FilamentSystem(p, fil, precision=precision, use_graph=graph,
+                              spatial_order=precision == "mixed")
+        s.set_launch_shape(pair_block=0, pair_xcd=1, integrate_waves=2,
+                           integrate_layout=2 if precision == "mixed" else 1, pair_parts=parts)
+        assert s.launch_shape()["pair_parts"] == parts
+        s.step(41)
+        runs.append((s.sites(), s.tensions(), s.counters(), s.bond_forces()))
+        s.close()
+    assert runs[0][2]["n_rebuilds"] >= 2 or kind == "long"
+    for j in range(1, len(runs)):
+        for x, y in zip(runs[0][0], runs[j][0]):
+            np.testing.assert_array_equal(x, y)
+        np.testing.assert_array_equal(runs[0][1], runs[j][1])
+        assert runs[0][2] == runs[j][2]
+        for x, y in zip(runs[0][3], runs[j][3]):
+            np.testing.assert_array_equal(x, y)
+    if precision == "fp64":
+        ora = OracleSystem(p, fil)
+        assert ora.step(41) == 0
+        np.testing.assert_array_equal(runs[2][0][0], ora.sites()[0])
 
 
 def test_pair_timing_probe_leaves_state_unchanged():
