@@ -364,7 +364,7 @@ def main():
     log(f"[bench] profile: {json.dumps(prof)} nc={nc} pair back-to-back {pair_warm_ms:.5f} ms")
     balanced = int(dev.launch_shape()["pair_block"]) == 0
     kname = ("k_pair_bal" if balanced else "k_pair_fused") + (
-        "<true>" if args.precision == "mixed" else "<false>")
+        "<true, false>" if args.precision == "mixed" else "<false, false>")
     ttag = f"{TRAFFIC_TAG[args.workload]}_{args.precision}" if args.workload in TRAFFIC_TAG else None
     # the pair pass: the step's largest single kernel on the uneven-length
     # system and the kernel the HBM target names
@@ -385,14 +385,15 @@ def main():
     achieved_int = bytes_int / (int_ms * 1e-3) / 1e9 if int_ms > 0 else 0.0
     int_traffic = None
     if ttag and args.precision == "mixed":
-        tf = measured_traffic_sum(ttag, "k_integrate_long<", ", false>")
-        tm = measured_traffic_sum(ttag, "k_integrate_long<", ", true>")
+        tf = measured_traffic_sum(ttag, "k_integrate_", "false>")
+        tm = measured_traffic_sum(ttag, "k_integrate_", "true>")
         if tf is not None and tm is not None:
             int_traffic = 0.5 * (tf + tm)
     roofline_int = {"bound": "hbm", "achieved": round(achieved_int, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved_int / HBM_PEAK_GBS, 4),
                     "traffic": int_traffic,
-                    "kernel": "integrator (k_integrate_long, all length bins of a step)",
+                    "kernel": "integrator (k_integrate_packed and k_integrate_long: every "
+                              "filament of a step)",
                     "kernel_avg_ms": int_ms, "algorithmic_bytes": bytes_int}
     r_end = dev.sites()[0]
     dev.close()

@@ -244,7 +244,8 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
  *                       length bin; 2 (mixed-precision default): two sites
  *                       per lane, every filament of up to 128 sites in a
  *                       lane segment of its own width, packed into shared
- *                       wavefronts (same results as 1)
+ *                       wavefronts, one launch (same results as 1); 3: the
+ *                       same, one launch per lane-group width (16/32/64)
  *   integrate_streams   0-3: the integrator's length bins are spread over the
  *                       system's stream and this many side streams (forked
  *                       and joined inside the step) and run concurrently
@@ -263,8 +264,9 @@ int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
 /* Autotuner (no counterpart in the reference): times the launch shapes
  * that leave every result unchanged -- first the pair kernel's (block size
  * or balanced variant, XCD order), then the integrator's (waves per block,
- * side streams; the layout is kept), then the overlap of the two over bond
- * ranges -- over `steps` steps each
+ * side streams, and on the packed layout one launch or one per lane-group
+ * width; the sites per lane are kept), then the overlap of the two over
+ * bond ranges -- over `steps` steps each
  * from a checkpoint of the current state, keeps the fastest and restores
  * the state, so the trajectory is the one without tuning.  The cell size
  * is not tuned: in the reference it sets the rebuild threshold and the

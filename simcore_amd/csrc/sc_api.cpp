@@ -47,7 +47,8 @@ struct sc_system {
   std::vector<Bin> bins;
   // packed integrator layout (mixed precision): lane table and per-wave
   // lane-move class of the filaments of at most 128 sites
-  int32_t *pack_tab = nullptr, *pack_code = nullptr;
+  int32_t *pack_tab = nullptr;
+  int pack_cs[4] = {0, 0, 0, 0};  // waves of lane-group width 16 << c: [pack_cs[c], pack_cs[c+1])
   int pack_waves = 0;
   int64_t pack_sites = 0;
   sc::Tune tune;
@@ -81,6 +82,7 @@ struct sc_system {
   // overlapped pair pass / integrator: one event per bond range
   hipEvent_t part_ev[sc::kMaxParts] = {};
   std::vector<int64_t> pack_last;  // per packed wave, its filaments' last bond
+                                   // (ascending inside each width group)
   // allocations (from `arena` when set, else hipMalloc)
   sc::Arena *arena = nullptr;
   bool own_stream = true;
@@ -375,7 +377,7 @@ void enqueue_step(sc_system *h, bool odd) {
       sc::launch_pair_range(d, g, h->p.precision, h->tune, st, g.thermo != 0, cut[p], cut[p + 1]);
       (void)hipEventRecord(h->part_ev[p], st);
     }
-    const bool packed = h->tune.int_layout == 2 && h->pack_waves > 0;
+    const bool packed = h->tune.int_layout >= 2 && h->pack_waves > 0;
     // first entry whose last bond is at or above bond c
     auto first_at = [](const std::vector<int64_t> &v, int64_t c) -> int {
       return (int)(std::lower_bound(v.begin(), v.end(), c) - v.begin());
@@ -387,13 +389,19 @@ void enqueue_step(sc_system *h, bool odd) {
       (void)hipStreamWaitEvent(si, h->part_ev[p], 0);
       (void)hipStreamWaitEvent(sl, h->part_ev[p], 0);
       const int64_t lo = cut[p], hi = p + 1 < P ? cut[p + 1] : nb + 1;
-      if (packed) {
-        const int w0 = p == 0 ? 0 : first_at(h->pack_last, lo);
-        const int w1 = p + 1 < P ? first_at(h->pack_last, hi) : h->pack_waves;
-        if (w1 > w0)
-          sc::launch_integrate_packed(d, g, h->pack_tab + (int64_t)64 * w0, h->pack_code + w0,
-                                      w1 - w0, h->tune.int_waves, odd, si);
-      }
+      if (packed)
+        for (int c = 0; c < 3; ++c) {
+          const int c0 = h->pack_cs[c], c1 = h->pack_cs[c + 1];
+          auto at = [&](int64_t v) -> int {
+            return (int)(std::lower_bound(h->pack_last.begin() + c0, h->pack_last.begin() + c1, v) -
+                         h->pack_last.begin());
+          };
+          const int w0 = p == 0 ? c0 : at(lo);
+          const int w1 = p + 1 < P ? at(hi) : c1;
+          if (w1 > w0)
+            sc::launch_integrate_packed(d, g, h->pack_tab + (int64_t)64 * w0, w1 - w0, 16 << c,
+                                        h->tune.int_waves, odd, si);
+        }
       for (const Bin &b : h->bins) {
         if (packed && b.maxn <= 128) continue;
         const int i0 = p == 0 ? 0 : first_at(b.last, lo);
@@ -425,16 +433,25 @@ void enqueue_step(sc_system *h, bool odd) {
   // bin of longer ones
   sc::BinDesc bins[sc::kMaxBins + 1];
   int nbins = 0;
-  const bool packed = h->tune.int_layout == 2 && h->pack_waves > 0;
-  if (packed) bins[nbins++] = {nullptr, h->pack_waves, 0};
+  const bool packed = h->tune.int_layout >= 2 && h->pack_waves > 0;
+  if (packed && h->tune.int_layout == 2)
+    bins[nbins++] = {h->pack_tab, h->pack_waves, 0, 0};  // one launch, all widths
+  else if (packed)
+    for (int c = 0; c < 3; ++c)
+      if (h->pack_cs[c + 1] > h->pack_cs[c])
+        bins[nbins++] = {h->pack_tab + (int64_t)64 * h->pack_cs[c],
+                         h->pack_cs[c + 1] - h->pack_cs[c], 0, 16 << c};
   for (const Bin &b : h->bins)
-    if (!packed || b.maxn > 128) bins[nbins++] = {b.list, b.count, b.maxn};
+    if (!packed || b.maxn > 128) bins[nbins++] = {b.list, b.count, b.maxn, 0};
   auto load_of = [&](int k) -> int64_t {
-    return bins[k].maxn ? (int64_t)bins[k].count * bins[k].maxn : h->pack_sites;
+    return bins[k].maxn ? (int64_t)bins[k].count * bins[k].maxn : (int64_t)bins[k].count * 120;
   };
   auto launch_bin = [&](int k, hipStream_t sk) {
-    if (bins[k].maxn == 0)
-      sc::launch_integrate_packed(d, g, h->pack_tab, h->pack_code, h->pack_waves,
+    if (bins[k].maxn == 0 && bins[k].width == 0)
+      sc::launch_integrate_packed_all(d, g, bins[k].list, bins[k].count, h->pack_cs[1],
+                                      h->pack_cs[2], h->tune.int_waves, odd, sk);
+    else if (bins[k].maxn == 0)
+      sc::launch_integrate_packed(d, g, bins[k].list, bins[k].count, bins[k].width,
                                   h->tune.int_waves, odd, sk);
     else
       sc::launch_integrate(d, g, bins[k].list, bins[k].count, bins[k].maxn, h->p.precision,
@@ -1137,13 +1154,15 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     h->bins.push_back(b);
   }
   // (host arrays of the packed layout live until the copies below have run)
-  std::vector<int32_t> tab, code;
+  std::vector<int32_t> tab;
   if (packing && !waves.empty()) {
     // caller-side ids -> storage indices
     std::vector<int32_t> st_of(nf);
     for (int f = 0; f < nf; ++f) st_of[h->reordered ? h->perm[f] : f] = f;
-    // waves in the order of their filaments' last bond, so that the waves
-    // whose bonds all lie below a cut form a prefix (overlapped steps)
+    // waves grouped by lane-group width (one launch each), inside a group
+    // in the order of their filaments' last bond, so that the waves whose
+    // bonds all lie below a cut form a prefix of the group (overlapped
+    // steps)
     {
       std::vector<int64_t> wl(waves.size(), -1);
       for (size_t wi = 0; wi < waves.size(); ++wi)
@@ -1154,21 +1173,23 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
           }
       std::vector<size_t> ord(waves.size());
       for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
-      std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wl[a] < wl[b]; });
+      std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+        return waves[a].code != waves[b].code ? waves[a].code < waves[b].code : wl[a] < wl[b];
+      });
       std::vector<PackWave> sorted_waves;
       sorted_waves.reserve(waves.size());
       h->pack_last.clear();
+      for (int c = 0; c < 4; ++c) h->pack_cs[c] = 0;
       for (size_t i : ord) {
         sorted_waves.push_back(waves[i]);
         h->pack_last.push_back(wl[i]);
+        for (int c = waves[i].code + 1; c < 4; ++c) h->pack_cs[c] += 1;
       }
       waves.swap(sorted_waves);
     }
     const size_t nw = waves.size();
     tab.assign(64 * nw, -1);
-    code.assign(nw, 0);
     for (size_t wi = 0; wi < nw; ++wi) {
-      code[wi] = waves[wi].code;
       for (int l = 0; l < 64; ++l) {
         const int32_t id = waves[wi].lane[l];
         tab[64 * wi + l] = id < 0 ? -1 : ((st_of[id] << 8) | (waves[wi].code << 6) | waves[wi].gl[l]);
@@ -1176,10 +1197,7 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
       }
     }
     if ((rc = dalloc(h, &h->pack_tab, tab.size(), false))) return rc;
-    if ((rc = dalloc(h, &h->pack_code, nw, false))) return rc;
     HIPCHK(h, hipMemcpyAsync(h->pack_tab, tab.data(), sizeof(int32_t) * tab.size(),
-                             hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->pack_code, code.data(), sizeof(int32_t) * nw,
                              hipMemcpyHostToDevice, h->stream));
     h->pack_waves = (int)nw;
     h->tune.int_layout = 2;
@@ -1985,24 +2003,33 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
       t.pair_xcd = x;
       consider(t);
     }
-  // integrator: waves per block and, with several length bins, bins on
-  // side streams (the layout, sites per lane, changes the rounding of the
-  // mixed path and is not tuned)
+  // integrator: waves per block and, with several launches, launches on
+  // side streams; on the packed layout also one launch for all lane-group
+  // widths or one per width (the same arithmetic).  Sites per lane (layouts
+  // 0 and 1) change the rounding of the mixed path and are not tuned.
   const sc::Tune pair_best = best;
   const int waves[] = {1, 2, 4};
-  // side streams only matter with several integrator launches (0..3 of them)
-  int launches = 0;
-  for (const Bin &b : h->bins)
-    if (!(h->tune.int_layout == 2 && h->pack_waves > 0) || b.maxn > 128) ++launches;
-  if (h->tune.int_layout == 2 && h->pack_waves > 0) ++launches;
-  const int n_str = std::min(launches, 1 + sc_system::kSide);
-  for (int w : waves)
-    for (int sm = 0; sm < n_str; ++sm) {
-      sc::Tune t = pair_best;
-      t.int_waves = w;
-      t.int_streams = sm;
-      consider(t);
-    }
+  const bool packed = h->tune.int_layout >= 2 && h->pack_waves > 0;
+  std::vector<int> layouts = {h->tune.int_layout};
+  if (packed) layouts = {2, 3};
+  for (int lay : layouts) {
+    // side streams only matter with several integrator launches (0..3 of them)
+    int launches = 0;
+    for (const Bin &b : h->bins)
+      if (!packed || b.maxn > 128) ++launches;
+    if (packed && lay == 2) ++launches;
+    if (packed && lay == 3)
+      for (int c = 0; c < 3; ++c) launches += h->pack_cs[c + 1] > h->pack_cs[c] ? 1 : 0;
+    const int n_str = std::min(launches, 1 + sc_system::kSide);
+    for (int w : waves)
+      for (int sm = 0; sm < n_str; ++sm) {
+        sc::Tune t = pair_best;
+        t.int_waves = w;
+        t.int_streams = sm;
+        t.int_layout = lay;
+        consider(t);
+      }
+  }
   // overlap of the pair pass and the integrator over bond ranges (systems
   // without crosslinks); the ranges take the side stream, so the
   // integrator's own side-stream spreading is off there
@@ -2045,7 +2072,7 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
        v.pair_block != 512) ||
       (v.pair_xcd != 0 && v.pair_xcd != 1) ||
       (v.integrate_waves != 1 && v.integrate_waves != 2 && v.integrate_waves != 4) ||
-      (v.integrate_layout < 0 || v.integrate_layout > 2) ||
+      (v.integrate_layout < 0 || v.integrate_layout > 3) ||
       (v.integrate_streams < 0 || v.integrate_streams > 3) ||
       (v.pair_parts < 0 || v.pair_parts > sc::kMaxParts))
     return fail(h, SC_ERR_ARG, "unsupported launch shape");

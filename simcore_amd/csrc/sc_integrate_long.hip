@@ -722,15 +722,13 @@ void k_integrate_long(Dev d, Geo g, const int32_t *list,
 // their own width ceil(n/2) (lane table entry: filament << 8 | W code << 6
 // | lane in segment, -1 for an idle lane), a segment of at most 16 lanes
 // inside one 16-lane row, of 17-32 lanes inside one half, of 33-64 lanes
-// from lane 0; the wave's W code (0: 16, 1: 32, 2: 64) selects the lane
-// moves.  Same arithmetic as the binned layout, far fewer idle lanes.
-// Three waves per SIMD: the three lane-group variants in one kernel need
-// about 140 VGPRs, and the spills of a 128-register budget cost more than
-// the fourth wave gains (measured on the uneven-length system).
-template <bool MID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-void k_integrate_packed(Dev d, Geo g, const int32_t *lane_tab, const int32_t *wave_code,
-                        int n_waves) {
+// from lane 0.  The waves are grouped by the lane-group width W of their
+// widest segment (16, 32 or 64); one launch per width carries only its own
+// lane moves and register budget.  Same arithmetic as the binned layout,
+// far fewer idle lanes.
+template <int W, bool MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 64 ? 3 : 4)))
+void k_integrate_packed(Dev d, Geo g, const int32_t *lane_tab, int n_waves) {
   Ctl *ctl = d.ctl;
   if (ctl->abort) return;
   if ((ctl->midstep != 0) != MID) {
@@ -749,26 +747,77 @@ void k_integrate_packed(Dev d, Geo g, const int32_t *lane_tab, const int32_t *wa
   const int gl = factive ? (e & 63) : 0;
   int Gf = 1;
   if (factive) Gf = (d.site_off[f + 1] - d.site_off[f] + 1) >> 1;
-  const int code = __builtin_amdgcn_readfirstlane(wave_code[wave]);
-  if (code == 0)
+  long_body<W, 2, MID>(d, g, factive, f, gl, Gf, wave);
+}
+
+// All packed waves in one launch: the wave's lane-group width follows from
+// its index (the waves are grouped by width, [0, cs1) 16, [cs1, cs2) 32,
+// the rest 64) and selects the variant, uniformly per wave.  One register
+// budget for the three variants (about 140 VGPRs, three waves per SIMD;
+// the spills of a 128-register budget cost more than the fourth wave
+// gains); fewer launches than one per width.
+template <bool MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_integrate_packed_any(Dev d, Geo g, const int32_t *lane_tab, int n_waves, int cs1,
+                            int cs2) {
+  Ctl *ctl = d.ctl;
+  if (ctl->abort) return;
+  if ((ctl->midstep != 0) != MID) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl->err = SC_ERR_STATE;
+      ctl->abort = 1;
+    }
+    return;
+  }
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave >= n_waves) return;
+  const int lane = threadIdx.x & 63;
+  const int32_t e = lane_tab[(int64_t)wave * 64 + lane];
+  const bool factive = e >= 0;
+  const int f = factive ? (e >> 8) : 0;
+  const int gl = factive ? (e & 63) : 0;
+  int Gf = 1;
+  if (factive) Gf = (d.site_off[f + 1] - d.site_off[f] + 1) >> 1;
+  if (wave < cs1)
     long_body<16, 2, MID>(d, g, factive, f, gl, Gf, wave);
-  else if (code == 1)
+  else if (wave < cs2)
     long_body<32, 2, MID>(d, g, factive, f, gl, Gf, wave);
   else
     long_body<64, 2, MID>(d, g, factive, f, gl, Gf, wave);
 }
 
-void launch_integrate_packed(const Dev &d, const Geo &g, const int32_t *lane_tab,
-                             const int32_t *wave_code, int n_waves, int waves, bool mid,
-                             hipStream_t st) {
+void launch_integrate_packed_all(const Dev &d, const Geo &g, const int32_t *lane_tab,
+                                 int n_waves, int cs1, int cs2, int waves, bool mid,
+                                 hipStream_t st) {
   if (n_waves <= 0) return;
   dim3 grid((n_waves + waves - 1) / waves), block(64 * waves);
   if (mid)
-    hipLaunchKernelGGL((k_integrate_packed<true>), grid, block, 0, st, d, g, lane_tab, wave_code,
-                       n_waves);
+    hipLaunchKernelGGL((k_integrate_packed_any<true>), grid, block, 0, st, d, g, lane_tab,
+                       n_waves, cs1, cs2);
   else
-    hipLaunchKernelGGL((k_integrate_packed<false>), grid, block, 0, st, d, g, lane_tab,
-                       wave_code, n_waves);
+    hipLaunchKernelGGL((k_integrate_packed_any<false>), grid, block, 0, st, d, g, lane_tab,
+                       n_waves, cs1, cs2);
+}
+
+void launch_integrate_packed(const Dev &d, const Geo &g, const int32_t *lane_tab, int n_waves,
+                             int width, int waves, bool mid, hipStream_t st) {
+  if (n_waves <= 0) return;
+  dim3 grid((n_waves + waves - 1) / waves), block(64 * waves);
+#define SC_PACKED(WW)                                                                          \
+  if (mid)                                                                                     \
+    hipLaunchKernelGGL((k_integrate_packed<WW, true>), grid, block, 0, st, d, g, lane_tab,    \
+                       n_waves);                                                               \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_integrate_packed<WW, false>), grid, block, 0, st, d, g, lane_tab,   \
+                       n_waves)
+  if (width == 16) {
+    SC_PACKED(16);
+  } else if (width == 32) {
+    SC_PACKED(32);
+  } else {
+    SC_PACKED(64);
+  }
+#undef SC_PACKED
 }
 
 bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list, int bin_count,

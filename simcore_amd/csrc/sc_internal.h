@@ -150,7 +150,9 @@ struct Tune {
   int pair_bal = 0;      // 1: incidences dealt evenly over a wave's lanes
   int int_waves = 1;     // waves per block of the register integrator
   int int_layout = 1;    // up to 64 sites: 0 one site per lane, 1 two (the
-                         // two round differently; not changed by tuning)
+                         // two round differently; not changed by tuning);
+                         // mixed path: 2 packed lane segments, one launch,
+                         // 3 the same, one launch per lane-group width
   int int_streams = 0;   // length bins spread over this many side streams (0-3)
   int parts = 1;         // >1: the bonds are cut into this many ranges; the
                          // integrator of a range's filaments runs on a side
@@ -162,9 +164,10 @@ constexpr int64_t kPairAlign = 512;  // range cuts: a multiple of every block si
 // Length bins of the integrator, as the step loop hands them over.
 constexpr int kMaxBins = 6;
 struct BinDesc {
-  const int32_t *list;  // filament indices of the bin
-  int count;
-  int maxn;  // 8, 16, 32, 64, 128 or 256 sites
+  const int32_t *list;  // filament indices of the bin (packed: lane table)
+  int count;            // filaments (packed: wavefronts)
+  int maxn;             // 8, 16, 32, 64, 128 or 256 sites; 0: packed waves
+  int width;            // packed: lane-group width of the waves (16/32/64)
 };
 
 
@@ -219,9 +222,11 @@ bool launch_integrate_long(const Dev &d, const Geo &g, const int32_t *bin_list,
                            int layout, bool mid, hipStream_t st);
 // packed layout for the mixed path: filaments of up to 128 sites as lane
 // segments of their own width (sc_integrate_long.hip)
-void launch_integrate_packed(const Dev &d, const Geo &g, const int32_t *lane_tab,
-                             const int32_t *wave_code, int n_waves, int waves, bool mid,
-                             hipStream_t st);
+void launch_integrate_packed(const Dev &d, const Geo &g, const int32_t *lane_tab, int n_waves,
+                             int width, int waves, bool mid, hipStream_t st);
+void launch_integrate_packed_all(const Dev &d, const Geo &g, const int32_t *lane_tab,
+                                 int n_waves, int cs1, int cs2, int waves, bool mid,
+                                 hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
 // crosslinks (sc_crosslink.hip)

@@ -377,10 +377,11 @@ def test_autotune_leaves_trajectory_unchanged(precision):
     rep = b.autotune(steps=6)
     print(rep)
     # 8 pair shapes, then 3 waves per block x side streams, then 3 counts of
-    # overlapped bond ranges; the mixed path's packed integrator is one
-    # launch here (no filament above 128 sites), so side streams are not
-    # tried
-    expect = 23 if precision == "fp64" else 14
+    # overlapped bond ranges; the mixed path's packed integrator is tried as
+    # one launch (no side stream: no filament above 128 sites) and as one
+    # launch per lane-group width (two here, 16- and 32-lane groups: one
+    # side stream)
+    expect = 23 if precision == "fp64" else 8 + 3 * 1 + 3 * 2 + 3
     assert rep["n_trials"] == expect and rep["step_ms_best"] <= rep["step_ms_start"]
     assert b.counters()["i_step"] == 31
     a.step(40)
@@ -389,7 +390,7 @@ def test_autotune_leaves_trajectory_unchanged(precision):
         np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2])
+@pytest.mark.parametrize("layout", [0, 1, 2, 3])
 @pytest.mark.parametrize("kind", ["short", "long"])
 def test_integrator_layouts_within_tolerance(kind, layout):
     """Both integrator layouts for filaments of up to 64 sites (one site per
@@ -421,21 +422,22 @@ def test_integrator_layouts_within_tolerance(kind, layout):
 def test_packed_integrator_matches_length_bins(kind, spatial):
     """The packed layout (every filament of up to 128 sites in a lane segment
     of its own width, several lengths per wavefront, lane moves chosen per
-    wavefront) runs the length-binned layout's arithmetic: sites, tensions
-    and counters bit-identical over 41 steps, with and without side streams."""
+    wavefront; one launch, or one per lane-group width) runs the
+    length-binned layout's arithmetic: sites, tensions and counters
+    bit-identical over 41 steps, with and without side streams."""
     print(f"case: {kind} mixed spatial={spatial}, packed vs binned integrator, 41 steps",
           flush=True)
     from tests.helpers import long_system, uneven_system
     p, fil = {"c0": c0_system, "short": uneven_system, "long": long_system}[kind]()
     runs = []
-    for layout, streams in ((1, 0), (2, 0), (2, 1)):
+    for layout, streams in ((1, 0), (2, 0), (2, 1), (3, 0), (3, 1)):
         s = sc.FilamentSystem(p, fil, precision="mixed", use_graph=True, spatial_order=spatial)
         s.set_launch_shape(pair_block=0, integrate_waves=2, integrate_layout=layout,
                            integrate_streams=streams)
         s.step(41)
         runs.append((s.sites(), s.tensions(), s.counters()))
         s.close()
-    for j in (1, 2):
+    for j in range(1, len(runs)):
         for x, y in zip(runs[0][0], runs[j][0]):
             np.testing.assert_array_equal(x, y)
         np.testing.assert_array_equal(runs[0][1], runs[j][1])
