@@ -363,6 +363,7 @@ def main():
     pair_warm_ms = dev.time_pair(50)
     log(f"[bench] profile: {json.dumps(prof)} nc={nc} pair back-to-back {pair_warm_ms:.5f} ms")
     balanced = int(dev.launch_shape()["pair_block"]) == 0
+    tune_layout = int(dev.launch_shape()["integrate_layout"])
     kname = ("k_pair_bal" if balanced else "k_pair_fused") + (
         "<true, false>" if args.precision == "mixed" else "<false, false>")
     ttag = f"{TRAFFIC_TAG[args.workload]}_{args.precision}" if args.workload in TRAFFIC_TAG else None
@@ -385,8 +386,15 @@ def main():
     achieved_int = bytes_int / (int_ms * 1e-3) / 1e9 if int_ms > 0 else 0.0
     int_traffic = None
     if ttag and args.precision == "mixed":
-        tf = measured_traffic_sum(ttag, "k_integrate_", "false>")
-        tm = measured_traffic_sum(ttag, "k_integrate_", "true>")
+        # the kernels this launch shape runs: the packed filaments in one
+        # launch (layout 2) or one per lane-group width (3), the longer ones
+        # in their length bin
+        packed_part = ("k_integrate_packed_any<" if tune_layout == 2 else "k_integrate_packed<")
+        tf = measured_traffic_sum(ttag, packed_part, "false>")
+        tm = measured_traffic_sum(ttag, packed_part, "true>")
+        if tf is not None and tm is not None:
+            tf += measured_traffic_sum(ttag, "k_integrate_long<", "false>") or 0.0
+            tm += measured_traffic_sum(ttag, "k_integrate_long<", "true>") or 0.0
         if tf is not None and tm is not None:
             int_traffic = 0.5 * (tf + tm)
     roofline_int = {"bound": "hbm", "achieved": round(achieved_int, 2), "peak": HBM_PEAK_GBS,
