@@ -89,7 +89,91 @@ def serving_requests(sc, n_requests, rate, seed=20240004):
     return out
 
 
+def serve_trace(args, sc, reqs):
+    """Replays one arrival trace through a fresh ensemble runner; returns its
+    stats."""
+    ens = sc.Ensemble(hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
+                      max_active=args.max_active, steps_per_launch=args.steps_per_launch)
+    t0 = ens.clock()
+    k = 0
+    last_log = time.perf_counter()
+    while True:
+        now = ens.clock() - t0
+        while k < len(reqs) and reqs[k][0] <= now:
+            ens.submit(reqs[k][1], reqs[k][2], args.request_steps, precision=args.precision,
+                       t_submit=t0 + reqs[k][0],
+                       spatial_order=args.precision == "mixed" and not args.caller_order)
+            k += 1
+        nr, nq = ens.progress()
+        if k == len(reqs) and nr == 0 and nq == 0:
+            break
+        if time.perf_counter() - last_log > 5.0:
+            last_log = time.perf_counter()
+            log(f"[bench] t={now:.1f}s submitted {k} running {nr} queued {nq}")
+        time.sleep(50e-6)
+    st = ens.stats()
+    ens.close()
+    return st
+
+
+def run_sweep(args, sc):
+    """Saturation throughput: the same request mix replayed at doubling
+    arrival rates from --rate, --requests requests per rate; stops at the
+    first rate with a failed request, or whose p99 latency exceeds 10x the
+    lowest rate's, or whose throughput gained less than 5% over the
+    previous rate (the runner no longer keeps up).  Prints one JSON line
+    with every rate's throughput and latencies and the largest throughput
+    reached."""
+    base = serving_requests(sc, args.requests, 1.0)
+    nbonds = np.array([f.nb for _, _, f in base])
+    warm = sc.Ensemble(hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
+                       max_active=args.max_active, steps_per_launch=args.steps_per_launch)
+    warm.submit(base[0][1], base[0][2], 20, precision=args.precision)
+    warm.drain(120)
+    warm.close()
+    points, rate, stop = [], args.rate, ""
+    while not stop and rate <= 65536:
+        # the same systems; arrival times scaled to the rate
+        reqs = [(t / rate, p, f) for t, p, f in base]
+        st = serve_trace(args, sc, reqs)
+        value = st["segment_steps"] / st["span"] if st["span"] > 0 else 0.0
+        pt = {"rate_per_s": rate, "value": value, "latency_p50_s": st["latency_p50"],
+              "latency_p99_s": st["latency_p99"], "completed": st["completed"],
+              "failed": st["failed"], "peak_active": st["peak_active"],
+              "offered": float(nbonds.mean()) * args.request_steps * rate}
+        points.append(pt)
+        log(f"[bench] sweep {json.dumps(pt)}")
+        if st["failed"]:
+            stop = "failed request"
+        elif st["latency_p99"] > 10 * points[0]["latency_p99_s"]:
+            stop = "p99 latency above 10x the lowest rate's"
+        elif len(points) > 1 and value < 1.05 * points[-2]["value"]:
+            stop = "throughput gained less than 5%"
+        rate *= 2
+    best = max(points, key=lambda q: q["value"])
+    out = {
+        "metric": "segment-steps/s", "value": best["value"], "unit": "segment-steps/s",
+        "n_gpus": 1, "steps": args.request_steps, "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": "ensemble serving, rate sweep: %d systems of 1k-100k bonds "
+                               "(log-uniform, 32-site filaments, volume fraction 0.05), %d "
+                               "steps each, lp U[10,1000] sigma, dt 1e-5, Poisson arrivals at "
+                               "doubling rates from %g/s" % (args.requests, args.request_steps,
+                                                             args.rate),
+                   "max_active": args.max_active, "steps_per_launch": args.steps_per_launch,
+                   "hbm_budget_gb": args.hbm_budget_gb, "precision": args.precision},
+        "saturation": {"value": best["value"], "rate_per_s": best["rate_per_s"],
+                       "stopped_at_rate_per_s": points[-1]["rate_per_s"], "reason": stop},
+        "sweep": points,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_serving(args, sc):
+    if args.sweep:
+        run_sweep(args, sc)
+        return
     log(f"[bench] generating {args.requests} requests")
     reqs = serving_requests(sc, args.requests, args.rate)
     nbonds = np.array([f.nb for _, _, f in reqs])
@@ -295,6 +379,8 @@ def main():
     # serving workload (--workload c4)
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--sweep", action="store_true",
+                    help="c4: arrival rates doubling from --rate until saturation")
     ap.add_argument("--request-steps", type=int, default=2000)
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--steps-per-launch", type=int, default=64)
@@ -336,7 +422,7 @@ def main():
     tune = None
     if args.launch_shape:
         keys = ["pair_block", "pair_xcd", "integrate_waves", "integrate_layout",
-                "integrate_streams", "pair_parts"]
+                "integrate_streams", "pair_parts", "early_long"]
         shape = dict(zip(keys, (int(v) for v in args.launch_shape.split(","))))
         dev.set_launch_shape(**shape)
         tune = dict(dev.launch_shape(), set="fixed")

@@ -253,10 +253,15 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
  *                       bonds are cut into this many ranges, and the
  *                       integrator of the filaments below a cut runs on a
  *                       side stream beside the pair pass of the next range
- *                       (systems without crosslinks; results unchanged) */
+ *                       (systems without crosslinks; results unchanged)
+ *   early_long          1: the pair pass is cut after the bonds of the
+ *                       filaments above 128 sites (stored first in spatial
+ *                       order), and their integrator runs on a side stream
+ *                       beside the rest of the pass (systems without
+ *                       crosslinks; results unchanged) */
 typedef struct sc_launch_shape {
   int32_t pair_block, pair_xcd, integrate_waves, integrate_layout, integrate_streams;
-  int32_t pair_parts;
+  int32_t pair_parts, early_long;
 } sc_launch_shape;
 int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape);
 int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
@@ -266,7 +271,8 @@ int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
  * or balanced variant, XCD order), then the integrator's (waves per block,
  * side streams, and on the packed layout one launch or one per lane-group
  * width; the sites per lane are kept), then the overlap of the two over
- * bond ranges -- over `steps` steps each
+ * bond ranges, and the early integration of the long filaments -- over
+ * `steps` steps each
  * from a checkpoint of the current state, keeps the fastest and restores
  * the state, so the trajectory is the one without tuning.  The cell size
  * is not tuned: in the reference it sets the rebuild threshold and the

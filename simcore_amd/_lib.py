@@ -100,7 +100,8 @@ class ScCrosslinkParams(ctypes.Structure):
 class ScLaunchShape(ctypes.Structure):
     _fields_ = [("pair_block", c_int32), ("pair_xcd", c_int32),
                 ("integrate_waves", c_int32), ("integrate_layout", c_int32),
-                ("integrate_streams", c_int32), ("pair_parts", c_int32)]
+                ("integrate_streams", c_int32), ("pair_parts", c_int32),
+                ("early_long", c_int32)]
 
 
 class ScTuneReport(ctypes.Structure):

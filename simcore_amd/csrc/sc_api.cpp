@@ -81,6 +81,7 @@ struct sc_system {
   hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
   // overlapped pair pass / integrator: one event per bond range
   hipEvent_t part_ev[sc::kMaxParts] = {};
+  int64_t long_last = -1;  // last bond of the filaments above 128 sites (-1: none)
   std::vector<int64_t> pack_last;  // per packed wave, its filaments' last bond
                                    // (ascending inside each width group)
   // allocations (from `arena` when set, else hipMalloc)
@@ -365,6 +366,43 @@ void enqueue_step(sc_system *h, bool odd) {
   // are done, beside the pair pass of the next range.  Each step reads one
   // record buffer and writes the other, so the two never touch the same
   // memory; the results are those of the sequential order.
+  // Early integration of the long filaments (no crosslinks, not
+  // profiling): with the filaments above 128 sites stored first, the pair
+  // pass is cut after their last bond; their integrator -- few wavefronts,
+  // bound by latency -- then runs on a side stream beside the rest of the
+  // pair pass, and the packed filaments follow the pass on the system's
+  // stream.  Same memory argument as below; the results are those of the
+  // sequential order.
+  if (h->tune.early_long && g.nx == 0 && h->side[1] && !pf && h->long_last >= 0) {
+    const int64_t nb = g.nb;
+    const int64_t cut = std::min<int64_t>(nb, (h->long_last / sc::kPairAlign + 1) * sc::kPairAlign);
+    hipStream_t sl = h->side[1];
+    sc::launch_pair_range(d, g, h->p.precision, h->tune, st, g.thermo != 0, 0, cut);
+    (void)hipEventRecord(h->part_ev[0], st);
+    (void)hipStreamWaitEvent(sl, h->part_ev[0], 0);
+    const bool packed = h->tune.int_layout >= 2 && h->pack_waves > 0;
+    for (const Bin &b : h->bins)
+      if (b.maxn > 128)
+        sc::launch_integrate(d, g, b.list, b.count, b.maxn, h->p.precision, h->tune, odd, sl);
+    (void)hipEventRecord(h->join_ev[1], sl);
+    sc::launch_pair_range(d, g, h->p.precision, h->tune, st, g.thermo != 0, cut, nb);
+    if (packed && h->tune.int_layout == 2) {
+      sc::launch_integrate_packed_all(d, g, h->pack_tab, h->pack_waves, h->pack_cs[1],
+                                      h->pack_cs[2], h->tune.int_waves, odd, st);
+    } else if (packed) {
+      for (int c = 0; c < 3; ++c)
+        if (h->pack_cs[c + 1] > h->pack_cs[c])
+          sc::launch_integrate_packed(d, g, h->pack_tab + (int64_t)64 * h->pack_cs[c],
+                                      h->pack_cs[c + 1] - h->pack_cs[c], 16 << c,
+                                      h->tune.int_waves, odd, st);
+    }
+    for (const Bin &b : h->bins)
+      if (b.maxn <= 128 && !packed)
+        sc::launch_integrate(d, g, b.list, b.count, b.maxn, h->p.precision, h->tune, odd, st);
+    (void)hipStreamWaitEvent(st, h->join_ev[1], 0);
+    sc::launch_commit(d, g, st);
+    return;
+  }
   const int P = std::min(h->tune.parts, sc::kMaxParts);
   if (P > 1 && g.nx == 0 && h->side[0] && !pf) {
     const int64_t nb = g.nb;
@@ -1148,6 +1186,7 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
     b.maxn = bin_max[k];
     b.count = (int)lists[k].size();
     for (int32_t f : lists[k]) b.last.push_back((int64_t)site_off[f + 1] - f - 2);
+    if (b.maxn > 128) h->long_last = std::max(h->long_last, b.last.back());
     if ((rc = dalloc(h, &b.list, lists[k].size(), false))) return rc;
     HIPCHK(h, hipMemcpyAsync(b.list, lists[k].data(), sizeof(int32_t) * lists[k].size(),
                              hipMemcpyHostToDevice, h->stream));
@@ -2043,6 +2082,13 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
       consider(t);
     }
   }
+  // early integration of the long filaments beside the pair pass
+  if (h->g.nx == 0 && h->side[1] && h->long_last >= 0) {
+    sc::Tune t = best;
+    t.early_long = 1;
+    t.parts = 1;
+    consider(t);
+  }
   // leave the system where it was, with the chosen shapes
   h->tune = rc ? start : best;
   drop_graph(h);
@@ -2074,7 +2120,8 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
       (v.integrate_waves != 1 && v.integrate_waves != 2 && v.integrate_waves != 4) ||
       (v.integrate_layout < 0 || v.integrate_layout > 3) ||
       (v.integrate_streams < 0 || v.integrate_streams > 3) ||
-      (v.pair_parts < 0 || v.pair_parts > sc::kMaxParts))
+      (v.pair_parts < 0 || v.pair_parts > sc::kMaxParts) ||
+      (v.early_long != 0 && v.early_long != 1))
     return fail(h, SC_ERR_ARG, "unsupported launch shape");
   if (h->pending) return fail(h, SC_ERR_STATE, "steps in flight: poll them first");
   h->tune.pair_bal = v.pair_block == 0 ? 1 : 0;
@@ -2084,8 +2131,9 @@ int sc_set_launch_shape(sc_handle h, const sc_launch_shape *shape) {
   h->tune.int_layout = v.integrate_layout;
   h->tune.int_streams = v.integrate_streams;
   h->tune.parts = v.pair_parts > 1 ? v.pair_parts : 1;
+  h->tune.early_long = v.early_long;
   drop_graph(h);
-  if (v.integrate_streams > 0 || v.pair_parts > 1) return ensure_side_streams(h);
+  if (v.integrate_streams > 0 || v.pair_parts > 1 || v.early_long) return ensure_side_streams(h);
   return SC_OK;
 }
 
@@ -2100,6 +2148,7 @@ int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape) {
   shape->integrate_layout = t.int_layout;
   shape->integrate_streams = t.int_streams;
   shape->pair_parts = t.parts;
+  shape->early_long = t.early_long;
   return SC_OK;
 }
 

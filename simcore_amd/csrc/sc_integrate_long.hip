@@ -75,10 +75,15 @@ __device__ __forceinline__ double seg_from(double v, int lane, int gl, int Gf, i
 
 // Parallel cyclic reduction over the segment's lanes (row gl in lane gl;
 // rows >= m inert).  Levels beyond the segment's width leave every row
-// unchanged (both couplings are zero by then), so W bounds the loop.
+// unchanged (both couplings are zero by then), so W bounds the loop.  The
+// values moved in from lanes outside the segment are not masked: they only
+// enter multiplied by a coupling that is zero there (lo / hi), and every
+// lane's rows are finite (other segments' rows, or inert ones), so they
+// add zero.
 template <int W>
 __device__ __forceinline__ double pcr_seg(double a, double b, double c, double d, int m, int lane,
                                           int gl, int Gf) {
+  (void)Gf;
   if (gl >= m) {
     a = 0.0;
     b = 1.0;
@@ -88,14 +93,14 @@ __device__ __forceinline__ double pcr_seg(double a, double b, double c, double d
 #pragma unroll
   for (int s = 1; s < W; s <<= 1) {
     const double rb = rcp_fast(b);
-    const double am = seg_from<W>(a, lane, gl, Gf, -s, 0.0);
-    const double cm = seg_from<W>(c, lane, gl, Gf, -s, 0.0);
-    const double dm = seg_from<W>(d, lane, gl, Gf, -s, 0.0);
-    const double rm = seg_from<W>(rb, lane, gl, Gf, -s, 0.0);
-    const double ap = seg_from<W>(a, lane, gl, Gf, s, 0.0);
-    const double cp = seg_from<W>(c, lane, gl, Gf, s, 0.0);
-    const double dp = seg_from<W>(d, lane, gl, Gf, s, 0.0);
-    const double rp = seg_from<W>(rb, lane, gl, Gf, s, 0.0);
+    const double am = lane_shift_c<W>(a, lane, -s);
+    const double cm = lane_shift_c<W>(c, lane, -s);
+    const double dm = lane_shift_c<W>(d, lane, -s);
+    const double rm = lane_shift_c<W>(rb, lane, -s);
+    const double ap = lane_shift_c<W>(a, lane, s);
+    const double cp = lane_shift_c<W>(c, lane, s);
+    const double dp = lane_shift_c<W>(d, lane, s);
+    const double rp = lane_shift_c<W>(rb, lane, s);
     const bool lo = gl - s >= 0, hi = gl + s < m;
     const double al = lo ? -a * rm : 0.0;
     const double ga = hi ? -c * rp : 0.0;

@@ -157,6 +157,9 @@ struct Tune {
   int parts = 1;         // >1: the bonds are cut into this many ranges; the
                          // integrator of a range's filaments runs on a side
                          // stream beside the pair pass of the next range
+  int early_long = 0;    // 1: the pair pass is cut after the bonds of the
+                         // filaments above 128 sites (stored first), whose
+                         // integrator then runs beside the rest of the pass
 };
 constexpr int kMaxParts = 8;
 constexpr int64_t kPairAlign = 512;  // range cuts: a multiple of every block size

@@ -267,11 +267,11 @@ class FilamentSystem:
         return b.reshape(-1, 2), la.reshape(-1, 2), f.reshape(-1, 9), ev.value
 
     def set_launch_shape(self, pair_block=256, pair_xcd=0, integrate_waves=1,
-                         integrate_layout=1, integrate_streams=0, pair_parts=1):
+                         integrate_layout=1, integrate_streams=0, pair_parts=1, early_long=0):
         """Launch shapes (sc_set_launch_shape); only integrate_layout changes
         results (the rounding of the mixed integrator)."""
         v = ScLaunchShape(pair_block, pair_xcd, integrate_waves, integrate_layout,
-                          integrate_streams, pair_parts)
+                          integrate_streams, pair_parts, early_long)
         _check(lib.sc_set_launch_shape(self.h, byref(v)), self.h, "sc_set_launch_shape")
 
     def time_pair(self, launches=50):

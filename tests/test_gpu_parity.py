@@ -493,10 +493,11 @@ def test_length_bins_on_side_streams_are_bit_identical(precision):
 @pytest.mark.parametrize("kind", ["uneven2000", "long"])
 def test_overlapped_ranges_are_bit_identical(precision, kind):
     """The pair pass cut into bond ranges with the integrator of finished
-    ranges on a side stream (graph and eager; 2, 4 and 8 ranges) gives the
-    results of the sequential order: sites, tensions, counters and bond
-    forces identical over 41 steps with rebuilds; FP64 equal to the CPU
-    path."""
+    ranges on a side stream (graph and eager; 2, 4 and 8 ranges), and cut
+    after the long filaments with their integrator beside the rest of the
+    pass, gives the results of the sequential order: sites, tensions,
+    counters and bond forces identical over 41 steps with rebuilds; FP64
+    equal to the CPU path."""
     from simcore_amd import workloads as W
     from tests.helpers import long_system
     if kind == "uneven2000":
@@ -505,12 +506,15 @@ def test_overlapped_ranges_are_bit_identical(precision, kind):
         p, fil = long_system()
     print(f"case: {kind} {precision}, overlapped pair/integrator ranges, 41 steps", flush=True)
     runs = []
-    for parts, graph in ((1, True), (2, True), (4, True), (8, True), (4, False)):
+    for parts, graph, early in ((1, True, 0), (2, True, 0), (4, True, 0), (8, True, 0),
+                                (4, False, 0), (1, True, 1), (1, False, 1)):
         s = sc.FilamentSystem(p, fil, precision=precision, use_graph=graph,
                               spatial_order=precision == "mixed")
         s.set_launch_shape(pair_block=0, pair_xcd=1, integrate_waves=2,
-                           integrate_layout=2 if precision == "mixed" else 1, pair_parts=parts)
+                           integrate_layout=2 if precision == "mixed" else 1, pair_parts=parts,
+                           early_long=early)
         assert s.launch_shape()["pair_parts"] == parts
+        assert s.launch_shape()["early_long"] == early
         s.step(41)
         runs.append((s.sites(), s.tensions(), s.counters(), s.bond_forces()))
         s.close()
