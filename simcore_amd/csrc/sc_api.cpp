@@ -903,7 +903,7 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   const int64_t rec_bytes = params->precision == SC_PREC_MIXED ? 32 : 64;
   b += R(rec_bytes * nb) + 2 * R(24 * nb) + R(8 * nb);    // record, F, T (at most), tension
   const int64_t ncol = n1d * n1d, occ_words = (n1d + 31) / 32;
-  b += R(4 * nb) + R(4 * ncol) + R(4 * (ncol + 1)) + R(4 * nb);  // keys, columns, col_tmp
+  b += R(4 * nb) + R(4 * ncol) + R(4 * (ncol + 1)) + 2 * R(4 * nb);  // keys, columns, col_tmp/rank
   b += R(8 * n_cells) + R(4 * ncol * occ_words);                 // cell runs, occupancy
   b += 2 * R(4 * nb) + R(4 * nb) + R(4 * (nb + 1));  // sorted, skey, deg, inc_start
   b += R(4 * nb);                                      // neighbour masks
@@ -1108,6 +1108,7 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = dalloc(h, &d.col_count, ncol, false))) return rc;
   if ((rc = dalloc(h, &d.col_start, ncol + 1, false))) return rc;
   if ((rc = dalloc(h, &d.col_tmp, nb, false))) return rc;
+  if ((rc = dalloc(h, &d.col_rank, nb, false))) return rc;
   if ((rc = dalloc(h, &d.cell_run, g.n_cells, false))) return rc;
   if ((rc = dalloc(h, &d.sorted, nb, false))) return rc;
   if ((rc = dalloc(h, &d.skey, nb, false))) return rc;

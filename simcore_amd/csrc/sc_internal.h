@@ -110,6 +110,7 @@ struct Dev {
   int32_t *keys;           // nb: cell key of each bond
   int32_t *col_count;      // n1d^2: column populations (zero between builds)
   int32_t *col_start;      // n1d^2 + 1: column starts in sorted
+  int32_t *col_rank;       // nb: a bond's place in its column (build scratch)
   int32_t *col_tmp;        // nb: bonds grouped by column, unordered
   int32_t *sorted, *skey;  // nb: bonds in (key, bond) order, their keys
   int2 *cell_run;          // n_cells: [start, end) of an occupied cell

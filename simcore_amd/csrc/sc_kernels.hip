@@ -247,8 +247,9 @@ void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
 
 // CellList::FindCellCoords over every bond (src/cell_list.cpp:139-157) and
 // the population of each cell column (x, y) (n1d^2 counters instead of
-// n1d^3, so the scan that follows is small).  Streaming passes loop over
-// their range, so any grid size serves.
+// n1d^3, so the scan that follows is small); the count each bond's atomic
+// returns is its place in its column.  Streaming passes loop over their
+// range, so any grid size serves.
 __global__ void k_keys(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -261,7 +262,7 @@ __global__ void k_keys(Dev d, Geo g) {
       k = cell_key(g.n1d, rec[0], rec[1], rec[2]);
     }
     d.keys[b] = k;
-    atomicAdd(&d.col_count[k / g.n1d], 1);
+    d.col_rank[b] = atomicAdd(&d.col_count[k / g.n1d], 1);
   }
 }
 
@@ -269,18 +270,18 @@ __global__ void k_keys(Dev d, Geo g) {
 __global__ void k_key_count(Dev d, Geo g) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride)
-    atomicAdd(&d.col_count[d.keys[b] / g.n1d], 1);
+    d.col_rank[b] = atomicAdd(&d.col_count[d.keys[b] / g.n1d], 1);
 }
 
-// Groups the bonds by column (order inside a column arbitrary; the rank
-// sort orders it) and returns col_count to all-zero for the next build.
+// Groups the bonds by column at the place the population count handed
+// each bond (order inside a column arbitrary; the rank sort orders it): a
+// pure scatter, no atomics.
 __global__ void k_scatter(Dev d, Geo g, int guard) {
   if (guard_off(d.ctl, guard)) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
     const int32_t c = d.keys[b] / g.n1d;
-    const int32_t pos = d.col_start[c] + atomicSub(&d.col_count[c], 1) - 1;
-    d.col_tmp[pos] = (int32_t)b;
+    d.col_tmp[d.col_start[c] + d.col_rank[b]] = (int32_t)b;
   }
 }
 
@@ -292,7 +293,7 @@ __global__ void k_scatter(Dev d, Geo g, int guard) {
 // order, the order Cell::AddObj appends in (src/cell.cpp:5-8).  The pass
 // also writes the run of each occupied cell and the column's occupancy
 // words (every word of the column, so no clearing pass is needed).
-constexpr int kColStage = 512;
+constexpr int kColStage = 256;
 
 __device__ __forceinline__ int64_t kb_pair(const Dev &d, int32_t b) {
   return ((int64_t)d.keys[b] << 32) | (uint32_t)b;
@@ -309,6 +310,7 @@ __global__ __launch_bounds__(64) void k_colsort(Dev d, Geo g, int guard) {
     const int32_t s = d.col_start[c], e = d.col_start[c + 1], m = e - s;
     const bool staged = m <= kColStage;
     if (lane < W) occ[lane] = 0u;
+    if (lane == 0) d.col_count[c] = 0;  // for the next build's count
     if (staged)
       for (int i = lane; i < m; i += 64) ks[i] = kb_pair(d, d.col_tmp[s + i]);
     __syncthreads();
