@@ -446,14 +446,41 @@ __device__ __forceinline__ void deg_one(const Dev &d, const Geo &g, int64_t p) {
   const int cx = k / (n * n), cy = (k / n) % n, cz = k % n;
   const uint32_t m = shell_mask(d, g, cx, cy, cz);
   d.nmask[b] = m;
-  for (uint32_t mm = m; mm; mm &= mm - 1) {
+  // the bond's own run and those of its first kRunSlab - 1 occupied
+  // stencil cells (in bit order) go to the slab of its sorted position
+  // (written in order, so the stores coalesce), where k_pull finds them
+  // without more dependent lookups
+  // (the first kRunSlab - 1 lookups are unrolled and independent, so they
+  // are in flight together; the slab is stored after them)
+  uint32_t mm = m;
+  int2 r[kRunSlab - 1];
+#pragma unroll
+  for (int i = 0; i < kRunSlab - 1; ++i) {
+    r[i] = make_int2(0, 0);
+    if (mm) {
+      const int q = __ffs(mm) - 1;
+      mm &= mm - 1;
+      const int32_t kc = q < 13 ? shell_key<false>(n, cx, cy, cz, q)
+                                : shell_key<true>(n, cx, cy, cz, q - 13);
+      r[i] = d.cell_run[kc];
+      deg += (r[i].y - r[i].x) - (klo == kc) - (khi == kc);
+    }
+  }
+  for (; mm; mm &= mm - 1) {
     const int q = __ffs(mm) - 1;
     const int32_t kc = q < 13 ? shell_key<false>(n, cx, cy, cz, q)
                               : shell_key<true>(n, cx, cy, cz, q - 13);
-    const int2 r = d.cell_run[kc];
-    deg += (r.y - r.x) - (klo == kc) - (khi == kc);
+    const int2 rr = d.cell_run[kc];
+    deg += (rr.y - rr.x) - (klo == kc) - (khi == kc);
   }
   d.deg[b] = deg;
+  d.pos_of[b] = (int32_t)p;
+  int4 *slab = reinterpret_cast<int4 *>(d.nrun + p * kRunSlab);
+  static_assert(kRunSlab == 8, "four int4 stores per slab");
+  slab[0] = make_int4(own.x, own.y, r[0].x, r[0].y);
+  slab[1] = make_int4(r[1].x, r[1].y, r[2].x, r[2].y);
+  slab[2] = make_int4(r[3].x, r[3].y, r[4].x, r[4].y);
+  slab[3] = make_int4(r[5].x, r[5].y, r[6].x, r[6].y);
 }
 
 // Grid-stride over the sorted positions: a capped grid keeps the launch
@@ -511,7 +538,25 @@ __device__ __forceinline__ void pull_one(const Dev &d, const Geo &g, int32_t b,
   const int cls = (cx == 0 ? 0 : cx == n - 1 ? 2 : 1) * 6 +
                   (cy == 0 ? 0 : cy == n - 1 ? 2 : 1) * 2 + (cz == 0 ? 0 : 1);
   const uint32_t m = d.nmask[b];  // occupied neighbours (k_deg)
-  const int2 own = d.cell_run[k];
+  // the runs k_deg kept: own cell first, then the occupied stencil cells
+  // in bit order; one beyond the slab is looked up in the cell list
+  const int4 *sl = reinterpret_cast<const int4 *>(d.nrun + (int64_t)d.pos_of[b] * kRunSlab);
+  static_assert(kRunSlab == 8, "four int4 loads per slab");
+  const int4 s0 = sl[0], s1 = sl[1], s2 = sl[2], s3 = sl[3];
+  const int2 own = make_int2(s0.x, s0.y);
+  auto run_of = [&](int q, int32_t key) -> int2 {
+    const int idx = 1 + __popc(m & ((1u << q) - 1u));
+    switch (idx) {
+      case 1: return make_int2(s0.z, s0.w);
+      case 2: return make_int2(s1.x, s1.y);
+      case 3: return make_int2(s1.z, s1.w);
+      case 4: return make_int2(s2.x, s2.y);
+      case 5: return make_int2(s2.z, s2.w);
+      case 6: return make_int2(s3.x, s3.y);
+      case 7: return make_int2(s3.z, s3.w);
+      default: return d.cell_run[key];
+    }
+  };
   uint64_t order = g.home_order[cls];
   for (int step = 0; step < 14; ++step, order >>= 4) {
     const int c = (int)(order & 15);
@@ -523,12 +568,13 @@ __device__ __forceinline__ void pull_one(const Dev &d, const Geo &g, int32_t b,
       for (int32_t j = own.x; j < r; ++j) put(d.sorted[j], 1);
       for (int32_t j = r + 1; j < own.y; ++j) put(d.sorted[j], 0);
       for (uint32_t mm = m & 0x1fffu; mm; mm &= mm - 1) {
-        const int2 rn = d.cell_run[shell_key<false>(n, cx, cy, cz, __ffs(mm) - 1)];
+        const int q = __ffs(mm) - 1;
+        const int2 rn = run_of(q, shell_key<false>(n, cx, cy, cz, q));
         for (int32_t j = rn.x; j < rn.y; ++j) put(d.sorted[j], 0);
       }
     } else if ((m >> (13 + c)) & 1u) {
       // mirrored half-shell cell c: b is obj2 of each of its members
-      const int2 rn = d.cell_run[shell_key<true>(n, cx, cy, cz, c)];
+      const int2 rn = run_of(13 + c, shell_key<true>(n, cx, cy, cz, c));
       for (int32_t j = rn.x; j < rn.y; ++j) put(d.sorted[j], 1);
     }
   }
