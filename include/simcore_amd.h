@@ -359,16 +359,22 @@ int sc_generate_crosslinks(const sc_params *params, int32_t n_filaments,
  * n_runs use case, src/simulation_manager.cpp:461-545, which runs them one
  * after another): requests are queued, admitted while the runner's device
  * memory (one arena, fixed at creation) and its concurrency limit allow,
- * kept resident on the device, advanced on their own streams so their work
- * overlaps, and retired -- final positions copied to the host, memory
- * returned -- as they finish.  Admission is first-fit in arrival order.
- * A system's result does not depend on what else runs beside it. */
+ * kept resident on the device, advanced in chunks of steps, and retired --
+ * final positions copied to the host, memory returned -- as they finish.
+ * Admission is first-fit in arrival order.  Mixed-precision systems without
+ * crosslinks, thermodynamic sums or filaments above 128 sites are stepped
+ * in batches: one set of launches per step for up to batch_size systems,
+ * members joining and leaving at chunk boundaries; the others run their
+ * chunks on their own streams.  A system's result does not depend on what
+ * else runs beside it. */
 typedef struct sc_ensemble_s *sc_ensemble;
 
 typedef struct sc_ensemble_params {
   size_t hbm_budget;        /* arena bytes; 0: 80% of the free device memory */
   int32_t max_active;       /* systems resident at once; 0: 16             */
-  int32_t steps_per_launch; /* steps enqueued per system per round; 0: 64  */
+  int32_t steps_per_launch; /* steps per chunk; 0: 64                       */
+  int32_t batch_size;       /* systems per batch, at most 32; 0: 8; 1: no
+                               batches (every system on its own stream)     */
 } sc_ensemble_params;
 
 /* request status */
@@ -393,6 +399,8 @@ typedef struct sc_ensemble_stats {
   int32_t peak_active;   /* most systems resident at once                  */
   size_t peak_bytes;     /* most arena bytes in use at once                */
   size_t arena_bytes;
+  int64_t batched_steps; /* system-steps taken in batches                  */
+  int64_t solo_steps;    /* system-steps taken on the systems' own streams */
 } sc_ensemble_stats;
 
 int sc_ensemble_create(const sc_ensemble_params *params, sc_ensemble *out);

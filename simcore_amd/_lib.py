@@ -72,7 +72,7 @@ class ScFilamentParams(ctypes.Structure):
 
 class ScEnsembleParams(ctypes.Structure):
     _fields_ = [("hbm_budget", c_size_t), ("max_active", c_int32),
-                ("steps_per_launch", c_int32)]
+                ("steps_per_launch", c_int32), ("batch_size", c_int32)]
 
 
 class ScRequestInfo(ctypes.Structure):
@@ -89,7 +89,8 @@ class ScEnsembleStats(ctypes.Structure):
                 ("latency_mean", c_double), ("latency_max", c_double),
                 ("segment_steps", c_double), ("span", c_double),
                 ("peak_active", c_int32), ("peak_bytes", c_size_t),
-                ("arena_bytes", c_size_t)]
+                ("arena_bytes", c_size_t), ("batched_steps", c_int64),
+                ("solo_steps", c_int64)]
 
 
 class ScCrosslinkParams(ctypes.Structure):

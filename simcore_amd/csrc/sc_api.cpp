@@ -891,6 +891,37 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
   return SC_OK;
 }
 
+bool system_batchable(sc_handle h) {
+  return h && h->loaded && !h->pending && !h->prof && h->p.precision == SC_PREC_MIXED &&
+         h->g.nx == 0 && !h->g.thermo && !h->g.xi_buf && h->long_last < 0 &&
+         h->pack_waves > 0 && h->rec[0] && !h->ctl_host->abort && (h->i_step_host & 1) == 0;
+}
+
+void system_member(sc_handle h, Member *m) {
+  m->d[0] = h->d;
+  m->d[0].br = h->rec[0];
+  m->d[0].br_out = h->rec[1];
+  m->d[1] = h->d;
+  m->d[1].br = h->rec[1];
+  m->d[1].br_out = h->rec[0];
+  m->g = h->g;
+  m->pack_tab = h->pack_tab;
+  m->pack_waves = h->pack_waves;
+  m->cs1 = h->pack_cs[1];
+  m->cs2 = h->pack_cs[2];
+}
+
+int system_batch_done(sc_handle h, const Ctl &c, int64_t target, int *done) {
+  // the batch's range becomes this handle's pending range, completed on
+  // the (idle) system stream, and is polled as such
+  *h->ctl_host = c;
+  h->target = target;
+  h->grow_rounds = 0;
+  h->pending = true;
+  HIPCHK(h, hipEventRecord(h->done_ev, h->stream));
+  return sc_step_poll(h, done);
+}
+
 size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, int n_bins) {
   const int64_t n1d = cells_per_side(*params);
   const int64_t n_cells = n1d * n1d * n1d;

@@ -4,13 +4,21 @@
 // The reference runs the n_runs of a parameter file one after another
 // (SimulationManager::RunSimulations, src/simulation_manager.cpp:461-545).
 // Here requests arrive at any time and share the device: a request is
-// admitted when a stream slot is free and its footprint fits the runner's
-// arena, its state stays resident until its last step, its steps are
-// enqueued in chunks on its own stream (work of different systems overlaps
-// on the device), and it is retired -- final positions copied out, arena
-// blocks returned -- as soon as it finishes, which lets the next request in.
-// Systems never share buffers or streams while resident, so a result is the
-// same whatever else runs beside it.
+// admitted when a slot is free and its footprint fits the runner's arena,
+// its state stays resident until its last step, and it is retired -- final
+// positions copied out, arena blocks returned -- as soon as it finishes,
+// which lets the next request in.
+//
+// Resident systems are stepped in chunks.  Systems that qualify
+// (system_batchable) are stepped together: a batch is a set of up to
+// batch_size systems whose chunk is one sequence of grouped launches
+// (each kernel of the step covers every member, block ranges per member),
+// so a step of sixteen small systems costs the launches of one.  Members
+// join and leave at chunk boundaries; a chunk is an even number of steps,
+// so every member starts it at an odd step.  The others run their chunks
+// on their own streams (graph replay).  Systems never share buffers, and a
+// member's blocks run the single-system kernels' code on its own data, so
+// a result is the same whatever else runs beside it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +30,7 @@
 #include <thread>
 #include <vector>
 
+#include "sc_internal.h"
 #include "sc_system.h"
 
 namespace sc {
@@ -112,8 +121,22 @@ struct Request {
   size_t est_bytes = 0, bytes = 0;
   sc_handle h = nullptr;
   int slot = -1;
+  int batch = -1;     // batch running its current chunk, -1: none
+  bool solo = false;  // a chunk pending on the system's own stream
   std::vector<double> r_out;
   std::string err;
+};
+
+// One batch: its stream, member tables (pinned host copy, device copy),
+// the control blocks gathered at the end of a chunk, and the members of
+// the chunk in flight.
+struct Batch {
+  hipStream_t st = nullptr;
+  sc::Member *hm = nullptr, *dm = nullptr;
+  sc::Ctl *hctl = nullptr, *dctl = nullptr;
+  hipEvent_t ev = nullptr;
+  std::vector<int64_t> ids;
+  bool busy = false;
 };
 
 int count_bins(const std::vector<int32_t> &so) {
@@ -142,10 +165,12 @@ struct sc_ensemble_s {
   std::vector<std::unique_ptr<Request>> reqs;  // index = request id
   std::deque<int64_t> queue;
   std::vector<int64_t> running;
+  std::vector<Batch> batches;
   std::chrono::steady_clock::time_point t0;
   std::string err;
   int32_t peak_active = 0;
   size_t peak_bytes = 0;
+  int64_t batched_steps = 0, solo_steps = 0;
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -172,8 +197,72 @@ int enqueue_chunk(sc_ensemble e, Request &R) {
   const int64_t k = std::min<int64_t>(e->prm.steps_per_launch, R.n_steps - R.enqueued);
   if (k <= 0) return SC_OK;
   const int rc = sc_step_async(R.h, (int32_t)k);
-  if (rc == SC_OK) R.enqueued += k;
+  if (rc == SC_OK) {
+    R.enqueued += k;
+    R.solo = true;
+    e->solo_steps += k;
+  }
   return rc;
+}
+
+// Enqueues one chunk of batch B over the members in B.ids: k steps (even,
+// at most every member's remaining steps), then the members' control
+// blocks copied back, then B's event.
+int enqueue_batch(sc_ensemble e, Batch &B) {
+  const int n = (int)B.ids.size();
+  int64_t k = e->prm.steps_per_launch;
+  for (int64_t id : B.ids) k = std::min(k, e->reqs[id]->n_steps - e->reqs[id]->enqueued);
+  k &= ~(int64_t)1;
+  if (n == 0 || k <= 0) return SC_ERR_STATE;
+  for (int j = 0; j < n; ++j) {
+    Request &R = *e->reqs[B.ids[j]];
+    sc::system_member(R.h, &B.hm[j]);
+    R.batch = (int)(&B - e->batches.data());
+    R.enqueued += k;
+  }
+  if (hipMemcpyAsync(B.dm, B.hm, sizeof(sc::Member) * n, hipMemcpyHostToDevice, B.st) !=
+      hipSuccess)
+    return SC_ERR_HIP;
+  for (int64_t s = 0; s < k; s += 2) {
+    // odd step (the rebuild stage, then the midstep), even step
+    sc::launch_group_rebuild(B.dm, B.hm, n, B.st);
+    sc::launch_group_pair(B.dm, B.hm, n, true, B.st);
+    sc::launch_group_integrate(B.dm, B.hm, n, true, B.st);
+    sc::launch_group_commit(B.dm, n, B.st);
+    sc::launch_group_pair(B.dm, B.hm, n, false, B.st);
+    sc::launch_group_integrate(B.dm, B.hm, n, false, B.st);
+    sc::launch_group_commit(B.dm, n, B.st);
+  }
+  sc::launch_group_ctl(B.dm, n, B.dctl, B.st);
+  if (hipMemcpyAsync(B.hctl, B.dctl, sizeof(sc::Ctl) * n, hipMemcpyDeviceToHost, B.st) !=
+          hipSuccess ||
+      hipGetLastError() != hipSuccess || hipEventRecord(B.ev, B.st) != hipSuccess)
+    return SC_ERR_HIP;
+  B.busy = true;
+  e->batched_steps += k * n;
+  return SC_OK;
+}
+
+void fail_request(sc_ensemble e, Request &R, int rc) {
+  R.status = rc;
+  if (R.err.empty()) R.err = R.h ? sc_last_error(R.h) : "request failed";
+  R.t_done = e->now();
+  retire(e, R);
+}
+
+// R's last chunk has ended (R idle): retire it if finished
+bool finish_if_done(sc_ensemble e, Request &R) {
+  if (R.enqueued < R.n_steps) return false;
+  R.r_out.resize(3 * (size_t)R.ns);
+  const int rc = sc_get_sites(R.h, R.r_out.data(), nullptr, nullptr);
+  if (rc != SC_OK) {
+    fail_request(e, R, rc);
+    return true;
+  }
+  R.status = SC_REQ_DONE;
+  R.t_done = e->now();
+  retire(e, R);
+  return true;
 }
 
 // Creates and loads R's system in slot `slot`; SC_ERR_NOMEM leaves R queued.
@@ -225,6 +314,33 @@ int sc_ensemble_create(const sc_ensemble_params *params, sc_ensemble *out) {
     e->streams.push_back(st);
     e->free_slots.push_back(e->prm.max_active - 1 - k);
   }
+  if (e->prm.batch_size <= 0) e->prm.batch_size = 8;
+  e->prm.batch_size = std::min<int32_t>(e->prm.batch_size, sc::kMaxGroup);
+  if (e->prm.batch_size > 1) {
+    // enough batches for every resident system, at most four (one chunk
+    // runs while the host sets up the next)
+    const int nbat =
+        std::min(4, (e->prm.max_active + e->prm.batch_size - 1) / e->prm.batch_size);
+    e->batches.resize(nbat);
+    for (Batch &B : e->batches) {
+      void *hm = nullptr, *hc = nullptr;
+      if (hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&B.ev, hipEventDisableTiming) != hipSuccess ||
+          hipHostMalloc(&hm, sizeof(sc::Member) * sc::kMaxGroup, hipHostMallocDefault) !=
+              hipSuccess ||
+          hipHostMalloc(&hc, sizeof(sc::Ctl) * sc::kMaxGroup, hipHostMallocDefault) !=
+              hipSuccess ||
+          hipMalloc(&B.dm, sizeof(sc::Member) * sc::kMaxGroup) != hipSuccess ||
+          hipMalloc(&B.dctl, sizeof(sc::Ctl) * sc::kMaxGroup) != hipSuccess) {
+        B.hm = static_cast<sc::Member *>(hm);
+        B.hctl = static_cast<sc::Ctl *>(hc);
+        sc_ensemble_destroy(e.release());
+        return SC_ERR_HIP;
+      }
+      B.hm = static_cast<sc::Member *>(hm);
+      B.hctl = static_cast<sc::Ctl *>(hc);
+    }
+  }
   e->t0 = std::chrono::steady_clock::now();
   *out = e.release();
   return SC_OK;
@@ -263,32 +379,48 @@ int sc_ensemble_submit(sc_ensemble e, const sc_params *params, int32_t n_filamen
 
 int sc_ensemble_progress(sc_ensemble e, int32_t *n_running, int32_t *n_queued) {
   if (!e) return SC_ERR_ARG;
-  // 1. running systems: finished chunks get the next one, finished
-  //    requests are retired
+  // 1. batches whose chunk has ended: each member's outcome as its own
+  //    poll would report it
+  for (Batch &B : e->batches) {
+    if (!B.busy) continue;
+    const hipError_t q = hipEventQuery(B.ev);
+    if (q == hipErrorNotReady) continue;
+    B.busy = false;
+    for (size_t j = 0; j < B.ids.size(); ++j) {
+      Request &R = *e->reqs[B.ids[j]];
+      R.batch = -1;
+      int done = 0;
+      const int rc = q != hipSuccess ? SC_ERR_HIP
+                                     : sc::system_batch_done(R.h, B.hctl[j], R.enqueued, &done);
+      if (rc != SC_OK) {
+        if (q != hipSuccess) R.err = hipGetErrorString(q);
+        fail_request(e, R, rc);
+      } else if (!done) {
+        R.solo = true;  // a capacity re-run on the system's own stream
+      }
+    }
+    B.ids.clear();
+  }
+  // 2. chunks on the systems' own streams
   const std::vector<int64_t> run = e->running;
   for (int64_t id : run) {
     Request &R = *e->reqs[id];
+    if (!R.solo) continue;
     int done = 0;
-    int rc = sc_step_poll(R.h, &done);
-    if (rc == SC_OK && done && R.enqueued < R.n_steps) rc = enqueue_chunk(e, R);
-    else if (rc == SC_OK && done) {
-      R.r_out.resize(3 * (size_t)R.ns);
-      rc = sc_get_sites(R.h, R.r_out.data(), nullptr, nullptr);
-      if (rc == SC_OK) {
-        R.status = SC_REQ_DONE;
-        R.t_done = e->now();
-        retire(e, R);
-        continue;
-      }
-    }
+    const int rc = sc_step_poll(R.h, &done);
     if (rc != SC_OK) {
-      R.status = rc;
-      R.err = sc_last_error(R.h);
-      R.t_done = e->now();
-      retire(e, R);
+      fail_request(e, R, rc);
+      continue;
     }
+    if (done) R.solo = false;
   }
-  // 2. admission, first fit in arrival order
+  // 3. finished systems leave (before admission, which may use their room)
+  const std::vector<int64_t> idle_scan = e->running;
+  for (int64_t id : idle_scan) {
+    Request &R = *e->reqs[id];
+    if (!R.solo && R.batch < 0) finish_if_done(e, R);
+  }
+  // 4. admission, first fit in arrival order
   for (auto it = e->queue.begin(); it != e->queue.end() && !e->free_slots.empty();) {
     Request &R = *e->reqs[*it];
     if (R.est_bytes > e->arena.largest_free()) {
@@ -313,15 +445,56 @@ int sc_ensemble_progress(sc_ensemble e, int32_t *n_running, int32_t *n_queued) {
       continue;
     }
     e->running.push_back(R.id);
-    const int rc2 = enqueue_chunk(e, R);
-    if (rc2 != SC_OK) {
-      R.status = rc2;
-      R.err = sc_last_error(R.h);
-      R.t_done = e->now();
-      retire(e, R);
-    }
     e->peak_active = std::max<int32_t>(e->peak_active, (int32_t)e->running.size());
     e->peak_bytes = std::max(e->peak_bytes, e->arena.used());
+  }
+  // 5. idle systems take their next chunk, in a free batch when they
+  //    qualify, else on their own stream (a request of 0 steps finishes)
+  const std::vector<int64_t> idle_now = e->running;
+  std::vector<int64_t> ready;
+  for (int64_t id : idle_now) {
+    Request &R = *e->reqs[id];
+    if (R.solo || R.batch >= 0) continue;
+    if (finish_if_done(e, R)) continue;
+    if (!e->batches.empty() && R.n_steps - R.enqueued >= 2 && sc::system_batchable(R.h)) {
+      ready.push_back(id);
+      continue;
+    }
+    const int rc = enqueue_chunk(e, R);
+    if (rc != SC_OK) fail_request(e, R, rc);
+  }
+  if (!ready.empty()) {
+    // largest first onto the free batch with the fewest bonds so far
+    std::sort(ready.begin(), ready.end(), [&](int64_t a, int64_t b) {
+      return e->reqs[a]->ns - e->reqs[a]->nf > e->reqs[b]->ns - e->reqs[b]->nf;
+    });
+    std::vector<int64_t> load(e->batches.size(), 0);
+    for (int64_t id : ready) {
+      int best = -1;
+      for (size_t b = 0; b < e->batches.size(); ++b) {
+        const Batch &B = e->batches[b];
+        if (B.busy || (int)B.ids.size() >= e->prm.batch_size) continue;
+        if (best < 0 || load[b] < load[best]) best = (int)b;
+      }
+      if (best < 0) break;  // every batch busy or full: wait for a boundary
+      e->batches[best].ids.push_back(id);
+      load[best] += e->reqs[id]->ns - e->reqs[id]->nf;
+    }
+    for (Batch &B : e->batches) {
+      if (B.busy || B.ids.empty()) continue;
+      const int rc = enqueue_batch(e, B);
+      if (rc != SC_OK) {
+        // nothing of the chunk is known to have run: fail its members
+        for (int64_t id : B.ids) {
+          Request &R = *e->reqs[id];
+          R.batch = -1;
+          R.err = "batch launch failed";
+          fail_request(e, R, rc);
+        }
+        B.ids.clear();
+        B.busy = false;
+      }
+    }
   }
   if (n_running) *n_running = (int32_t)e->running.size();
   if (n_queued) *n_queued = (int32_t)e->queue.size();
@@ -418,6 +591,8 @@ int sc_ensemble_stats_get(sc_ensemble e, sc_ensemble_stats *out) {
   out->peak_active = e->peak_active;
   out->peak_bytes = e->peak_bytes;
   out->arena_bytes = e->arena.capacity();
+  out->batched_steps = e->batched_steps;
+  out->solo_steps = e->solo_steps;
   return SC_OK;
 }
 
@@ -429,6 +604,16 @@ const char *sc_ensemble_last_error(sc_ensemble e) {
 
 int sc_ensemble_destroy(sc_ensemble e) {
   if (!e) return SC_ERR_ARG;
+  for (Batch &B : e->batches) {
+    if (B.st) (void)hipStreamSynchronize(B.st);
+    if (B.dm) (void)hipFree(B.dm);
+    if (B.dctl) (void)hipFree(B.dctl);
+    if (B.hm) (void)hipHostFree(B.hm);
+    if (B.hctl) (void)hipHostFree(B.hctl);
+    if (B.ev) (void)hipEventDestroy(B.ev);
+    if (B.st) (void)hipStreamDestroy(B.st);
+  }
+  e->batches.clear();
   for (auto &p : e->reqs)
     if (p->h) {
       sc_destroy(p->h);

@@ -762,19 +762,19 @@ void k_integrate_packed(Dev d, Geo g, const int32_t *lane_tab, int n_waves) {
 // the spills of a 128-register budget cost more than the fourth wave
 // gains); fewer launches than one per width.
 template <bool MID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-void k_integrate_packed_any(Dev d, Geo g, const int32_t *lane_tab, int n_waves, int cs1,
-                            int cs2) {
+__device__ __forceinline__ void packed_any_body(const Dev &d, const Geo &g,
+                                                const int32_t *lane_tab, int n_waves, int cs1,
+                                                int cs2, int bid) {
   Ctl *ctl = d.ctl;
   if (ctl->abort) return;
   if ((ctl->midstep != 0) != MID) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
       ctl->err = SC_ERR_STATE;
       ctl->abort = 1;
     }
     return;
   }
-  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int wave = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave >= n_waves) return;
   const int lane = threadIdx.x & 63;
   const int32_t e = lane_tab[(int64_t)wave * 64 + lane];
@@ -789,6 +789,40 @@ void k_integrate_packed_any(Dev d, Geo g, const int32_t *lane_tab, int n_waves, 
     long_body<32, 2, MID>(d, g, factive, f, gl, Gf, wave);
   else
     long_body<64, 2, MID>(d, g, factive, f, gl, Gf, wave);
+}
+
+template <bool MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_integrate_packed_any(Dev d, Geo g, const int32_t *lane_tab, int n_waves, int cs1,
+                            int cs2) {
+  packed_any_body<MID>(d, g, lane_tab, n_waves, cs1, cs2, blockIdx.x);
+}
+
+// Ensemble batches: every member's packed waves, one wave per block.  Odd
+// steps (midsteps) use the members' odd-step pointers.
+template <bool MID>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+void k_group_integrate(const Member *__restrict__ ms, GroupGrid gg) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  packed_any_body<MID>(M.d[MID ? 0 : 1], M.g, M.pack_tab, M.pack_waves, M.cs1, M.cs2,
+                       blockIdx.x - b0);
+}
+
+void launch_group_integrate(const Member *dm, const Member *hm, int n, bool odd,
+                            hipStream_t st) {
+  GroupGrid gg;
+  int32_t t = 0;
+  for (int m = 0; m < n; ++m) {
+    gg.off[m] = t;
+    t += hm[m].pack_waves;
+  }
+  for (int m = n; m <= kMaxGroup; ++m) gg.off[m] = t;
+  if (t <= 0) return;
+  if (odd)
+    hipLaunchKernelGGL((k_group_integrate<true>), dim3((unsigned)t), dim3(64), 0, st, dm, gg);
+  else
+    hipLaunchKernelGGL((k_group_integrate<false>), dim3((unsigned)t), dim3(64), 0, st, dm, gg);
 }
 
 void launch_integrate_packed_all(const Dev &d, const Geo &g, const int32_t *lane_tab,

@@ -178,6 +178,49 @@ struct BinDesc {
   int width;            // packed: lane-group width of the waves (16/32/64)
 };
 
+// Grouped launches: one launch steps several independent systems (a batch
+// of the ensemble runner), each with its own buffers and constants.  Block
+// b of a grouped launch belongs to member m with off[m] <= b < off[m + 1];
+// the entries past the last member repeat the total.  Each member's blocks
+// run the single-system kernel's code with the member's block index and
+// block count, so a member's results are those of its own launches.
+constexpr int kMaxGroup = 32;
+struct Member {
+  Dev d[2];                 // [0]: odd steps (read rec[0], write rec[1]); [1]: even
+  Geo g;
+  const int32_t *pack_tab;  // packed integrator lane table
+  int pack_waves, cs1, cs2; // its waves; first wave of width 32, of width 64
+};
+struct GroupGrid {
+  int32_t off[kMaxGroup + 1];
+};
+
+// Member of block `bid`, its first block and block count: a uniform scan
+// over the offsets (constant indices, so they stay kernel arguments).
+__device__ __forceinline__ int group_member(const GroupGrid &gg, int bid, int *b0, int *nblk) {
+  int m = 0, lo = 0, hi = gg.off[1];
+#pragma unroll
+  for (int i = 1; i < kMaxGroup; ++i) {
+    const bool past = bid >= gg.off[i];
+    m = past ? i : m;
+    lo = past ? gg.off[i] : lo;
+    hi = past ? gg.off[i + 1] : hi;
+  }
+  *b0 = lo;
+  *nblk = hi - lo;
+  return m;
+}
+
+// Grouped launchers (sc_kernels.hip, sc_pair.hip, sc_integrate_long.hip):
+// `dm` the device copy of the members, `hm` the host copy (sizes), n
+// members, `odd` the parity of the step all of them take.
+void launch_group_rebuild(const Member *dm, const Member *hm, int n, hipStream_t st);
+void launch_group_pair(const Member *dm, const Member *hm, int n, bool odd, hipStream_t st);
+void launch_group_integrate(const Member *dm, const Member *hm, int n, bool odd, hipStream_t st);
+void launch_group_commit(const Member *dm, int n, hipStream_t st);
+// each member's control block into out[m]
+void launch_group_ctl(const Member *dm, int n, Ctl *out, hipStream_t st);
+
 
 // Per-bond force / torque component q of bond j (summed in FP64 by the pair
 // kernel; stored in FP32 on the mixed path).

@@ -77,7 +77,7 @@ __global__ void k_decide(Ctl *c, unsigned long long *slots, double thr) {
 // end of a step: commit it and decide the next one (one launch for both).
 // Every value it needs is loaded up front, so the kernel costs one memory
 // round trip; the decision is decide_next's on those values.
-__global__ void k_commit(Ctl *c, unsigned long long *slots, double thr) {
+__device__ __forceinline__ void commit_body(Ctl *c, unsigned long long *slots, double thr) {
   static_assert(kDrSlots == 4 * 64, "four slots per lane");
   const int lane = threadIdx.x;
   unsigned long long v[4];
@@ -86,7 +86,7 @@ __global__ void k_commit(Ctl *c, unsigned long long *slots, double thr) {
   const int32_t abort = c->abort;
   const int64_t cur = c->cur_step;
   const unsigned long long drb = c->dr_bits;
-  if (blockIdx.x != 0 || abort) return;
+  if (abort) return;
   unsigned long long m = drb;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -115,6 +115,22 @@ __global__ void k_commit(Ctl *c, unsigned long long *slots, double thr) {
   }
   c->dr_bits = (k & 1) ? m : 0ull;  // a full step clears what it will fill
   c->rebuild = rb;
+}
+
+__global__ void k_commit(Ctl *c, unsigned long long *slots, double thr) {
+  if (blockIdx.x != 0) return;
+  commit_body(c, slots, thr);
+}
+
+// one block per member
+__global__ void k_group_commit(const Member *__restrict__ ms) {
+  const Member &M = ms[blockIdx.x];
+  commit_body(M.d[0].ctl, M.d[0].dr_slots, M.g.thr);
+}
+
+__global__ void k_group_ctl(const Member *__restrict__ ms, int n, Ctl *out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n) out[m] = *ms[m].d[0].ctl;
 }
 
 __global__ void k_zero_i32(int32_t *p, int64_t n) {
@@ -157,12 +173,11 @@ __device__ __forceinline__ int block_excl_scan256(int v, int *total,
   return base + x - v;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(
-    const int32_t *in, int64_t n, int32_t *bsum, const Ctl *ctl, int guard) {
-  if (guard_off(ctl, guard)) return;
+__device__ __forceinline__ void scan_reduce_body(const int32_t *in, int64_t n, int32_t *bsum,
+                                                 int64_t bid, int64_t nblk) {
   __shared__ int wsum[kScanBlock / 64];
   const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int64_t t = bid; t < ntiles; t += nblk) {
     const int64_t base = t * kScanTile;
     int s = 0;
 #pragma unroll
@@ -174,6 +189,12 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(
     block_excl_scan256(s, &tot, wsum);
     if (threadIdx.x == 0) bsum[t] = tot;
   }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(
+    const int32_t *in, int64_t n, int32_t *bsum, const Ctl *ctl, int guard) {
+  if (guard_off(ctl, guard)) return;
+  scan_reduce_body(in, n, bsum, blockIdx.x, gridDim.x);
 }
 
 __device__ void tile_scan(int *tile, int *wsum, int carry_in, int *total) {
@@ -200,17 +221,12 @@ __device__ void tile_scan(int *tile, int *wsum, int carry_in, int *total) {
 // few thousand, read from L2) and scans itself; the last tile also writes
 // the total.  Two launches per scan; blocks loop over tiles, so any grid
 // size serves.
-__global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
-                                                          int64_t n,
-                                                          const int32_t *bsum,
-                                                          int32_t *out,
-                                                          const Ctl *ctl,
-                                                          int guard) {
-  if (guard_off(ctl, guard)) return;
+__device__ __forceinline__ void scan_down_body(const int32_t *in, int64_t n, const int32_t *bsum,
+                                               int32_t *out, int64_t bid, int64_t nblk) {
   __shared__ int tile[kScanTile];
   __shared__ int wsum[kScanBlock / 64];
   const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int64_t t = bid; t < ntiles; t += nblk) {
     int part = 0;
     for (int64_t i = threadIdx.x; i < t; i += kScanBlock) part += bsum[i];
     int carry;
@@ -226,6 +242,16 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
     if (t == ntiles - 1 && threadIdx.x == 0) out[n] = carry + tot;
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_down(const int32_t *in,
+                                                          int64_t n,
+                                                          const int32_t *bsum,
+                                                          int32_t *out,
+                                                          const Ctl *ctl,
+                                                          int guard) {
+  if (guard_off(ctl, guard)) return;
+  scan_down_body(in, n, bsum, out, blockIdx.x, gridDim.x);
 }
 
 int64_t scan_tmp_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
@@ -250,10 +276,9 @@ void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
 // n1d^3, so the scan that follows is small); the count each bond's atomic
 // returns is its place in its column.  Streaming passes loop over their
 // range, so any grid size serves.
-__global__ void k_keys(Dev d, Geo g) {
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
+__device__ __forceinline__ void keys_body(const Dev &d, const Geo &g, int64_t bid, int64_t nblk) {
+  const int64_t stride = nblk * blockDim.x;
+  for (int64_t b = bid * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
     int32_t k;
     if (g.compact) {
       k = rec_key_compact(d.br, g, b);
@@ -266,6 +291,11 @@ __global__ void k_keys(Dev d, Geo g) {
   }
 }
 
+__global__ void k_keys(Dev d, Geo g) {
+  if (guard_off(d.ctl, kGuardRebuild)) return;
+  keys_body(d, g, blockIdx.x, gridDim.x);
+}
+
 // Column populations from the stored keys (cell list restore).
 __global__ void k_key_count(Dev d, Geo g) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -276,13 +306,18 @@ __global__ void k_key_count(Dev d, Geo g) {
 // Groups the bonds by column at the place the population count handed
 // each bond (order inside a column arbitrary; the rank sort orders it): a
 // pure scatter, no atomics.
-__global__ void k_scatter(Dev d, Geo g, int guard) {
-  if (guard_off(d.ctl, guard)) return;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
+__device__ __forceinline__ void scatter_body(const Dev &d, const Geo &g, int64_t bid,
+                                             int64_t nblk) {
+  const int64_t stride = nblk * blockDim.x;
+  for (int64_t b = bid * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
     const int32_t c = d.keys[b] / g.n1d;
     d.col_tmp[d.col_start[c] + d.col_rank[b]] = (int32_t)b;
   }
+}
+
+__global__ void k_scatter(Dev d, Geo g, int guard) {
+  if (guard_off(d.ctl, guard)) return;
+  scatter_body(d, g, blockIdx.x, gridDim.x);
 }
 
 // Each cell column sorted by (cell key, bond index): one wavefront per
@@ -299,14 +334,14 @@ __device__ __forceinline__ int64_t kb_pair(const Dev &d, int32_t b) {
   return ((int64_t)d.keys[b] << 32) | (uint32_t)b;
 }
 
-__global__ __launch_bounds__(64) void k_colsort(Dev d, Geo g, int guard) {
-  if (guard_off(d.ctl, guard)) return;
+__device__ __forceinline__ void colsort_body(const Dev &d, const Geo &g, int64_t bid,
+                                             int64_t nblk) {
   __shared__ int64_t ks[kColStage], kr[kColStage];
   __shared__ uint32_t occ[64];  // occ_words <= 41 (n1d^3 < 2^31)
   const int lane = threadIdx.x;
   const int n = g.n1d, W = g.occ_words;
   const int64_t ncol = (int64_t)n * n;
-  for (int64_t c = blockIdx.x; c < ncol; c += gridDim.x) {
+  for (int64_t c = bid; c < ncol; c += nblk) {
     const int32_t s = d.col_start[c], e = d.col_start[c + 1], m = e - s;
     const bool staged = m <= kColStage;
     if (lane < W) occ[lane] = 0u;
@@ -345,6 +380,11 @@ __global__ __launch_bounds__(64) void k_colsort(Dev d, Geo g, int guard) {
     if (lane < W) d.cell_occ[c * W + lane] = occ[lane];
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(64) void k_colsort(Dev d, Geo g, int guard) {
+  if (guard_off(d.ctl, guard)) return;
+  colsort_body(d, g, blockIdx.x, gridDim.x);
 }
 
 // Coordinates of the q-th cell of the half-shell stencil of (x, y, z) in
@@ -485,11 +525,14 @@ __device__ __forceinline__ void deg_one(const Dev &d, const Geo &g, int64_t p) {
 
 // Grid-stride over the sorted positions: a capped grid keeps the launch
 // cheap on the steps that do not rebuild (the guard returns at once).
+__device__ __forceinline__ void deg_body(const Dev &d, const Geo &g, int64_t bid, int64_t nblk) {
+  const int64_t stride = nblk * blockDim.x;
+  for (int64_t p = bid * blockDim.x + threadIdx.x; p < g.nb; p += stride) deg_one(d, g, p);
+}
+
 __global__ __launch_bounds__(256) void k_deg(Dev d, Geo g) {
   if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.nb; p += stride)
-    deg_one(d, g, p);
+  deg_body(d, g, blockIdx.x, gridDim.x);
 }
 
 // Incidence lists in candidate order, one thread per bond, no atomics and
@@ -586,10 +629,9 @@ __device__ __forceinline__ void pull_one(const Dev &d, const Geo &g, int32_t b,
 
 constexpr int kPullStage = 512;
 
-__global__ __launch_bounds__(256) void k_pull(Dev d, Geo g) {
-  if (guard_off(d.ctl, kGuardRebuild)) return;
-  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+__device__ __forceinline__ void pull_body(const Dev &d, const Geo &g, int64_t bid, int64_t nblk) {
+  const int64_t i0 = bid * blockDim.x + threadIdx.x;
+  const int64_t stride = nblk * blockDim.x;
   const int64_t total = d.inc_start[g.nb];  // two incidences per candidate
   const bool fits = total >= 0 && total <= 2 * g.cap;
   if (i0 == 0) {
@@ -613,7 +655,7 @@ __global__ __launch_bounds__(256) void k_pull(Dev d, Geo g) {
   __shared__ int2 stage[4][kPullStage];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t ngrp = (g.nb + 255) / 256;
-  for (int64_t grp = blockIdx.x; grp < ngrp; grp += gridDim.x) {
+  for (int64_t grp = bid; grp < ngrp; grp += nblk) {
     const int64_t b0 = grp * 256 + wv * 64, b = b0 + lane;
     int32_t base = 0, cnt = 0;
     if (b0 < g.nb) {
@@ -627,6 +669,11 @@ __global__ __launch_bounds__(256) void k_pull(Dev d, Geo g) {
       for (int i = lane; i < cnt; i += 64) d.inc[base + i] = stage[wv][i];
     __syncthreads();
   }
+}
+
+__global__ __launch_bounds__(256) void k_pull(Dev d, Geo g) {
+  if (guard_off(d.ctl, kGuardRebuild)) return;
+  pull_body(d, g, blockIdx.x, gridDim.x);
 }
 
 static inline unsigned nblocks(int64_t n, int bs) {
@@ -661,6 +708,125 @@ void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
   hipLaunchKernelGGL(k_pull, dim3(std::min(nblocks(g.ns > g.nb ? g.ns : g.nb, 256), kListBlocks)),
                      dim3(256), 0, st, d, g);
+}
+
+// Ensemble batches: the launches of launch_rebuild, each over all members
+// (odd-step pointers: a rebuild only ever starts an odd step); a member
+// that does not rebuild returns at its guard.  Per member, the same grid as
+// its own launch.
+__global__ void k_group_keys(const Member *__restrict__ ms, GroupGrid gg) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  if (guard_off(M.d[0].ctl, kGuardRebuild)) return;
+  keys_body(M.d[0], M.g, blockIdx.x - b0, nblk);
+}
+
+__global__ void k_group_scatter(const Member *__restrict__ ms, GroupGrid gg) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  if (guard_off(M.d[0].ctl, kGuardRebuild)) return;
+  scatter_body(M.d[0], M.g, blockIdx.x - b0, nblk);
+}
+
+__global__ __launch_bounds__(64) void k_group_colsort(const Member *__restrict__ ms,
+                                                      GroupGrid gg) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  if (guard_off(M.d[0].ctl, kGuardRebuild)) return;
+  colsort_body(M.d[0], M.g, blockIdx.x - b0, nblk);
+}
+
+__global__ __launch_bounds__(256) void k_group_deg(const Member *__restrict__ ms, GroupGrid gg) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  if (guard_off(M.d[0].ctl, kGuardRebuild)) return;
+  deg_body(M.d[0], M.g, blockIdx.x - b0, nblk);
+}
+
+__global__ __launch_bounds__(256) void k_group_pull(const Member *__restrict__ ms, GroupGrid gg) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  if (guard_off(M.d[0].ctl, kGuardRebuild)) return;
+  pull_body(M.d[0], M.g, blockIdx.x - b0, nblk);
+}
+
+// which 0: the column populations into column starts; 1: the degrees into
+// the incidence starts
+__global__ __launch_bounds__(kScanBlock) void k_group_scan_reduce(const Member *__restrict__ ms,
+                                                                  GroupGrid gg, int which) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  const Dev &d = M.d[0];
+  if (guard_off(d.ctl, kGuardRebuild)) return;
+  const int64_t n = which ? M.g.nb : (int64_t)M.g.n1d * M.g.n1d;
+  scan_reduce_body(which ? d.deg : d.col_count, n, d.scan_tmp, blockIdx.x - b0, nblk);
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_group_scan_down(const Member *__restrict__ ms,
+                                                                GroupGrid gg, int which) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  const Dev &d = M.d[0];
+  if (guard_off(d.ctl, kGuardRebuild)) return;
+  const int64_t n = which ? M.g.nb : (int64_t)M.g.n1d * M.g.n1d;
+  scan_down_body(which ? d.deg : d.col_count, n, d.scan_tmp, which ? d.inc_start : d.col_start,
+                 blockIdx.x - b0, nblk);
+}
+
+namespace {
+// offsets of a grouped launch from each member's block count; the total
+template <typename F>
+int32_t group_grid(GroupGrid &gg, const Member *hm, int n, F blocks) {
+  int32_t t = 0;
+  for (int m = 0; m < n; ++m) {
+    gg.off[m] = t;
+    t += (int32_t)blocks(hm[m].g);
+  }
+  for (int m = n; m <= kMaxGroup; ++m) gg.off[m] = t;
+  return t;
+}
+
+unsigned scan_blocks(int64_t n) {
+  int64_t nblk = (n + kScanTile - 1) / kScanTile;
+  if (nblk == 0) nblk = 1;
+  return (unsigned)(nblk > kStreamBlocks ? kStreamBlocks : nblk);
+}
+}  // namespace
+
+void launch_group_rebuild(const Member *dm, const Member *hm, int n, hipStream_t st) {
+  if (n <= 0) return;
+  GroupGrid gk, gcs, gsrt, gdeg, gds, gpl;
+  const int32_t tk = group_grid(gk, hm, n, [](const Geo &g) { return sgrid(g.nb); });
+  const int32_t tcs = group_grid(gcs, hm, n, [](const Geo &g) {
+    return scan_blocks((int64_t)g.n1d * g.n1d);
+  });
+  const int32_t tsrt = group_grid(gsrt, hm, n, [](const Geo &g) {
+    return (unsigned)std::min<int64_t>((int64_t)g.n1d * g.n1d, 8192);
+  });
+  const int32_t tdeg = group_grid(gdeg, hm, n, [](const Geo &g) {
+    return std::min(nblocks(g.nb, 256), kListBlocks);
+  });
+  const int32_t tds = group_grid(gds, hm, n, [](const Geo &g) { return scan_blocks(g.nb); });
+  const int32_t tpl = group_grid(gpl, hm, n, [](const Geo &g) {
+    return std::min(nblocks(g.ns > g.nb ? g.ns : g.nb, 256), kListBlocks);
+  });
+  hipLaunchKernelGGL(k_group_keys, dim3(tk), dim3(kScanBlock), 0, st, dm, gk);
+  hipLaunchKernelGGL(k_group_scan_reduce, dim3(tcs), dim3(kScanBlock), 0, st, dm, gcs, 0);
+  hipLaunchKernelGGL(k_group_scan_down, dim3(tcs), dim3(kScanBlock), 0, st, dm, gcs, 0);
+  hipLaunchKernelGGL(k_group_scatter, dim3(tk), dim3(kScanBlock), 0, st, dm, gk);
+  hipLaunchKernelGGL(k_group_colsort, dim3(tsrt), dim3(64), 0, st, dm, gsrt);
+  hipLaunchKernelGGL(k_group_deg, dim3(tdeg), dim3(256), 0, st, dm, gdeg);
+  hipLaunchKernelGGL(k_group_scan_reduce, dim3(tds), dim3(kScanBlock), 0, st, dm, gds, 1);
+  hipLaunchKernelGGL(k_group_scan_down, dim3(tds), dim3(kScanBlock), 0, st, dm, gds, 1);
+  hipLaunchKernelGGL(k_group_pull, dim3(tpl), dim3(256), 0, st, dm, gpl);
+}
+
+void launch_group_commit(const Member *dm, int n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_group_commit, dim3(n), dim3(64), 0, st, dm);
+}
+
+void launch_group_ctl(const Member *dm, int n, Ctl *out, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_group_ctl, dim3(1), dim3(64), 0, st, dm, n, out);
 }
 
 // The cell list of the last rebuild (sorted, runs, occupancy) from the

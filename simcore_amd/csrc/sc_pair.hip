@@ -403,8 +403,7 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd, int64
 constexpr int kChunk = 128;
 
 template <bool MIXED, bool THERMO>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_bal(Dev d, Geo g, int xcd,
-                                                                                           int64_t blk0) {
+__device__ __forceinline__ void pair_bal_body(const Dev &d, const Geo &g, int64_t blk) {
   using FT = typename std::conditional<MIXED, float, double>::type;
   __shared__ BondRec ownr[64];
   __shared__ FT cF[kChunk][3];
@@ -413,7 +412,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
   if (d.ctl->abort) return;
   const int64_t nb = g.nb;
   const int lane = threadIdx.x;
-  const int64_t b0 = (blk0 + logical_block(xcd)) * 64;
+  const int64_t b0 = blk * 64;
   const int64_t b = b0 + lane;
   const bool valid = b < nb;
   qs[lane] = d.inc_start[valid ? b : nb];
@@ -470,6 +469,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
   }
 }
 
+template <bool MIXED, bool THERMO>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_bal(Dev d, Geo g, int xcd,
+                                                                                           int64_t blk0) {
+  pair_bal_body<MIXED, THERMO>(d, g, blk0 + logical_block(xcd));
+}
+
+// Ensemble batches: the balanced kernel over every member's bonds (mixed
+// precision, no stress sums); `par` 0 on odd steps, 1 on even ones.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_group_pair(
+    const Member *__restrict__ ms, GroupGrid gg, int par) {
+  int b0, nblk;
+  const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
+  pair_bal_body<true, false>(M.d[par], M.g, blockIdx.x - b0);
+}
+
 int64_t pair_blocks(const Geo &g, const Tune &t) {
   const int bs = t.pair_bal ? 64 : t.pair_block;
   const int64_t blocks = (g.nb + bs - 1) / bs;
@@ -507,6 +521,18 @@ void launch_pair_range(const Dev &d, const Geo &g, int precision, const Tune &t,
   else
     SC_PAIR(k_pair_fused, t.pair_block);
 #undef SC_PAIR
+}
+
+void launch_group_pair(const Member *dm, const Member *hm, int n, bool odd, hipStream_t st) {
+  GroupGrid gg;
+  int32_t t = 0;
+  for (int m = 0; m < n; ++m) {
+    gg.off[m] = t;
+    t += (int32_t)((hm[m].g.nb + 63) / 64);
+  }
+  for (int m = n; m <= kMaxGroup; ++m) gg.off[m] = t;
+  if (t <= 0) return;
+  hipLaunchKernelGGL(k_group_pair, dim3((unsigned)t), dim3(64), 0, st, dm, gg, odd ? 0 : 1);
 }
 
 }  // namespace sc

@@ -45,6 +45,22 @@ class Arena {
 int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
                   sc_handle *out);
 
+// Ensemble batches (grouped launches, sc_internal.h).  A system can join a
+// batch when it is loaded, idle, at an even step count (batches take an odd
+// and an even step in turn), in mixed precision with the packed integrator
+// covering every filament (none above 128 sites), and without crosslinks,
+// thermodynamic sums, a caller noise buffer or profiling.
+struct Member;
+struct Ctl;
+bool system_batchable(sc_handle h);
+// The system's entry in a batch's member table.
+void system_member(sc_handle h, Member *m);
+// After a batch took the system through step `target`: its control block
+// as the batch left it.  Same outcomes as sc_step_poll: SC_OK with *done
+// set, a capacity overflow re-run on the system's own stream (*done 0:
+// poll it), or the step's error.
+int system_batch_done(sc_handle h, const Ctl &c, int64_t target, int *done);
+
 // Device bytes sc_load_filaments allocates for a system of nf filaments and
 // ns sites under these parameters (before any candidate-capacity growth).
 size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns,

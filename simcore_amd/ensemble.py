@@ -3,8 +3,9 @@ a thin mirror of the sc_ensemble_* C ABI.
 
 The reference runs the n_runs of a parameter file one after another
 (src/simulation_manager.cpp:461-545); here requests are queued, admitted by
-device-memory footprint while slots are free, advanced concurrently on
-their own streams, and retired as they finish.
+device-memory footprint while slots are free, advanced in chunks -- up to
+`batch_size` mixed-precision systems per set of grouped launches, the rest
+on their own streams -- and retired as they finish.
 """
 import ctypes
 from ctypes import byref, c_int32, c_int64, c_void_p
@@ -19,8 +20,9 @@ STATUS = {0: "queued", 1: "running", 2: "done"}
 
 
 class Ensemble:
-    def __init__(self, hbm_budget=0, max_active=16, steps_per_launch=64):
-        prm = ScEnsembleParams(int(hbm_budget), int(max_active), int(steps_per_launch))
+    def __init__(self, hbm_budget=0, max_active=16, steps_per_launch=64, batch_size=0):
+        prm = ScEnsembleParams(int(hbm_budget), int(max_active), int(steps_per_launch),
+                               int(batch_size))
         self.h = c_void_p()
         rc = lib.sc_ensemble_create(byref(prm), byref(self.h))
         if rc != SC_OK:

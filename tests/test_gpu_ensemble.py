@@ -146,3 +146,73 @@ def test_ensemble_admits_while_others_run():
             np.testing.assert_array_equal(ens.result(rid), ref)
         np.testing.assert_array_equal(ens.result(first), ref0)
         assert ens.stats()["peak_active"] >= 2
+
+
+def _batchable():
+    """Mixed-precision systems of different sizes and seeds with filaments of
+    at most 128 sites (every length class of the packed integrator), odd and
+    even step counts."""
+    out = []
+    for k, (num, n) in enumerate([(16, 64), (32, 50), (48, 41), (64, 80), (24, 33), (40, 72)]):
+        p, f = c0_system(num=num, seed=50 + k, noise_seed=600 + k)
+        out.append((p, f, n))
+    p, f = uneven_system(sites=[3, 4, 7, 8, 9, 15, 16, 17, 31, 32, 33, 47, 63, 64, 65, 100,
+                                127, 128], seed=33, noise_seed=707)
+    out.append((p, f, 57))
+    return out
+
+
+@pytest.mark.parametrize("batch_size", [1, 3, 8])
+def test_ensemble_batches_match_alone(batch_size):
+    """Systems stepped in batches -- one set of grouped launches per step for
+    several systems, members joining and leaving at chunk boundaries, the
+    odd remainders on their own streams -- end with the positions of each
+    system stepped alone, bit for bit; batch_size 1 steps every system on
+    its own stream."""
+    cases = _batchable()
+    print(f"case: {len(cases)} systems, batch_size {batch_size}, max_active 5", flush=True)
+    refs = [_alone(p, f, n) for p, f, n in cases]
+    with sc.Ensemble(hbm_budget=2 << 30, max_active=5, steps_per_launch=16,
+                     batch_size=batch_size) as ens:
+        ids = [ens.submit(p, f, n, precision="mixed") for p, f, n in cases]
+        ens.drain(timeout_s=120)
+        st = ens.stats()
+        print(st)
+        for rid, ref in zip(ids, refs):
+            assert ens.request(rid)["state"] == "done"
+            np.testing.assert_array_equal(ens.result(rid), ref)
+        total = sum(n for _, _, n in cases)
+        assert st["completed"] == len(cases) and st["failed"] == 0
+        assert st["batched_steps"] + st["solo_steps"] == total
+        if batch_size == 1:
+            assert st["batched_steps"] == 0
+        else:
+            assert st["batched_steps"] >= total - 2 * len(cases)
+
+
+def test_ensemble_batch_with_rebuilds_and_spatial_order():
+    """Spatially ordered mixed systems whose candidate lists are rebuilt
+    inside batched chunks (the grouped rebuild launches), among a system that
+    cannot join a batch (filaments above 128 sites): results equal each
+    system alone, rebuild counts included."""
+    print("case: spatial order, rebuilds inside batches, one long system solo", flush=True)
+    cases = [(p, f, 120) for p, f, _ in _batchable()[:4]]
+    p, f = long_system(seed=41, noise_seed=404)
+    cases.append((p, f, 40))
+    refs, rebuilds = [], []
+    for p, f, n in cases:
+        with sc.FilamentSystem(p, f, precision="mixed", use_graph=True, spatial_order=True) as s:
+            s.step(n)
+            refs.append(s.sites()[0])
+            rebuilds.append(s.counters()["n_rebuilds"])
+    assert min(rebuilds[:4]) >= 3
+    with sc.Ensemble(hbm_budget=2 << 30, max_active=8, steps_per_launch=32,
+                     batch_size=8) as ens:
+        ids = [ens.submit(p, f, n, precision="mixed", spatial_order=True) for p, f, n in cases]
+        ens.drain(timeout_s=120)
+        st = ens.stats()
+        print(st)
+        for rid, ref in zip(ids, refs):
+            np.testing.assert_array_equal(ens.result(rid), ref)
+        assert st["batched_steps"] == 4 * 120
+        assert st["solo_steps"] == 40
