@@ -93,10 +93,12 @@ def serve_trace(args, sc, reqs):
     """Replays one arrival trace through a fresh ensemble runner; returns its
     stats."""
     ens = sc.Ensemble(hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
-                      max_active=args.max_active, steps_per_launch=args.steps_per_launch)
+                      max_active=args.max_active, steps_per_launch=args.steps_per_launch,
+                      batch_size=args.batch_size)
     t0 = ens.clock()
     k = 0
     last_log = time.perf_counter()
+    host = 0.0  # seconds spent inside the runner's scheduling rounds
     while True:
         now = ens.clock() - t0
         while k < len(reqs) and reqs[k][0] <= now:
@@ -104,7 +106,9 @@ def serve_trace(args, sc, reqs):
                        t_submit=t0 + reqs[k][0],
                        spatial_order=args.precision == "mixed" and not args.caller_order)
             k += 1
+        th = time.perf_counter()
         nr, nq = ens.progress()
+        host += time.perf_counter() - th
         if k == len(reqs) and nr == 0 and nq == 0:
             break
         if time.perf_counter() - last_log > 5.0:
@@ -112,6 +116,7 @@ def serve_trace(args, sc, reqs):
             log(f"[bench] t={now:.1f}s submitted {k} running {nr} queued {nq}")
         time.sleep(50e-6)
     st = ens.stats()
+    st["host_s"] = host
     ens.close()
     return st
 
@@ -127,7 +132,8 @@ def run_sweep(args, sc):
     base = serving_requests(sc, args.requests, 1.0)
     nbonds = np.array([f.nb for _, _, f in base])
     warm = sc.Ensemble(hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
-                       max_active=args.max_active, steps_per_launch=args.steps_per_launch)
+                       max_active=args.max_active, steps_per_launch=args.steps_per_launch,
+                      batch_size=args.batch_size)
     warm.submit(base[0][1], base[0][2], 20, precision=args.precision)
     warm.drain(120)
     warm.close()
@@ -140,6 +146,11 @@ def run_sweep(args, sc):
         pt = {"rate_per_s": rate, "value": value, "latency_p50_s": st["latency_p50"],
               "latency_p99_s": st["latency_p99"], "completed": st["completed"],
               "failed": st["failed"], "peak_active": st["peak_active"],
+              "batched_steps": st["batched_steps"], "solo_steps": st["solo_steps"],
+              "host_busy": st["host_s"] / st["span"] if st["span"] > 0 else None,
+              "host_admit_s": st["host_admit_s"], "host_chunk_s": st["host_chunk_s"],
+              "host_retire_s": st["host_retire_s"], "host_capture_s": st["host_capture_s"],
+              "graph_captures": st["graph_captures"],
               "offered": float(nbonds.mean()) * args.request_steps * rate}
         points.append(pt)
         log(f"[bench] sweep {json.dumps(pt)}")
@@ -162,6 +173,7 @@ def run_sweep(args, sc):
                                "doubling rates from %g/s" % (args.requests, args.request_steps,
                                                              args.rate),
                    "max_active": args.max_active, "steps_per_launch": args.steps_per_launch,
+                   "batch_size": args.batch_size,
                    "hbm_budget_gb": args.hbm_budget_gb, "precision": args.precision},
         "saturation": {"value": best["value"], "rate_per_s": best["rate_per_s"],
                        "stopped_at_rate_per_s": points[-1]["rate_per_s"], "reason": stop},
@@ -180,13 +192,15 @@ def run_serving(args, sc):
     log(f"[bench] bonds per request: mean {nbonds.mean():.0f}, min {nbonds.min()}, "
         f"max {nbonds.max()}")
     ens = sc.Ensemble(hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
-                      max_active=args.max_active, steps_per_launch=args.steps_per_launch)
+                      max_active=args.max_active, steps_per_launch=args.steps_per_launch,
+                      batch_size=args.batch_size)
     # warm the code paths (graph capture, first launches) outside the trace
     w = ens.submit(reqs[0][1], reqs[0][2], 20, precision=args.precision)
     ens.drain(120)
     ens.close()
     ens = sc.Ensemble(hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
-                      max_active=args.max_active, steps_per_launch=args.steps_per_launch)
+                      max_active=args.max_active, steps_per_launch=args.steps_per_launch,
+                      batch_size=args.batch_size)
     t0 = ens.clock()
     k = 0
     last_log = time.perf_counter()
@@ -220,6 +234,7 @@ def run_serving(args, sc):
                                "0.05), %d steps each, lp U[10,1000] sigma, dt 1e-5, per-request "
                                "seeds" % (args.rate, args.request_steps),
                    "requests": args.requests, "max_active": args.max_active,
+                   "batch_size": args.batch_size,
                    "steps_per_launch": args.steps_per_launch,
                    "hbm_budget_gb": args.hbm_budget_gb, "precision": args.precision},
         "latency_p50_s": st["latency_p50"], "latency_p99_s": st["latency_p99"],
@@ -382,7 +397,10 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="c4: arrival rates doubling from --rate until saturation")
     ap.add_argument("--request-steps", type=int, default=2000)
-    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--max-active", type=int, default=64)
+    ap.add_argument("--batch-size", type=int, default=16,
+                    help="c4: systems per batch of grouped launches (0: the runner's "
+                         "default, 1: no batches)")
     ap.add_argument("--steps-per-launch", type=int, default=64)
     ap.add_argument("--hbm-budget-gb", type=float, default=64.0)
     # measurement helpers

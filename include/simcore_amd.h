@@ -401,6 +401,11 @@ typedef struct sc_ensemble_stats {
   size_t arena_bytes;
   int64_t batched_steps; /* system-steps taken in batches                  */
   int64_t solo_steps;    /* system-steps taken on the systems' own streams */
+  /* host seconds spent in sc_ensemble_progress admitting systems, setting
+   * up and enqueuing chunks, retiring finished systems */
+  double host_admit_s, host_chunk_s, host_retire_s;
+  double host_capture_s;  /* of host_chunk_s: capturing batch graphs      */
+  int64_t graph_captures; /* batch graphs captured (membership changes)   */
 } sc_ensemble_stats;
 
 int sc_ensemble_create(const sc_ensemble_params *params, sc_ensemble *out);

@@ -90,7 +90,9 @@ class ScEnsembleStats(ctypes.Structure):
                 ("segment_steps", c_double), ("span", c_double),
                 ("peak_active", c_int32), ("peak_bytes", c_size_t),
                 ("arena_bytes", c_size_t), ("batched_steps", c_int64),
-                ("solo_steps", c_int64)]
+                ("solo_steps", c_int64), ("host_admit_s", c_double),
+                ("host_chunk_s", c_double), ("host_retire_s", c_double),
+                ("host_capture_s", c_double), ("graph_captures", c_int64)]
 
 
 class ScCrosslinkParams(ctypes.Structure):

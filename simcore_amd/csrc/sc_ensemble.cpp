@@ -137,6 +137,11 @@ struct Batch {
   hipEvent_t ev = nullptr;
   std::vector<int64_t> ids;
   bool busy = false;
+  // an odd and an even step of the members, replayed over a chunk; the
+  // launch grids depend only on the members' sizes (key), the members'
+  // buffers are read from the device table at run time
+  hipGraphExec_t gx = nullptr;
+  std::vector<int64_t> gkey;
 };
 
 int count_bins(const std::vector<int32_t> &so) {
@@ -171,6 +176,9 @@ struct sc_ensemble_s {
   int32_t peak_active = 0;
   size_t peak_bytes = 0;
   int64_t batched_steps = 0, solo_steps = 0;
+  double host_admit = 0, host_chunk = 0, host_retire = 0;  // seconds in each
+  double host_capture = 0;  // of host_chunk: batch graph captures
+  int64_t captures = 0;
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -223,7 +231,19 @@ int enqueue_batch(sc_ensemble e, Batch &B) {
   if (hipMemcpyAsync(B.dm, B.hm, sizeof(sc::Member) * n, hipMemcpyHostToDevice, B.st) !=
       hipSuccess)
     return SC_ERR_HIP;
-  for (int64_t s = 0; s < k; s += 2) {
+  std::vector<int64_t> key;
+  for (int j = 0; j < n; ++j) {
+    const sc::Geo &g = B.hm[j].g;
+    key.insert(key.end(), {g.nb, g.ns, (int64_t)g.n1d, (int64_t)B.hm[j].pack_waves});
+  }
+  if (!B.gx || key != B.gkey) {
+    const double tcap = e->now();
+    ++e->captures;
+    if (B.gx) (void)hipGraphExecDestroy(B.gx);
+    B.gx = nullptr;
+    B.gkey.clear();
+    if (hipStreamBeginCapture(B.st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return SC_ERR_HIP;
     // odd step (the rebuild stage, then the midstep), even step
     sc::launch_group_rebuild(B.dm, B.hm, n, B.st);
     sc::launch_group_pair(B.dm, B.hm, n, true, B.st);
@@ -232,7 +252,19 @@ int enqueue_batch(sc_ensemble e, Batch &B) {
     sc::launch_group_pair(B.dm, B.hm, n, false, B.st);
     sc::launch_group_integrate(B.dm, B.hm, n, false, B.st);
     sc::launch_group_commit(B.dm, n, B.st);
+    hipGraph_t gr = nullptr;
+    if (hipStreamEndCapture(B.st, &gr) != hipSuccess) return SC_ERR_HIP;
+    const hipError_t ie = hipGraphInstantiate(&B.gx, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ie != hipSuccess) {
+      B.gx = nullptr;
+      return SC_ERR_HIP;
+    }
+    B.gkey.swap(key);
+    e->host_capture += e->now() - tcap;
   }
+  for (int64_t s = 0; s < k; s += 2)
+    if (hipGraphLaunch(B.gx, B.st) != hipSuccess) return SC_ERR_HIP;
   sc::launch_group_ctl(B.dm, n, B.dctl, B.st);
   if (hipMemcpyAsync(B.hctl, B.dctl, sizeof(sc::Ctl) * n, hipMemcpyDeviceToHost, B.st) !=
           hipSuccess ||
@@ -416,10 +448,12 @@ int sc_ensemble_progress(sc_ensemble e, int32_t *n_running, int32_t *n_queued) {
   }
   // 3. finished systems leave (before admission, which may use their room)
   const std::vector<int64_t> idle_scan = e->running;
+  const double tr = e->now();
   for (int64_t id : idle_scan) {
     Request &R = *e->reqs[id];
     if (!R.solo && R.batch < 0) finish_if_done(e, R);
   }
+  e->host_retire += e->now() - tr;
   // 4. admission, first fit in arrival order
   for (auto it = e->queue.begin(); it != e->queue.end() && !e->free_slots.empty();) {
     Request &R = *e->reqs[*it];
@@ -428,7 +462,9 @@ int sc_ensemble_progress(sc_ensemble e, int32_t *n_running, int32_t *n_queued) {
       continue;
     }
     const int slot = e->free_slots.back();
+    const double ta = e->now();
     const int rc = admit(e, R, slot);
+    e->host_admit += e->now() - ta;
     if (rc == SC_ERR_NOMEM) {
       ++it;
       continue;
@@ -450,6 +486,7 @@ int sc_ensemble_progress(sc_ensemble e, int32_t *n_running, int32_t *n_queued) {
   }
   // 5. idle systems take their next chunk, in a free batch when they
   //    qualify, else on their own stream (a request of 0 steps finishes)
+  const double tc = e->now();
   const std::vector<int64_t> idle_now = e->running;
   std::vector<int64_t> ready;
   for (int64_t id : idle_now) {
@@ -496,6 +533,7 @@ int sc_ensemble_progress(sc_ensemble e, int32_t *n_running, int32_t *n_queued) {
       }
     }
   }
+  e->host_chunk += e->now() - tc;
   if (n_running) *n_running = (int32_t)e->running.size();
   if (n_queued) *n_queued = (int32_t)e->queue.size();
   return SC_OK;
@@ -593,6 +631,11 @@ int sc_ensemble_stats_get(sc_ensemble e, sc_ensemble_stats *out) {
   out->arena_bytes = e->arena.capacity();
   out->batched_steps = e->batched_steps;
   out->solo_steps = e->solo_steps;
+  out->host_admit_s = e->host_admit;
+  out->host_chunk_s = e->host_chunk;
+  out->host_retire_s = e->host_retire;
+  out->host_capture_s = e->host_capture;
+  out->graph_captures = e->captures;
   return SC_OK;
 }
 
@@ -606,6 +649,7 @@ int sc_ensemble_destroy(sc_ensemble e) {
   if (!e) return SC_ERR_ARG;
   for (Batch &B : e->batches) {
     if (B.st) (void)hipStreamSynchronize(B.st);
+    if (B.gx) (void)hipGraphExecDestroy(B.gx);
     if (B.dm) (void)hipFree(B.dm);
     if (B.dctl) (void)hipFree(B.dctl);
     if (B.hm) (void)hipHostFree(B.hm);
