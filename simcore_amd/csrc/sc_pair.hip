@@ -44,6 +44,7 @@ template <bool MIXED>
 __device__ __forceinline__ BondRec load_rec(const double *br, const Geo &g, int64_t b) {
   BondRec r;
   if (MIXED) {
+#pragma clang fp contract(fast)
     const uint4 *rec = reinterpret_cast<const uint4 *>(br + 4 * b);
     const uint4 a = rec[0], v = rec[1];
     const int packed = (int)v.z;
@@ -64,6 +65,7 @@ __device__ __forceinline__ BondRec load_rec(const double *br, const Geo &g, int6
 // box wraps), times the cell length.
 __device__ __forceinline__ void sep_cells(const Geo &g, const double *p1, const double *p2,
                                           double *sep) {
+#pragma clang fp contract(fast)
   const double n = (double)g.n1d, half = 0.5 * (double)g.n1d;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -81,10 +83,13 @@ __device__ __forceinline__ void sep_cells(const Geo &g, const double *p1, const 
 // (|lambda| <= l/2).  On CDNA4 an FP64 FMA issues at the rate of an
 // unpacked FP32 one, so keeping the geometry in FP64 costs nothing and
 // needs no separate near-parallel path; only the divisions and square roots
-// (many instructions in IEEE FP64) are replaced.
+// (many instructions in IEEE FP64) are replaced.  Multiply-adds are fused
+// (one rounding each); both owners of a pair still run the same
+// instructions on the same inputs.
 __device__ __forceinline__ void sphero_mixed(const double *dr, const double *u1, double hl1,
                                              const double *u2, double hl2, double *rmin,
                                              double *rm2, double *lam_o, double *mu_o) {
+#pragma clang fp contract(fast)
   double a = 0.0, b = 0.0, c = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -150,6 +155,7 @@ __device__ __forceinline__ void sphero_mixed(const double *dr, const double *u1,
 // one FP64 Newton step (relative error ~1e-14), rounded to FP32 for the
 // slot.  dr^2 >= 1e-12 here.
 __device__ __forceinline__ float4 wca_mixed(const Geo &g, const double *rmin, double rm2) {
+#pragma clang fp contract(fast)
   double rinv = (double)__builtin_amdgcn_rsqf((float)rm2);
   rinv = rinv * (1.5 - 0.5 * rm2 * rinv * rinv);
   const double rinv2 = rinv * rinv;
@@ -274,12 +280,21 @@ __device__ __forceinline__ void pair_slot(const Dev &d, const Geo &g, const Bond
 // The owner's share of a pair (src/interaction_manager.cpp:469-470,
 // 500-517; src/minimum_distance.cpp:339-342): +F and contact1 x F for obj1,
 // -F and -(contact2 x F) for obj2.  With f the signed force (-F for obj2)
-// the torque is contact x f, the same bits as -(contact x F).
+// the torque is contact x f, the same bits as -(contact x F).  The mixed
+// path fuses the cross product's multiply-adds.
+template <bool MIXED>
 __device__ __forceinline__ void pair_share(const double *u, double sc, const double *f,
                                            double *t) {
   double con[3];
   for (int k = 0; k < 3; ++k) con[k] = sc * u[k];
-  cross3(con, f, t);
+  if (MIXED) {
+#pragma clang fp contract(fast)
+    t[0] = con[1] * f[2] - con[2] * f[1];
+    t[1] = con[2] * f[0] - con[0] * f[2];
+    t[2] = con[0] * f[1] - con[1] * f[0];
+  } else {
+    cross3(con, f, t);
+  }
 }
 
 // Per-bond sums, accumulated in FP64; FP32 storage on the mixed path (one
@@ -374,7 +389,7 @@ __global__ __launch_bounds__(512) void k_pair_fused(Dev d, Geo g, int xcd, int64
       pair_slot<MIXED>(d, g, own, q, ent, oth, fc, &sc, &rs);
       const double sg = (ent.y & 1) ? -1.0 : 1.0;
       for (int k = 0; k < 3; ++k) cf[k] = sg * fc[k];
-      pair_share(own.u, sc, cf, ct);
+      pair_share<MIXED>(own.u, sc, cf, ct);
       for (int k = 0; k < 3; ++k) F[k] += cf[k];
       for (int k = 0; k < 3; ++k) T[k] += ct[k];
       if (THERMO) {
@@ -443,7 +458,7 @@ __device__ __forceinline__ void pair_bal_body(const Dev &d, const Geo &g, int64_
       pair_slot<MIXED>(d, g, own, q, ent, oth, fc, &sc, &rs);
       const double sg = (ent.y & 1) ? -1.0 : 1.0;
       for (int k = 0; k < 3; ++k) cf[k] = sg * fc[k];
-      pair_share(own.u, sc, cf, ct);
+      pair_share<MIXED>(own.u, sc, cf, ct);
       for (int k = 0; k < 3; ++k) {
         cF[q - cs][k] = (FT)cf[k];
         cT[q - cs][k] = ct[k];
