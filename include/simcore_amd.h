@@ -244,8 +244,11 @@ int sc_checkpoint_load(sc_handle h, const void *buf, size_t size);
  *                       length bin; 2 (mixed-precision default): two sites
  *                       per lane, every filament of up to 128 sites in a
  *                       lane segment of its own width, packed into shared
- *                       wavefronts, one launch (same results as 1); 3: the
- *                       same, one launch per lane-group width (16/32/64)
+ *                       wavefronts, one launch that also carries the
+ *                       filaments of 129-256 sites in its first waves
+ *                       (same results as 1); 3: the packed filaments in
+ *                       one launch per lane-group width (16/32/64), the
+ *                       longer ones in a launch of their own
  *   integrate_streams   0-3: the integrator's length bins are spread over the
  *                       system's stream and this many side streams (forked
  *                       and joined inside the step) and run concurrently

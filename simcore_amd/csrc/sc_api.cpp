@@ -472,20 +472,30 @@ void enqueue_step(sc_system *h, bool odd) {
   sc::BinDesc bins[sc::kMaxBins + 1];
   int nbins = 0;
   const bool packed = h->tune.int_layout >= 2 && h->pack_waves > 0;
-  if (packed && h->tune.int_layout == 2)
-    bins[nbins++] = {h->pack_tab, h->pack_waves, 0, 0};  // one launch, all widths
-  else if (packed)
+  // layout 2: one launch for every filament, the bin of 129-256 sites first
+  const Bin *merged = nullptr;
+  if (packed && h->tune.int_layout == 2) {
+    for (const Bin &b : h->bins)
+      if (b.maxn == 256) merged = &b;
+    bins[nbins++] = {h->pack_tab, h->pack_waves, 0, 0};
+  } else if (packed) {
     for (int c = 0; c < 3; ++c)
       if (h->pack_cs[c + 1] > h->pack_cs[c])
         bins[nbins++] = {h->pack_tab + (int64_t)64 * h->pack_cs[c],
                          h->pack_cs[c + 1] - h->pack_cs[c], 0, 16 << c};
+  }
   for (const Bin &b : h->bins)
-    if (!packed || b.maxn > 128) bins[nbins++] = {b.list, b.count, b.maxn, 0};
+    if ((!packed || b.maxn > 128) && &b != merged) bins[nbins++] = {b.list, b.count, b.maxn, 0};
   auto load_of = [&](int k) -> int64_t {
-    return bins[k].maxn ? (int64_t)bins[k].count * bins[k].maxn : (int64_t)bins[k].count * 120;
+    if (bins[k].maxn) return (int64_t)bins[k].count * bins[k].maxn;
+    const int64_t l = (int64_t)bins[k].count * 120;
+    return bins[k].width == 0 && merged ? l + (int64_t)merged->count * 256 : l;
   };
   auto launch_bin = [&](int k, hipStream_t sk) {
-    if (bins[k].maxn == 0 && bins[k].width == 0)
+    if (bins[k].maxn == 0 && bins[k].width == 0 && merged)
+      sc::launch_integrate_all(d, g, bins[k].list, bins[k].count, h->pack_cs[1], h->pack_cs[2],
+                               merged->list, merged->count, h->tune.int_waves, odd, sk);
+    else if (bins[k].maxn == 0 && bins[k].width == 0)
       sc::launch_integrate_packed_all(d, g, bins[k].list, bins[k].count, h->pack_cs[1],
                                       h->pack_cs[2], h->tune.int_waves, odd, sk);
     else if (bins[k].maxn == 0)

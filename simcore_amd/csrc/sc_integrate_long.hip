@@ -798,6 +798,57 @@ void k_integrate_packed_any(Dev d, Geo g, const int32_t *lane_tab, int n_waves, 
   packed_any_body<MID>(d, g, lane_tab, n_waves, cs1, cs2, blockIdx.x);
 }
 
+// Every filament of the step in one launch: the n_long filaments of 129-256
+// sites (four sites per lane, one filament per wave) take the first waves,
+// so these latency-bound waves start first and run beside the packed waves
+// that follow, instead of in a launch of their own after them.  The same
+// arithmetic as the two separate launches, bit for bit.
+template <bool MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_integrate_all(Dev d, Geo g, const int32_t *lane_tab, int n_waves, int cs1, int cs2,
+                     const int32_t *long_list, int n_long) {
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wave < n_long) {
+    Ctl *ctl = d.ctl;
+    if (ctl->abort) return;
+    if ((ctl->midstep != 0) != MID) {
+      if (wave == 0 && (threadIdx.x & 63) == 0) {
+        ctl->err = SC_ERR_STATE;
+        ctl->abort = 1;
+      }
+      return;
+    }
+    long_body<64, 4, MID>(d, g, true, long_list[wave], threadIdx.x & 63, 64, wave);
+    return;
+  }
+  // the packed waves, numbered from 0 after the long ones (the block index
+  // argument counts whole blocks, so the offset goes into the wave index)
+  const int pw = wave - n_long;
+  Ctl *ctl = d.ctl;
+  if (ctl->abort) return;
+  if ((ctl->midstep != 0) != MID) {
+    if (wave == 0 && (threadIdx.x & 63) == 0) {
+      ctl->err = SC_ERR_STATE;
+      ctl->abort = 1;
+    }
+    return;
+  }
+  if (pw >= n_waves) return;
+  const int lane = threadIdx.x & 63;
+  const int32_t e = lane_tab[(int64_t)pw * 64 + lane];
+  const bool factive = e >= 0;
+  const int f = factive ? (e >> 8) : 0;
+  const int gl = factive ? (e & 63) : 0;
+  int Gf = 1;
+  if (factive) Gf = (d.site_off[f + 1] - d.site_off[f] + 1) >> 1;
+  if (pw < cs1)
+    long_body<16, 2, MID>(d, g, factive, f, gl, Gf, pw);
+  else if (pw < cs2)
+    long_body<32, 2, MID>(d, g, factive, f, gl, Gf, pw);
+  else
+    long_body<64, 2, MID>(d, g, factive, f, gl, Gf, pw);
+}
+
 // Ensemble batches: every member's packed waves, one wave per block.  Odd
 // steps (midsteps) use the members' odd-step pointers.
 template <bool MID>
@@ -836,6 +887,20 @@ void launch_integrate_packed_all(const Dev &d, const Geo &g, const int32_t *lane
   else
     hipLaunchKernelGGL((k_integrate_packed_any<false>), grid, block, 0, st, d, g, lane_tab,
                        n_waves, cs1, cs2);
+}
+
+void launch_integrate_all(const Dev &d, const Geo &g, const int32_t *lane_tab, int n_waves,
+                          int cs1, int cs2, const int32_t *long_list, int n_long, int waves,
+                          bool mid, hipStream_t st) {
+  const int total = n_waves + n_long;
+  if (total <= 0) return;
+  dim3 grid((total + waves - 1) / waves), block(64 * waves);
+  if (mid)
+    hipLaunchKernelGGL((k_integrate_all<true>), grid, block, 0, st, d, g, lane_tab, n_waves,
+                       cs1, cs2, long_list, n_long);
+  else
+    hipLaunchKernelGGL((k_integrate_all<false>), grid, block, 0, st, d, g, lane_tab, n_waves,
+                       cs1, cs2, long_list, n_long);
 }
 
 void launch_integrate_packed(const Dev &d, const Geo &g, const int32_t *lane_tab, int n_waves,

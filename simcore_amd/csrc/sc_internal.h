@@ -155,8 +155,9 @@ struct Tune {
   int int_waves = 1;     // waves per block of the register integrator
   int int_layout = 1;    // up to 64 sites: 0 one site per lane, 1 two (the
                          // two round differently; not changed by tuning);
-                         // mixed path: 2 packed lane segments, one launch,
-                         // 3 the same, one launch per lane-group width
+                         // mixed path: 2 packed lane segments, one launch
+                         // for every filament (129-256 sites in its first
+                         // waves), 3 one launch per lane-group width
   int int_streams = 0;   // length bins spread over this many side streams (0-3)
   int parts = 1;         // >1: the bonds are cut into this many ranges; the
                          // integrator of a range's filaments runs on a side
@@ -278,6 +279,10 @@ void launch_integrate_packed(const Dev &d, const Geo &g, const int32_t *lane_tab
 void launch_integrate_packed_all(const Dev &d, const Geo &g, const int32_t *lane_tab,
                                  int n_waves, int cs1, int cs2, int waves, bool mid,
                                  hipStream_t st);
+// the same plus the filaments of 129-256 sites (long_list), in one launch
+void launch_integrate_all(const Dev &d, const Geo &g, const int32_t *lane_tab, int n_waves,
+                          int cs1, int cs2, const int32_t *long_list, int n_long, int waves,
+                          bool mid, hipStream_t st);
 void launch_bond_init(const Dev &d, const Geo &g, int precision,
                       hipStream_t st);
 // crosslinks (sc_crosslink.hip)
