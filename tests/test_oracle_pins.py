@@ -486,3 +486,35 @@ def test_noise_buffer_of_zeros_is_the_noiseless_step():
     d.set_noise_buffer(xi, 15)
     assert c.step(30) == 0 and d.step(30) == 0
     np.testing.assert_array_equal(c.sites()[0], d.sites()[0])
+
+
+def _bend_disorder(o, so):
+    """Mean 1 - cos of consecutive bond directions over all filaments."""
+    r = o.sites()[0].reshape(3, -1)
+    c = []
+    for f in range(len(so) - 1):
+        u = np.diff(r[:, so[f]:so[f + 1]], axis=1)
+        u /= np.linalg.norm(u, axis=0)
+        c.append(1.0 - np.mean(np.sum(u[:, 1:] * u[:, :-1], axis=0)))
+    return float(np.mean(c))
+
+
+def test_stiff_chains_need_the_small_time_step():
+    """Why the stiff (lp up to 1000 sigma) workloads run at dt 1e-5: eight
+    64-site filaments with lp = 1000 sigma, bond length 1.  At dt 1e-5 the
+    bending disorder <1 - cos theta> stays at the worm-like-chain value
+    1 - (coth(lp/b) - b/lp) ~ b/lp = 1e-3; at dt 1e-4 the explicit bending
+    update is unstable and the chains lose their orientation within 400
+    steps (<1 - cos theta> of order 1, a random walk)."""
+    wlc = 1.0 - (1.0 / np.tanh(1000.0) - 1.0 / 1000.0)
+    out = {}
+    for dt in (1e-5, 1e-4):
+        p = sc.params_init(40.0, delta=dt, min_cell_length=1.0, seed=3)
+        fil = sc.generate_filaments(p, [64] * 8, 63.0, 1000.0, 1.0, 2.0, seed=4)
+        o = O.OracleSystem(p, fil, n_threads=1)
+        assert o.step(400) == 0
+        out[dt] = _bend_disorder(o, np.asarray(fil.site_off))
+        o.close()
+    print(out, wlc)
+    assert 0.5 * wlc < out[1e-5] < 2.0 * wlc
+    assert out[1e-4] > 0.3
