@@ -490,22 +490,30 @@ def main():
     achieved_int = bytes_int / (int_ms * 1e-3) / 1e9 if int_ms > 0 else 0.0
     int_traffic = None
     if ttag and args.precision == "mixed":
-        # the kernels this launch shape runs: the packed filaments in one
-        # launch (layout 2) or one per lane-group width (3), the longer ones
-        # in their length bin
-        packed_part = ("k_integrate_packed_any<" if tune_layout == 2 else "k_integrate_packed<")
-        tf = measured_traffic_sum(ttag, packed_part, "false>")
-        tm = measured_traffic_sum(ttag, packed_part, "true>")
-        if tf is not None and tm is not None:
-            tf += measured_traffic_sum(ttag, "k_integrate_long<", "false>") or 0.0
-            tm += measured_traffic_sum(ttag, "k_integrate_long<", "true>") or 0.0
+        # the kernels this launch shape runs: layout 2 one launch for every
+        # filament (k_integrate_all, or k_integrate_packed_any without
+        # filaments above 128 sites); layout 3 one launch per lane-group
+        # width plus the longer filaments' length bin
+        if tune_layout == 2:
+            tf = measured_traffic_sum(ttag, "k_integrate_all<", "false>")
+            tm = measured_traffic_sum(ttag, "k_integrate_all<", "true>")
+            if tf is None or tm is None:
+                tf = measured_traffic_sum(ttag, "k_integrate_packed_any<", "false>")
+                tm = measured_traffic_sum(ttag, "k_integrate_packed_any<", "true>")
+        else:
+            tf = measured_traffic_sum(ttag, "k_integrate_packed<", "false>")
+            tm = measured_traffic_sum(ttag, "k_integrate_packed<", "true>")
+            if tf is not None and tm is not None:
+                tf += measured_traffic_sum(ttag, "k_integrate_long<", "false>") or 0.0
+                tm += measured_traffic_sum(ttag, "k_integrate_long<", "true>") or 0.0
         if tf is not None and tm is not None:
             int_traffic = 0.5 * (tf + tm)
     roofline_int = {"bound": "hbm", "achieved": round(achieved_int, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved_int / HBM_PEAK_GBS, 4),
                     "traffic": int_traffic,
-                    "kernel": "integrator (k_integrate_packed and k_integrate_long: every "
-                              "filament of a step)",
+                    "kernel": "integrator (every filament of a step: k_integrate_all / "
+                              "k_integrate_packed_any, or per-width k_integrate_packed "
+                              "and k_integrate_long)",
                     "kernel_avg_ms": int_ms, "algorithmic_bytes": bytes_int}
     r_end = dev.sites()[0]
     dev.close()
