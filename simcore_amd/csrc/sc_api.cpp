@@ -87,6 +87,7 @@ struct sc_system {
   // allocations (from `arena` when set, else hipMalloc)
   sc::Arena *arena = nullptr;
   bool own_stream = true;
+  bool own_ctl_host = true;  // ctl_host allocated here (else the caller's)
   int device = 0;  // HIP device current at creation
   double *xi_dev = nullptr;  // caller-supplied noise (sc_set_noise)
   double max_bond_length = 0.0;
@@ -829,7 +830,7 @@ int sc_create(const sc_params *params, sc_handle *out) {
 namespace sc {
 
 int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
-                  sc_handle *out) {
+                  sc_handle *out, Ctl *pinned_ctl) {
   if (!params || !out) return SC_ERR_ARG;
   *out = nullptr;
   const sc_params &p = *params;
@@ -888,10 +889,11 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
     delete h;
     return SC_ERR_HIP;
   }
-  void *pin = nullptr;
+  void *pin = pinned_ctl;
+  h->own_ctl_host = pinned_ctl == nullptr;
   if (hipEventCreateWithFlags(&h->done_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipHostMalloc(&pin, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) {
+      (!pin && hipHostMalloc(&pin, sizeof(Ctl), hipHostMallocDefault) != hipSuccess)) {
     sc_destroy(h);
     return SC_ERR_HIP;
   }
@@ -2315,7 +2317,7 @@ int sc_destroy(sc_handle h) {
   }
   for (int k = 0; k < sc::kMaxParts; ++k)
     if (h->part_ev[k]) (void)hipEventDestroy(h->part_ev[k]);
-  if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+  if (h->ctl_host && h->own_ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->xi_dev) (void)hipFree(h->xi_dev);
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   delete h;

@@ -171,6 +171,7 @@ struct sc_ensemble_s {
   std::deque<int64_t> queue;
   std::vector<int64_t> running;
   std::vector<Batch> batches;
+  sc::Ctl *ctl_pool = nullptr;  // pinned control-block mirror per slot
   std::chrono::steady_clock::time_point t0;
   std::string err;
   int32_t peak_active = 0;
@@ -300,7 +301,7 @@ bool finish_if_done(sc_ensemble e, Request &R) {
 // Creates and loads R's system in slot `slot`; SC_ERR_NOMEM leaves R queued.
 int admit(sc_ensemble e, Request &R, int slot) {
   sc_handle h = nullptr;
-  int rc = sc::system_create(&R.p, e->streams[slot], &e->arena, &h);
+  int rc = sc::system_create(&R.p, e->streams[slot], &e->arena, &h, &e->ctl_pool[slot]);
   if (rc) return rc;
   rc = sc_load_filaments(h, R.nf, R.site_off.data(), R.r.data(), R.fil.data());
   if (rc) {
@@ -337,10 +338,18 @@ int sc_ensemble_create(const sc_ensemble_params *params, sc_ensemble *out) {
   }
   e->prm.hbm_budget = budget;
   if (e->arena.init(budget) != SC_OK) return SC_ERR_NOMEM;
+  {
+    void *pin = nullptr;
+    if (hipHostMalloc(&pin, sizeof(sc::Ctl) * e->prm.max_active, hipHostMallocDefault) !=
+        hipSuccess)
+      return SC_ERR_HIP;
+    e->ctl_pool = static_cast<sc::Ctl *>(pin);
+  }
   for (int k = 0; k < e->prm.max_active; ++k) {
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
       for (hipStream_t s : e->streams) (void)hipStreamDestroy(s);
+      (void)hipHostFree(e->ctl_pool);
       return SC_ERR_HIP;
     }
     e->streams.push_back(st);
@@ -664,6 +673,7 @@ int sc_ensemble_destroy(sc_ensemble e) {
       p->h = nullptr;
     }
   for (hipStream_t s : e->streams) (void)hipStreamDestroy(s);
+  if (e->ctl_pool) (void)hipHostFree(e->ctl_pool);
   e->arena.release();
   delete e;
   return SC_OK;

@@ -40,10 +40,13 @@ class Arena {
   std::unordered_map<size_t, size_t> taken_;  // offset -> size
 };
 
-// sc_create with a caller-owned stream (not destroyed with the handle) and
-// an optional arena for every device buffer.
+// sc_create with a caller-owned stream (not destroyed with the handle), an
+// optional arena for every device buffer and an optional caller-owned
+// pinned control-block mirror (pinned allocations and frees are slow and
+// may wait for the device; the ensemble runner keeps one per slot).
+struct Ctl;
 int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
-                  sc_handle *out);
+                  sc_handle *out, Ctl *pinned_ctl = nullptr);
 
 // Ensemble batches (grouped launches, sc_internal.h).  A system can join a
 // batch when it is loaded, idle, at an even step count (batches take an odd
@@ -51,7 +54,6 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
 // covering every filament (none above 128 sites), and without crosslinks,
 // thermodynamic sums, a caller noise buffer or profiling.
 struct Member;
-struct Ctl;
 bool system_batchable(sc_handle h);
 // The system's entry in a batch's member table.
 void system_member(sc_handle h, Member *m);
