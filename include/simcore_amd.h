@@ -391,7 +391,8 @@ typedef struct sc_request_info {
   int32_t n_filaments;
   int64_t n_sites, n_bonds, n_steps, steps_done;
   double t_submit, t_admit, t_done; /* seconds on the ensemble clock       */
-  size_t device_bytes;              /* footprint while resident            */
+  size_t device_bytes;              /* footprint while resident (at the end
+                                       for a finished request)             */
 } sc_request_info;
 
 typedef struct sc_ensemble_stats {

@@ -286,6 +286,7 @@ void fail_request(sc_ensemble e, Request &R, int rc) {
 // R's last chunk has ended (R idle): retire it if finished
 bool finish_if_done(sc_ensemble e, Request &R) {
   if (R.enqueued < R.n_steps) return false;
+  (void)sc_device_bytes(R.h, &R.bytes);  // the footprint it ended with
   R.r_out.resize(3 * (size_t)R.ns);
   const int rc = sc_get_sites(R.h, R.r_out.data(), nullptr, nullptr);
   if (rc != SC_OK) {

@@ -216,3 +216,43 @@ def test_ensemble_batch_with_rebuilds_and_spatial_order():
             np.testing.assert_array_equal(ens.result(rid), ref)
         assert st["batched_steps"] == 4 * 120
         assert st["solo_steps"] == 40
+
+
+def test_ensemble_batch_member_grows_its_candidate_capacity():
+    """A batch member whose candidate capacity is its exact load-time count
+    outgrows it at a rebuild inside a batched chunk: its kernels stop at the
+    overflow while the other members go on, it grows its buffers and re-runs
+    the rest of the chunk on its own stream, then rejoins the batches.  Every
+    result equals the system stepped alone with room to spare."""
+    import ctypes
+    steps = 240
+    # a system whose list grows early: 724 candidates at load, 738 at the
+    # rebuild of step 17 (CPU oracle)
+    p0, f0 = c0_system(num=64, seed=53, noise_seed=653)
+    cases = [(p0, f0, steps)] + [(p, f, steps) for p, f, _ in _batchable()[:3]]
+    print(f"case: batch member with exact capacity, {f0.nb} bonds, {steps} steps", flush=True)
+    with sc.FilamentSystem(p0, f0, precision="mixed", use_graph=True) as s:
+        n0 = s.counters()["n_candidates"]
+    tight = type(p0)()
+    ctypes.pointer(tight)[0] = p0
+    tight.capacity_factor = -float(n0)  # the ensemble takes the parameters as given
+    with sc.FilamentSystem(p0, f0, precision="mixed", use_graph=True,
+                           capacity_factor=-float(n0)) as s:
+        b0 = s.device_bytes()
+        s.step(steps)
+        assert s.device_bytes() > b0, "the capacity never overflowed in this run"
+        grown_alone = s.sites()[0]
+    refs = [_alone(p, f, n) for p, f, n in cases]
+    np.testing.assert_array_equal(grown_alone, refs[0])
+    with sc.Ensemble(hbm_budget=2 << 30, max_active=8, steps_per_launch=16,
+                     batch_size=8) as ens:
+        ids = [ens.submit(tight, f0, steps, precision="mixed")]
+        ids += [ens.submit(p, f, n, precision="mixed") for p, f, n in cases[1:]]
+        ens.drain(timeout_s=120)
+        st = ens.stats()
+        print(st)
+        for rid, ref in zip(ids, refs):
+            assert ens.request(rid)["state"] == "done"
+            np.testing.assert_array_equal(ens.result(rid), ref)
+        assert ens.request(ids[0])["device_bytes"] > b0
+        assert st["batched_steps"] >= 3 * steps
