@@ -904,9 +904,14 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
 }
 
 bool system_batchable(sc_handle h) {
-  return h && h->loaded && !h->pending && !h->prof && h->p.precision == SC_PREC_MIXED &&
-         h->g.nx == 0 && !h->g.thermo && !h->g.xi_buf && h->long_last < 0 &&
-         h->pack_waves > 0 && h->rec[0] && !h->ctl_host->abort && (h->i_step_host & 1) == 0;
+  if (!h || !h->loaded || h->pending || h->prof || h->p.precision != SC_PREC_MIXED ||
+      h->g.nx != 0 || h->g.thermo || h->g.xi_buf || !h->rec[0] || h->ctl_host->abort ||
+      (h->i_step_host & 1) != 0)
+    return false;
+  // every filament in the packed waves or the 256-site bin
+  for (const Bin &b : h->bins)
+    if (b.maxn > 128 && b.maxn != 256) return false;
+  return h->pack_waves > 0 || !h->bins.empty();
 }
 
 void system_member(sc_handle h, Member *m) {
@@ -921,6 +926,13 @@ void system_member(sc_handle h, Member *m) {
   m->pack_waves = h->pack_waves;
   m->cs1 = h->pack_cs[1];
   m->cs2 = h->pack_cs[2];
+  m->long_list = nullptr;
+  m->n_long = 0;
+  for (const Bin &b : h->bins)
+    if (b.maxn == 256) {
+      m->long_list = b.list;
+      m->n_long = b.count;
+    }
 }
 
 int system_batch_done(sc_handle h, const Ctl &c, int64_t target, int *done) {

@@ -804,10 +804,10 @@ void k_integrate_packed_any(Dev d, Geo g, const int32_t *lane_tab, int n_waves, 
 // that follow, instead of in a launch of their own after them.  The same
 // arithmetic as the two separate launches, bit for bit.
 template <bool MID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-void k_integrate_all(Dev d, Geo g, const int32_t *lane_tab, int n_waves, int cs1, int cs2,
-                     const int32_t *long_list, int n_long) {
-  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+__device__ __forceinline__ void all_body(const Dev &d, const Geo &g, const int32_t *lane_tab,
+                                         int n_waves, int cs1, int cs2,
+                                         const int32_t *long_list, int n_long, int bid) {
+  const int wave = bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wave < n_long) {
     Ctl *ctl = d.ctl;
     if (ctl->abort) return;
@@ -849,15 +849,23 @@ void k_integrate_all(Dev d, Geo g, const int32_t *lane_tab, int n_waves, int cs1
     long_body<64, 2, MID>(d, g, factive, f, gl, Gf, pw);
 }
 
-// Ensemble batches: every member's packed waves, one wave per block.  Odd
-// steps (midsteps) use the members' odd-step pointers.
+template <bool MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_integrate_all(Dev d, Geo g, const int32_t *lane_tab, int n_waves, int cs1, int cs2,
+                     const int32_t *long_list, int n_long) {
+  all_body<MID>(d, g, lane_tab, n_waves, cs1, cs2, long_list, n_long, blockIdx.x);
+}
+
+// Ensemble batches: every member's filaments as in its one-launch layout
+// (those of 129-256 sites first, then the packed waves), one wave per
+// block.  Odd steps (midsteps) use the members' odd-step pointers.
 template <bool MID>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 void k_group_integrate(const Member *__restrict__ ms, GroupGrid gg) {
   int b0, nblk;
   const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
-  packed_any_body<MID>(M.d[MID ? 0 : 1], M.g, M.pack_tab, M.pack_waves, M.cs1, M.cs2,
-                       blockIdx.x - b0);
+  all_body<MID>(M.d[MID ? 0 : 1], M.g, M.pack_tab, M.pack_waves, M.cs1, M.cs2, M.long_list,
+                M.n_long, blockIdx.x - b0);
 }
 
 void launch_group_integrate(const Member *dm, const Member *hm, int n, bool odd,
@@ -866,7 +874,7 @@ void launch_group_integrate(const Member *dm, const Member *hm, int n, bool odd,
   int32_t t = 0;
   for (int m = 0; m < n; ++m) {
     gg.off[m] = t;
-    t += hm[m].pack_waves;
+    t += hm[m].pack_waves + hm[m].n_long;
   }
   for (int m = n; m <= kMaxGroup; ++m) gg.off[m] = t;
   if (t <= 0) return;

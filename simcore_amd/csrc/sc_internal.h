@@ -191,6 +191,8 @@ struct Member {
   Geo g;
   const int32_t *pack_tab;  // packed integrator lane table
   int pack_waves, cs1, cs2; // its waves; first wave of width 32, of width 64
+  const int32_t *long_list; // filaments of 129-256 sites (one wave each)
+  int n_long;
 };
 struct GroupGrid {
   int32_t off[kMaxGroup + 1];

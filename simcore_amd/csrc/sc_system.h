@@ -50,9 +50,9 @@ int system_create(const sc_params *params, hipStream_t stream, Arena *arena,
 
 // Ensemble batches (grouped launches, sc_internal.h).  A system can join a
 // batch when it is loaded, idle, at an even step count (batches take an odd
-// and an even step in turn), in mixed precision with the packed integrator
-// covering every filament (none above 128 sites), and without crosslinks,
-// thermodynamic sums, a caller noise buffer or profiling.
+// and an even step in turn), in mixed precision with every filament in the
+// packed waves or the 256-site bin, and without crosslinks, thermodynamic
+// sums, a caller noise buffer or profiling.
 struct Member;
 bool system_batchable(sc_handle h);
 // The system's entry in a batch's member table.

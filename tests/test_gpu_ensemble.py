@@ -192,30 +192,37 @@ def test_ensemble_batches_match_alone(batch_size):
 
 def test_ensemble_batch_with_rebuilds_and_spatial_order():
     """Spatially ordered mixed systems whose candidate lists are rebuilt
-    inside batched chunks (the grouped rebuild launches), among a system that
-    cannot join a batch (filaments above 128 sites): results equal each
+    inside batched chunks (the grouped rebuild launches), a system with
+    filaments of every length up to 256 sites (its 129-256-site filaments in
+    the first waves of the grouped integrator launch), and an FP64 request,
+    which cannot join a batch and runs on its own stream: results equal each
     system alone, rebuild counts included."""
-    print("case: spatial order, rebuilds inside batches, one long system solo", flush=True)
-    cases = [(p, f, 120) for p, f, _ in _batchable()[:4]]
+    print("case: spatial order, rebuilds inside batches, long filaments, one FP64 solo",
+          flush=True)
+    cases = [(p, f, 120, "mixed") for p, f, _ in _batchable()[:4]]
     p, f = long_system(seed=41, noise_seed=404)
-    cases.append((p, f, 40))
+    cases.append((p, f, 40, "mixed"))
+    p, f = c0_system(num=32, seed=8, noise_seed=808)
+    cases.append((p, f, 30, "fp64"))
     refs, rebuilds = [], []
-    for p, f, n in cases:
-        with sc.FilamentSystem(p, f, precision="mixed", use_graph=True, spatial_order=True) as s:
+    for p, f, n, prec in cases:
+        with sc.FilamentSystem(p, f, precision=prec, use_graph=True,
+                               spatial_order=prec == "mixed") as s:
             s.step(n)
             refs.append(s.sites()[0])
             rebuilds.append(s.counters()["n_rebuilds"])
     assert min(rebuilds[:4]) >= 3
     with sc.Ensemble(hbm_budget=2 << 30, max_active=8, steps_per_launch=32,
                      batch_size=8) as ens:
-        ids = [ens.submit(p, f, n, precision="mixed", spatial_order=True) for p, f, n in cases]
+        ids = [ens.submit(p, f, n, precision=prec, spatial_order=prec == "mixed")
+               for p, f, n, prec in cases]
         ens.drain(timeout_s=120)
         st = ens.stats()
         print(st)
         for rid, ref in zip(ids, refs):
             np.testing.assert_array_equal(ens.result(rid), ref)
-        assert st["batched_steps"] == 4 * 120
-        assert st["solo_steps"] == 40
+        assert st["batched_steps"] == 4 * 120 + 40
+        assert st["solo_steps"] == 30
 
 
 def test_ensemble_batch_member_grows_its_candidate_capacity():
