@@ -275,9 +275,10 @@ int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
  * side streams, and on the packed layout one launch or one per lane-group
  * width; the sites per lane are kept), then the overlap of the two over
  * bond ranges, and the early integration of the long filaments -- over
- * `steps` steps each
- * from a checkpoint of the current state, keeps the fastest and restores
- * the state, so the trajectory is the one without tuning.  The cell size
+ * `steps` steps each from a checkpoint of the current state; the four
+ * fastest are timed once more, each scored by the faster of its two
+ * timings; keeps the best and restores the state, so the trajectory is the
+ * one without tuning.  The cell size
  * is not tuned: in the reference it sets the rebuild threshold and the
  * candidate set (src/cell_list.cpp:9-64), i.e. it is part of the model. */
 typedef struct sc_tune_report {
