@@ -960,6 +960,7 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   const int64_t ncol = n1d * n1d, occ_words = (n1d + 31) / 32;
   b += R(4 * nb) + R(4 * ncol) + R(4 * (ncol + 1)) + 2 * R(4 * nb);  // keys, columns, col_tmp/rank
   b += R(8 * n_cells) + R(4 * ncol * occ_words);                 // cell runs, occupancy
+  b += R(2 * ncol * occ_words) + R(4 * ncol) + R(4 * nb);        // run tables
   b += 2 * R(4 * nb) + R(4 * nb) + R(4 * (nb + 1));  // sorted, skey, deg, inc_start
   b += 2 * R(4 * nb) + R(8 * sc::kRunSlab * nb);      // neighbour masks, positions, run slabs
   const int64_t tmp = std::max(scan_tmp_elems(ncol), scan_tmp_elems(nb));
@@ -1168,6 +1169,9 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = dalloc(h, &d.sorted, nb, false))) return rc;
   if ((rc = dalloc(h, &d.skey, nb, false))) return rc;
   if ((rc = dalloc(h, &d.cell_occ, ncol * g.occ_words, false))) return rc;
+  if ((rc = dalloc(h, &d.occ_pre, ncol * g.occ_words, false))) return rc;
+  if ((rc = dalloc(h, &d.col_ncell, ncol, false))) return rc;
+  if ((rc = dalloc(h, &d.cell_first, nb, false))) return rc;
   if ((rc = dalloc(h, &d.deg, nb, false))) return rc;
   if ((rc = dalloc(h, &d.nmask, nb, false))) return rc;
   if ((rc = dalloc(h, &d.nrun, sc::kRunSlab * (size_t)nb, false))) return rc;

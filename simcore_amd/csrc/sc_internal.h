@@ -115,6 +115,11 @@ struct Dev {
   int32_t *sorted, *skey;  // nb: bonds in (key, bond) order, their keys
   int2 *cell_run;          // n_cells: [start, end) of an occupied cell
   uint32_t *cell_occ;      // n1d^2 * occ_words: bit z of column (x, y)
+  uint16_t *occ_pre;       // n1d^2 * occ_words: occupied cells in the
+                           // column's words before this one
+  int32_t *col_ncell;      // n1d^2: occupied cells of each column
+  int32_t *cell_first;     // nb: per column, at its range of `sorted`, the
+                           // first entry of each occupied cell in z order
   // candidates: per bond, its incidences in candidate order (CSR over
   // inc_start), each with its own copy of the pair's sticky force -- both
   // owners compute the same bits, so the copies agree, and a bond's slot
