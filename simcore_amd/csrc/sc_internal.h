@@ -128,9 +128,8 @@ struct Dev {
   int32_t *deg, *inc_start;
   uint32_t *nmask;         // per bond, rebuild scratch: occupancy of its 13
                            // half-shell (bits 0-12) and 13 mirrored cells
-  int2 *nrun;              // per sorted position, rebuild scratch: kRunSlab
-                           // runs (own cell, then the occupied stencil cells)
-  int32_t *pos_of;         // per bond, rebuild scratch: its sorted position
+  int2 *nrun;              // per bond, rebuild scratch: kRunSlab runs (own
+                           // cell, then the occupied stencil cells)
   int2 *inc;               // {partner bond, role | written << 1 | (owner
                            // mod 64) << 2}: role 1 when the bond is obj2;
                            // written once the slot holds a force (a slot

@@ -536,9 +536,10 @@ __device__ __forceinline__ void deg_one(const Dev &d, const Geo &g, int64_t p) {
   const uint32_t m = shell_mask(d, g, cx, cy, cz);
   d.nmask[b] = m;
   // the bond's own run and those of its first kRunSlab - 1 occupied
-  // stencil cells (in bit order) go to the slab of its sorted position
-  // (written in order, so the stores coalesce), where k_pull finds them
-  // without more dependent lookups
+  // stencil cells (in bit order) go to the bond's slab (64 bytes, one
+  // whole line, so the scattered stores waste nothing), where the list
+  // pass, which runs in bond order, reads them in coalesced loads without
+  // more dependent lookups
   // (the first kRunSlab - 1 lookups are unrolled and independent, so they
   // are in flight together; the slab is stored after them)
   uint32_t mm = m;
@@ -571,8 +572,7 @@ __device__ __forceinline__ void deg_one(const Dev &d, const Geo &g, int64_t p) {
     deg += (rr.y - rr.x) - (klo == kc) - (khi == kc);
   }
   d.deg[b] = deg;
-  d.pos_of[b] = (int32_t)p;
-  int4 *slab = reinterpret_cast<int4 *>(d.nrun + p * kRunSlab);
+  int4 *slab = reinterpret_cast<int4 *>(d.nrun + (int64_t)b * kRunSlab);
   static_assert(kRunSlab == 8, "four int4 stores per slab");
   slab[0] = make_int4(own.x, own.y, r[0].x, r[0].y);
   slab[1] = make_int4(r[1].x, r[1].y, r[2].x, r[2].y);
@@ -640,7 +640,7 @@ __device__ __forceinline__ void pull_one(const Dev &d, const Geo &g, int32_t b,
   const uint32_t m = d.nmask[b];  // occupied neighbours (k_deg)
   // the runs k_deg kept: own cell first, then the occupied stencil cells
   // in bit order; one beyond the slab is looked up in the cell list
-  const int4 *sl = reinterpret_cast<const int4 *>(d.nrun + (int64_t)d.pos_of[b] * kRunSlab);
+  const int4 *sl = reinterpret_cast<const int4 *>(d.nrun + (int64_t)b * kRunSlab);
   static_assert(kRunSlab == 8, "four int4 loads per slab");
   const int4 s0 = sl[0], s1 = sl[1], s2 = sl[2], s3 = sl[3];
   const int2 own = make_int2(s0.x, s0.y);
