@@ -273,21 +273,51 @@ void launch_scan(const int32_t *in, int32_t *out, int64_t n, int32_t *tmp,
 
 // CellList::FindCellCoords over every bond (src/cell_list.cpp:139-157) and
 // the population of each cell column (x, y) (n1d^2 counters instead of
-// n1d^3, so the scan that follows is small); the count each bond's atomic
-// returns is its place in its column.  Streaming passes loop over their
-// range, so any grid size serves.
+// n1d^3, so the scan that follows is small), with each bond's place in its
+// column.  Streaming passes loop over their range, so any grid size serves.
+// The column counts are aggregated per block tile first: each bond enters
+// its column into an LDS hash table (linear probing; a tile of kScanBlock
+// bonds has at most that many columns, so it always fits) and takes its
+// place among the tile's bonds of that column from an LDS counter; one
+// device-scope atomic per distinct column of the tile then hands out the
+// tile's block of places.  With spatially ordered storage a tile touches
+// a few tens of columns, so the device-scope atomics (which on MI355X go
+// past the XCD's L2 to memory) drop by an order of magnitude.  A bond's
+// place is any unique slot of its column; the column sort orders them.
 __device__ __forceinline__ void keys_body(const Dev &d, const Geo &g, int64_t bid, int64_t nblk) {
-  const int64_t stride = nblk * blockDim.x;
-  for (int64_t b = bid * blockDim.x + threadIdx.x; b < g.nb; b += stride) {
-    int32_t k;
-    if (g.compact) {
-      k = rec_key_compact(d.br, g, b);
-    } else {
-      const double *rec = d.br + 8 * b;
-      k = cell_key(g.n1d, rec[0], rec[1], rec[2]);
+  __shared__ int32_t hcol[kScanBlock], hcnt[kScanBlock], hbase[kScanBlock];
+  const int t = threadIdx.x;
+  const int64_t ntiles = (g.nb + kScanBlock - 1) / kScanBlock;
+  for (int64_t tile = bid; tile < ntiles; tile += nblk) {
+    hcol[t] = -1;
+    hcnt[t] = 0;
+    __syncthreads();
+    const int64_t b = tile * kScanBlock + t;
+    int32_t k = 0, slot = 0, local = 0;
+    if (b < g.nb) {
+      if (g.compact) {
+        k = rec_key_compact(d.br, g, b);
+      } else {
+        const double *rec = d.br + 8 * b;
+        k = cell_key(g.n1d, rec[0], rec[1], rec[2]);
+      }
+      const int32_t c = k / g.n1d;
+      slot = (int32_t)(((uint32_t)c * 2654435761u) >> 24);  // 8 bits of a multiplicative hash
+      for (;;) {
+        const int32_t prev = atomicCAS(&hcol[slot], -1, c);
+        if (prev == -1 || prev == c) break;
+        slot = (slot + 1) & (kScanBlock - 1);
+      }
+      local = atomicAdd(&hcnt[slot], 1);
     }
-    d.keys[b] = k;
-    d.col_rank[b] = atomicAdd(&d.col_count[k / g.n1d], 1);
+    __syncthreads();
+    if (hcol[t] >= 0) hbase[t] = atomicAdd(&d.col_count[hcol[t]], hcnt[t]);
+    __syncthreads();
+    if (b < g.nb) {
+      d.keys[b] = k;
+      d.col_rank[b] = hbase[slot] + local;
+    }
+    __syncthreads();
   }
 }
 
