@@ -591,7 +591,11 @@ def main():
         "data": "synthetic",
         "config": dict(desc, precision=args.precision, graph=True, spatial_order=spatial,
                        launch_shape=tune),
-        "roofline": roofline, "roofline_integrator": roofline_int, "cpu_baseline": cpu,
+        # `roofline`: the step's dominant kernel by measured time (on both
+        # systems now the integrator's one launch); the pair pass beside it
+        "roofline": dict(roofline_int if roofline_int["kernel_avg_ms"] >= roofline["kernel_avg_ms"]
+                         else roofline),
+        "roofline_pair": roofline, "roofline_integrator": roofline_int, "cpu_baseline": cpu,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
