@@ -962,7 +962,7 @@ size_t system_estimate_bytes(const sc_params *params, int32_t nf, int64_t ns, in
   b += R(8 * n_cells) + R(4 * ncol * occ_words);                 // cell runs, occupancy
   b += R(2 * ncol * occ_words) + R(4 * ncol) + R(4 * nb);        // run tables
   b += 2 * R(4 * nb) + R(4 * nb) + R(4 * (nb + 1));  // sorted, skey, deg, inc_start
-  b += R(4 * nb) + R(8 * sc::kRunSlab * nb);          // neighbour masks, run slabs
+  b += R(8 * sc::kRunSlab * nb);                      // run slabs
   const int64_t tmp = std::max(scan_tmp_elems(ncol), scan_tmp_elems(nb));
   b += R(4 * tmp);
   b += R(4 * (int64_t)nf) + (size_t)n_bins * Arena::kAlign;  // bin lists
@@ -1173,7 +1173,6 @@ int sc_load_filaments(sc_handle h, int32_t nf, const int32_t *site_off,
   if ((rc = dalloc(h, &d.col_ncell, ncol, false))) return rc;
   if ((rc = dalloc(h, &d.cell_first, nb, false))) return rc;
   if ((rc = dalloc(h, &d.deg, nb, false))) return rc;
-  if ((rc = dalloc(h, &d.nmask, nb, false))) return rc;
   if ((rc = dalloc(h, &d.nrun, sc::kRunSlab * (size_t)nb, false))) return rc;
   if ((rc = dalloc(h, &d.inc_start, nb + 1, false))) return rc;
   int64_t tmp = std::max(sc::scan_tmp_elems(ncol), sc::scan_tmp_elems(nb));

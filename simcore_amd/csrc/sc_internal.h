@@ -126,10 +126,10 @@ struct Dev {
   // traffic follows its list.  The candidate list itself is implicit (the
   // reference's order is recovered from the lists for the taps).
   int32_t *deg, *inc_start;
-  uint32_t *nmask;         // per bond, rebuild scratch: occupancy of its 13
-                           // half-shell (bits 0-12) and 13 mirrored cells
-  int2 *nrun;              // per bond, rebuild scratch: kRunSlab runs (own
-                           // cell, then the occupied stencil cells)
+  int2 *nrun;              // per bond, rebuild scratch: kRunSlab entries (own
+                           // cell's run, the first six occupied stencil
+                           // cells' runs, the stencil occupancy mask: bits
+                           // 0-12 half shell, 13-25 mirrored)
   int2 *inc;               // {partner bond, role | written << 1 | (owner
                            // mod 64) << 2}: role 1 when the bond is obj2;
                            // written once the slot holds a force (a slot

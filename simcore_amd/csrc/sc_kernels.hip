@@ -564,18 +564,17 @@ __device__ __forceinline__ void deg_one(const Dev &d, const Geo &g, int64_t p) {
   const int n = g.n1d;
   const int cx = k / (n * n), cy = (k / n) % n, cz = k % n;
   const uint32_t m = shell_mask(d, g, cx, cy, cz);
-  d.nmask[b] = m;
-  // the bond's own run and those of its first kRunSlab - 1 occupied
-  // stencil cells (in bit order) go to the bond's slab (64 bytes, one
-  // whole line, so the scattered stores waste nothing), where the list
-  // pass, which runs in bond order, reads them in coalesced loads without
-  // more dependent lookups
-  // (the first kRunSlab - 1 lookups are unrolled and independent, so they
+  // the bond's own run, those of its first kRunSlab - 2 occupied stencil
+  // cells (in bit order) and the occupancy mask go to the bond's slab (64
+  // bytes, one whole line, so the scattered stores waste nothing), where
+  // the list pass, which runs in bond order, reads them in coalesced loads
+  // without more dependent lookups
+  // (the first kRunSlab - 2 lookups are unrolled and independent, so they
   // are in flight together; the slab is stored after them)
   uint32_t mm = m;
-  int2 r[kRunSlab - 1];
+  int2 r[kRunSlab - 2];
 #pragma unroll
-  for (int i = 0; i < kRunSlab - 1; ++i) {
+  for (int i = 0; i < kRunSlab - 2; ++i) {
     r[i] = make_int2(0, 0);
     if (mm) {
       const int q = __ffs(mm) - 1;
@@ -607,7 +606,7 @@ __device__ __forceinline__ void deg_one(const Dev &d, const Geo &g, int64_t p) {
   slab[0] = make_int4(own.x, own.y, r[0].x, r[0].y);
   slab[1] = make_int4(r[1].x, r[1].y, r[2].x, r[2].y);
   slab[2] = make_int4(r[3].x, r[3].y, r[4].x, r[4].y);
-  slab[3] = make_int4(r[5].x, r[5].y, r[6].x, r[6].y);
+  slab[3] = make_int4(r[5].x, r[5].y, (int32_t)m, 0);
 }
 
 // Grid-stride over the sorted positions: a capped grid keeps the launch
@@ -667,13 +666,14 @@ __device__ __forceinline__ void pull_one(const Dev &d, const Geo &g, int32_t b,
   const int cx = k / (n * n), cy = (k / n) % n, cz = k % n;
   const int cls = (cx == 0 ? 0 : cx == n - 1 ? 2 : 1) * 6 +
                   (cy == 0 ? 0 : cy == n - 1 ? 2 : 1) * 2 + (cz == 0 ? 0 : 1);
-  const uint32_t m = d.nmask[b];  // occupied neighbours (k_deg)
   // the runs k_deg kept: own cell first, then the occupied stencil cells
-  // in bit order; one beyond the slab is looked up in the cell list
+  // in bit order, and the occupancy mask of the 26 stencil cells; a run
+  // beyond the slab is looked up in the cell list
   const int4 *sl = reinterpret_cast<const int4 *>(d.nrun + (int64_t)b * kRunSlab);
   static_assert(kRunSlab == 8, "four int4 loads per slab");
   const int4 s0 = sl[0], s1 = sl[1], s2 = sl[2], s3 = sl[3];
   const int2 own = make_int2(s0.x, s0.y);
+  const uint32_t m = (uint32_t)s3.z;
   auto run_of = [&](int q, int32_t key) -> int2 {
     const int idx = 1 + __popc(m & ((1u << q) - 1u));
     switch (idx) {
@@ -683,7 +683,6 @@ __device__ __forceinline__ void pull_one(const Dev &d, const Geo &g, int32_t b,
       case 4: return make_int2(s2.x, s2.y);
       case 5: return make_int2(s2.z, s2.w);
       case 6: return make_int2(s3.x, s3.y);
-      case 7: return make_int2(s3.z, s3.w);
       default: return d.cell_run[key];
     }
   };
