@@ -376,7 +376,7 @@ typedef struct sc_ensemble_s *sc_ensemble;
 typedef struct sc_ensemble_params {
   size_t hbm_budget;        /* arena bytes; 0: 80% of the free device memory */
   int32_t max_active;       /* systems resident at once; 0: 16             */
-  int32_t steps_per_launch; /* steps per chunk; 0: 64                       */
+  int32_t steps_per_launch; /* steps per chunk (batches: at least 2); 0: 64 */
   int32_t batch_size;       /* systems per batch, at most 32; 0: 8; 1: no
                                batches (every system on its own stream)     */
 } sc_ensemble_params;

@@ -219,7 +219,8 @@ int enqueue_chunk(sc_ensemble e, Request &R) {
 // blocks copied back, then B's event.
 int enqueue_batch(sc_ensemble e, Batch &B) {
   const int n = (int)B.ids.size();
-  int64_t k = e->prm.steps_per_launch;
+  // a chunk is at least one odd and one even step (every member has two left)
+  int64_t k = std::max<int64_t>(2, e->prm.steps_per_launch);
   for (int64_t id : B.ids) k = std::min(k, e->reqs[id]->n_steps - e->reqs[id]->enqueued);
   k &= ~(int64_t)1;
   if (n == 0 || k <= 0) return SC_ERR_STATE;

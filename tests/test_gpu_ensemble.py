@@ -162,17 +162,19 @@ def _batchable():
     return out
 
 
-@pytest.mark.parametrize("batch_size", [1, 3, 8])
-def test_ensemble_batches_match_alone(batch_size):
+@pytest.mark.parametrize("batch_size,spl", [(1, 16), (3, 16), (8, 16), (4, 1)])
+def test_ensemble_batches_match_alone(batch_size, spl):
     """Systems stepped in batches -- one set of grouped launches per step for
     several systems, members joining and leaving at chunk boundaries, the
     odd remainders on their own streams -- end with the positions of each
     system stepped alone, bit for bit; batch_size 1 steps every system on
-    its own stream."""
+    its own stream; one step per chunk still takes batches two steps at a
+    time."""
     cases = _batchable()
-    print(f"case: {len(cases)} systems, batch_size {batch_size}, max_active 5", flush=True)
+    print(f"case: {len(cases)} systems, batch_size {batch_size}, steps per chunk {spl}, "
+          f"max_active 5", flush=True)
     refs = [_alone(p, f, n) for p, f, n in cases]
-    with sc.Ensemble(hbm_budget=2 << 30, max_active=5, steps_per_launch=16,
+    with sc.Ensemble(hbm_budget=2 << 30, max_active=5, steps_per_launch=spl,
                      batch_size=batch_size) as ens:
         ids = [ens.submit(p, f, n, precision="mixed") for p, f, n in cases]
         ens.drain(timeout_s=120)
