@@ -54,9 +54,12 @@ struct sc_system {
   sc::Tune tune;
   // graphs: [0] odd step (rebuild stage included), [1] even step (a rebuild
   // is only ever decided after a full step, i.e. at odd steps), [2] the
-  // odd+even pair
-  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
-  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  // odd+even pair, [3] kGraphPairs such pairs (fewer graph boundaries on
+  // long runs)
+  static constexpr int kGraphs = 4;
+  static constexpr int kGraphPairs = 4;
+  hipGraphExec_t gexec[kGraphs] = {};
+  hipGraph_t graph[kGraphs] = {};
   // profiling
   bool prof = false;
   hipEvent_t ev[2 * kFamilies] = {};
@@ -239,7 +242,7 @@ int dalloc(sc_system *h, T **p, size_t n, bool cand) {
 }
 
 void drop_graph(sc_system *h) {
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < sc_system::kGraphs; ++k) {
     if (h->gexec[k]) (void)hipGraphExecDestroy(h->gexec[k]);
     if (h->graph[k]) (void)hipGraphDestroy(h->graph[k]);
     h->gexec[k] = nullptr;
@@ -546,10 +549,13 @@ void enqueue_step(sc_system *h, bool odd) {
 
 int capture_graphs(sc_system *h) {
   drop_graph(h);
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < sc_system::kGraphs; ++k) {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    if (k == 0 || k == 2) enqueue_step(h, true);
-    if (k == 1 || k == 2) enqueue_step(h, false);
+    const int pairs = k == 3 ? sc_system::kGraphPairs : 1;
+    for (int q = 0; q < pairs; ++q) {
+      if (k == 0 || k >= 2) enqueue_step(h, true);
+      if (k >= 1) enqueue_step(h, false);
+    }
     hipGraph_t gr = nullptr;
     HIPCHK(h, hipStreamEndCapture(h->stream, &gr));
     h->graph[k] = gr;
@@ -785,6 +791,12 @@ int enqueue_range(sc_system *h, int64_t from, int64_t count) {
   while (s < count) {
     const bool odd = (next & 1) != 0;
     if (use_graph) {
+      if (odd && count - s >= 2 * sc_system::kGraphPairs) {
+        HIPCHK(h, hipGraphLaunch(h->gexec[3], h->stream));
+        s += 2 * sc_system::kGraphPairs;
+        next += 2 * sc_system::kGraphPairs;
+        continue;
+      }
       if (odd && count - s >= 2) {
         HIPCHK(h, hipGraphLaunch(h->gexec[2], h->stream));
         s += 2;
