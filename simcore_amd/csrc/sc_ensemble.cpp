@@ -137,12 +137,15 @@ struct Batch {
   hipEvent_t ev = nullptr;
   std::vector<int64_t> ids;
   bool busy = false;
-  // an odd and an even step of the members, replayed over a chunk; the
-  // launch grids depend only on the members' sizes (key), the members'
-  // buffers are read from the device table at run time
-  hipGraphExec_t gx = nullptr;
+  // an odd and an even step of the members (gx[0]) and kBatchGraphPairs
+  // such pairs (gx[1]), replayed over a chunk; the launch grids depend only
+  // on the members' sizes (key), the members' buffers are read from the
+  // device table at run time
+  hipGraphExec_t gx[2] = {nullptr, nullptr};
   std::vector<int64_t> gkey;
 };
+
+constexpr int kBatchGraphPairs = 4;
 
 int count_bins(const std::vector<int32_t> &so) {
   const int bin_max[] = {8, 16, 32, 64, 128, 256};
@@ -238,35 +241,44 @@ int enqueue_batch(sc_ensemble e, Batch &B) {
     const sc::Geo &g = B.hm[j].g;
     key.insert(key.end(), {g.nb, g.ns, (int64_t)g.n1d, (int64_t)B.hm[j].pack_waves});
   }
-  if (!B.gx || key != B.gkey) {
+  if (!B.gx[0] || key != B.gkey) {
     const double tcap = e->now();
     ++e->captures;
-    if (B.gx) (void)hipGraphExecDestroy(B.gx);
-    B.gx = nullptr;
     B.gkey.clear();
-    if (hipStreamBeginCapture(B.st, hipStreamCaptureModeThreadLocal) != hipSuccess)
-      return SC_ERR_HIP;
-    // odd step (the rebuild stage, then the midstep), even step
-    sc::launch_group_rebuild(B.dm, B.hm, n, B.st);
-    sc::launch_group_pair(B.dm, B.hm, n, true, B.st);
-    sc::launch_group_integrate(B.dm, B.hm, n, true, B.st);
-    sc::launch_group_commit(B.dm, n, B.st);
-    sc::launch_group_pair(B.dm, B.hm, n, false, B.st);
-    sc::launch_group_integrate(B.dm, B.hm, n, false, B.st);
-    sc::launch_group_commit(B.dm, n, B.st);
-    hipGraph_t gr = nullptr;
-    if (hipStreamEndCapture(B.st, &gr) != hipSuccess) return SC_ERR_HIP;
-    const hipError_t ie = hipGraphInstantiate(&B.gx, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    if (ie != hipSuccess) {
-      B.gx = nullptr;
-      return SC_ERR_HIP;
+    for (int v = 0; v < 2; ++v) {
+      if (B.gx[v]) (void)hipGraphExecDestroy(B.gx[v]);
+      B.gx[v] = nullptr;
+    }
+    for (int v = 0; v < 2; ++v) {
+      if (hipStreamBeginCapture(B.st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return SC_ERR_HIP;
+      for (int q = 0; q < (v ? kBatchGraphPairs : 1); ++q) {
+        // odd step (the rebuild stage, then the midstep), even step
+        sc::launch_group_rebuild(B.dm, B.hm, n, B.st);
+        sc::launch_group_pair(B.dm, B.hm, n, true, B.st);
+        sc::launch_group_integrate(B.dm, B.hm, n, true, B.st);
+        sc::launch_group_commit(B.dm, n, B.st);
+        sc::launch_group_pair(B.dm, B.hm, n, false, B.st);
+        sc::launch_group_integrate(B.dm, B.hm, n, false, B.st);
+        sc::launch_group_commit(B.dm, n, B.st);
+      }
+      hipGraph_t gr = nullptr;
+      if (hipStreamEndCapture(B.st, &gr) != hipSuccess) return SC_ERR_HIP;
+      const hipError_t ie = hipGraphInstantiate(&B.gx[v], gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (ie != hipSuccess) {
+        B.gx[v] = nullptr;
+        return SC_ERR_HIP;
+      }
     }
     B.gkey.swap(key);
     e->host_capture += e->now() - tcap;
   }
-  for (int64_t s = 0; s < k; s += 2)
-    if (hipGraphLaunch(B.gx, B.st) != hipSuccess) return SC_ERR_HIP;
+  for (int64_t s = 0; s < k;) {
+    const bool big = k - s >= 2 * kBatchGraphPairs;
+    if (hipGraphLaunch(B.gx[big ? 1 : 0], B.st) != hipSuccess) return SC_ERR_HIP;
+    s += big ? 2 * kBatchGraphPairs : 2;
+  }
   sc::launch_group_ctl(B.dm, n, B.dctl, B.st);
   if (hipMemcpyAsync(B.hctl, B.dctl, sizeof(sc::Ctl) * n, hipMemcpyDeviceToHost, B.st) !=
           hipSuccess ||
@@ -660,7 +672,8 @@ int sc_ensemble_destroy(sc_ensemble e) {
   if (!e) return SC_ERR_ARG;
   for (Batch &B : e->batches) {
     if (B.st) (void)hipStreamSynchronize(B.st);
-    if (B.gx) (void)hipGraphExecDestroy(B.gx);
+    for (hipGraphExec_t g : B.gx)
+      if (g) (void)hipGraphExecDestroy(g);
     if (B.dm) (void)hipFree(B.dm);
     if (B.dctl) (void)hipFree(B.dctl);
     if (B.hm) (void)hipHostFree(B.hm);
