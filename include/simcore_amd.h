@@ -366,8 +366,8 @@ int sc_generate_crosslinks(const sc_params *params, int32_t n_filaments,
  * kept resident on the device, advanced in chunks of steps, and retired --
  * final positions copied to the host, memory returned -- as they finish.
  * Admission is first-fit in arrival order.  Mixed-precision systems without
- * crosslinks, thermodynamic sums or filaments above 128 sites are stepped
- * in batches: one set of launches per step for up to batch_size systems,
+ * crosslinks, thermodynamic sums or a caller noise buffer are stepped in
+ * batches: one set of launches per step for up to batch_size systems,
  * members joining and leaving at chunk boundaries; the others run their
  * chunks on their own streams.  A system's result does not depend on what
  * else runs beside it. */
