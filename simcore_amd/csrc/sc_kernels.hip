@@ -373,18 +373,28 @@ __device__ __forceinline__ int64_t kb_pair(const Dev &d, int32_t b) {
   return ((int64_t)d.keys[b] << 32) | (uint32_t)b;
 }
 
+// WPB wavefronts per block, each on its own column (its own LDS stage); the
+// blocks step over groups of WPB columns, so every wave of a block runs the
+// same number of rounds and meets the same barriers (a wave past the last
+// column idles through its round).
+template <int WPB>
 __device__ __forceinline__ void colsort_body(const Dev &d, const Geo &g, int64_t bid,
                                              int64_t nblk) {
-  __shared__ int64_t ks[kColStage], kr[kColStage];
-  __shared__ uint32_t occ[64];  // occ_words <= 41 (n1d^3 < 2^31)
-  const int lane = threadIdx.x;
+  __shared__ int64_t ks_w[WPB][kColStage], kr_w[WPB][kColStage];
+  __shared__ uint32_t occ_w[WPB][64];  // occ_words <= 41 (n1d^3 < 2^31)
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int64_t *ks = ks_w[wv], *kr = kr_w[wv];
+  uint32_t *occ = occ_w[wv];
   const int n = g.n1d, W = g.occ_words;
   const int64_t ncol = (int64_t)n * n;
-  for (int64_t c = bid; c < ncol; c += nblk) {
-    const int32_t s = d.col_start[c], e = d.col_start[c + 1], m = e - s;
+  const int64_t ngrp = (ncol + WPB - 1) / WPB;
+  for (int64_t grp = bid; grp < ngrp; grp += nblk) {
+    const int64_t c = grp * WPB + wv;
+    const bool live = c < ncol;
+    const int32_t s = live ? d.col_start[c] : 0, e = live ? d.col_start[c + 1] : 0, m = e - s;
     const bool staged = m <= kColStage;
     if (lane < W) occ[lane] = 0u;
-    if (lane == 0) d.col_count[c] = 0;  // for the next build's count
+    if (live && lane == 0) d.col_count[c] = 0;  // for the next build's count
     if (staged)
       for (int i = lane; i < m; i += 64) ks[i] = kb_pair(d, d.col_tmp[s + i]);
     __syncthreads();
@@ -429,13 +439,13 @@ __device__ __forceinline__ void colsort_body(const Dev &d, const Geo &g, int64_t
       ncell += __popcll(bal);
     }
     __syncthreads();
-    if (lane < W) {
+    if (live && lane < W) {
       int pre = 0;
       for (int w = 0; w < lane; ++w) pre += __popc(occ[w]);
       d.cell_occ[c * W + lane] = occ[lane];
       d.occ_pre[c * W + lane] = (uint16_t)pre;
     }
-    if (lane == 0) d.col_ncell[c] = ncell;
+    if (live && lane == 0) d.col_ncell[c] = ncell;
     __syncthreads();
   }
 }
@@ -461,9 +471,18 @@ __device__ __forceinline__ int2 cell_run_key(const Dev &d, const Geo &g, int32_t
 }
 
 
-__global__ __launch_bounds__(64) void k_colsort(Dev d, Geo g, int guard) {
+// four columns per 256-thread block: as many waves in flight as one
+// column per 64-thread block, a quarter of the workgroups (the launch on
+// the odd steps that do not rebuild costs less)
+constexpr int kColWaves = 4;
+
+__global__ __launch_bounds__(64 * kColWaves) void k_colsort(Dev d, Geo g, int guard) {
   if (guard_off(d.ctl, guard)) return;
-  colsort_body(d, g, blockIdx.x, gridDim.x);
+  colsort_body<kColWaves>(d, g, blockIdx.x, gridDim.x);
+}
+
+static inline unsigned colsort_grid(int64_t ncol) {
+  return (unsigned)std::min<int64_t>((ncol + kColWaves - 1) / kColWaves, 8192 / kColWaves);
 }
 
 // Coordinates of the q-th cell of the half-shell stencil of (x, y, z) in
@@ -787,8 +806,8 @@ void launch_rebuild(const Dev &d, const Geo &g, hipStream_t st) {
   hipLaunchKernelGGL(k_keys, dim3(gs), dim3(kScanBlock), 0, st, d, g);
   launch_scan(d.col_count, d.col_start, ncol, d.scan_tmp, d.ctl, kGuardRebuild, st);
   hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(kScanBlock), 0, st, d, g, (int)kGuardRebuild);
-  hipLaunchKernelGGL(k_colsort, dim3((unsigned)std::min<int64_t>(ncol, 8192)), dim3(64), 0, st,
-                     d, g, (int)kGuardRebuild);
+  hipLaunchKernelGGL(k_colsort, dim3(colsort_grid(ncol)), dim3(64 * kColWaves), 0, st, d, g,
+                     (int)kGuardRebuild);
   const unsigned gl = std::min(nblocks(g.nb, 256), kListBlocks);
   hipLaunchKernelGGL(k_deg, dim3(gl), dim3(256), 0, st, d, g);
   launch_scan(d.deg, d.inc_start, g.nb, d.scan_tmp, d.ctl, kGuardRebuild, st);
@@ -819,7 +838,7 @@ __global__ __launch_bounds__(64) void k_group_colsort(const Member *__restrict__
   int b0, nblk;
   const Member &M = ms[group_member(gg, blockIdx.x, &b0, &nblk)];
   if (guard_off(M.d[0].ctl, kGuardRebuild)) return;
-  colsort_body(M.d[0], M.g, blockIdx.x - b0, nblk);
+  colsort_body<1>(M.d[0], M.g, blockIdx.x - b0, nblk);
 }
 
 __global__ __launch_bounds__(256) void k_group_deg(const Member *__restrict__ ms, GroupGrid gg) {
@@ -923,8 +942,7 @@ void launch_cells_from_keys(const Dev &d, const Geo &g, hipStream_t st) {
   hipLaunchKernelGGL(k_key_count, dim3(sgrid(g.nb)), dim3(256), 0, st, d, g);
   launch_scan(d.col_count, d.col_start, ncol, d.scan_tmp, nullptr, 0, st);
   hipLaunchKernelGGL(k_scatter, dim3(sgrid(g.nb)), dim3(256), 0, st, d, g, 0);
-  hipLaunchKernelGGL(k_colsort, dim3((unsigned)std::min<int64_t>(ncol, 8192)), dim3(64), 0, st,
-                     d, g, 0);
+  hipLaunchKernelGGL(k_colsort, dim3(colsort_grid(ncol)), dim3(64 * kColWaves), 0, st, d, g, 0);
 }
 
 // Replays the displacement bookkeeping of the last full step after the host
