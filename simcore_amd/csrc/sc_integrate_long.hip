@@ -317,7 +317,7 @@ __device__ __forceinline__ void long_body(const Dev &d, const Geo &g, bool facti
         o[q] = rnx[k][q] - rc[k][q];
         l += o[q] * o[q];
       }
-      const double il = rcp_fast(sqrt(l));
+      const double il = rsq_fast(l);
       for (int q = 0; q < 3; ++q) ub[k][q] = o[q] * il;
     }
     site_at3<W, S, -1>(ub, ubm, lane, gl, Gf);
@@ -346,7 +346,7 @@ __device__ __forceinline__ void long_body(const Dev &d, const Geo &g, bool facti
       for (int q = 0; q < 3; ++q) t[k][q] = u[k][q];
     } else {
       for (int q = 0; q < 3; ++q) t[k][q] = um[k][q] + u[k][q];
-      const double inv = rcp_fast(sqrt(dot(t[k], t[k])));
+      const double inv = rsq_fast(dot(t[k], t[k]));
       for (int q = 0; q < 3; ++q) t[k][q] *= inv;
     }
   }

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
       o[q] = d_next(rc[q]) - rc[q];
       l += o[q] * o[q];
     }
-    const double il = rcp_fast(sqrt(l));
+    const double il = rsq_fast(l);
     for (int q = 0; q < 3; ++q) ub[q] = o[q] * il;
     for (int q = 0; q < 3; ++q) {
       const double pv = d_prev(ub[q]);
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
     for (int q = 0; q < 3; ++q) t[q] = u[q];
   } else {
     for (int q = 0; q < 3; ++q) t[q] = um[q] + u[q];
-    const double inv = rcp_fast(sqrt(dot(t, t)));
+    const double inv = rsq_fast(dot(t, t));
     for (int q = 0; q < 3; ++q) t[q] *= inv;
   }
   // ---- random forces (midstep): construct and project ------------------

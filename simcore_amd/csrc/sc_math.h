@@ -11,18 +11,25 @@
 
 namespace sc {
 
-// 1/x for the mixed-precision path: the v_rcp_f64 estimate refined by two
-// Newton steps, e = 1 - x r, r += r e (error squared per step), within about
-// an ulp of the correctly rounded reciprocal.  The IEEE division sequence
-// adds operand scaling and a final rounding fix-up, about twice the
-// instructions; x here is a length, pivot or determinant of order one (0
-// gives a NaN instead of an infinity, in lanes whose values are unused).
+// 1/x for the mixed-precision path: the v_rcp_f64 estimate refined by one
+// Newton step, e = 1 - x r, r += r e (the error squared).  The IEEE
+// division sequence adds a second step, operand scaling and a final
+// rounding fix-up, about three times the instructions; x here is a length,
+// pivot or determinant of order one (0 gives a NaN instead of an infinity,
+// in lanes whose values are unused).
 __device__ __forceinline__ double rcp_fast(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, r, 1.0);
   return fma(r, e, r);
+}
+
+// 1/sqrt(x) for the mixed-precision path (x a squared length of order
+// one): the v_rsq_f64 estimate refined by one Newton step,
+// r += r (1 - x r^2) / 2, in place of an IEEE square root and a reciprocal.
+__device__ __forceinline__ double rsq_fast(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * r, r, 1.0);
+  return fma(0.5 * r, e, r);
 }
 
 // NINT (include/simcore/macros.hpp:6): round half away from zero to int.
