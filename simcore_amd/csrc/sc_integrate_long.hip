@@ -579,8 +579,9 @@ __device__ __forceinline__ void long_body(const Dev &d, const Geo &g, bool facti
     const int i = base + k;
     double rd[3];
     for (int q = 0; q < 3; ++q) rd[q] = rnn[k][q] - rn[k][q];
-    mm[k] = sqrt(dot(rd, rd));
-    const double inv = rcp_fast(mm[k]);
+    const double dd = dot(rd, rd);
+    const double inv = rsq_fast(dd);
+    mm[k] = dd * inv;
     for (int q = 0; q < 3; ++q) uo[k][q] = rd[q] * inv;
     if (i < m) {
       const double diff = bl - mm[k];
@@ -649,8 +650,9 @@ __device__ __forceinline__ void long_body(const Dev &d, const Geo &g, bool facti
     for (int k = 0; k < S; ++k) {
       double rd[3];
       for (int q = 0; q < 3; ++q) rd[q] = rnn[k][q] - rn[k][q];
-      mm[k] = sqrt(dot(rd, rd));
-      const double inv = rcp_fast(mm[k]);
+      const double dd = dot(rd, rd);
+      const double inv = rsq_fast(dd);
+      mm[k] = dd * inv;
       for (int q = 0; q < 3; ++q) uo[k][q] = rd[q] * inv;
     }
     if (need && gl == 0 && factive) {

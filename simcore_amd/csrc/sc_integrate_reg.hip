@@ -246,8 +246,9 @@ __global__ __launch_bounds__(256) void k_integrate_reg(Dev d, Geo g, const int32
   {
     double rd[3];
     for (int q = 0; q < 3; ++q) rd[q] = rnn[q] - rn[q];
-    const double mm = sqrt(dot(rd, rd));
-    const double inv = rcp_fast(mm);
+    const double dd = dot(rd, rd);
+    const double inv = rsq_fast(dd);
+    const double mm = dd * inv;
     for (int q = 0; q < 3; ++q) uo[q] = rd[q] * inv;
     if (i < m) {
       const double diff = bl - mm;
