@@ -78,8 +78,13 @@ __host__ __device__ __forceinline__ uint32_t fix32(double f, double scale) {
 // over the diagonals) and back
 __host__ __device__ __forceinline__ void oct_encode(const double *u, uint32_t *ox,
                                                     uint32_t *oy) {
-  const double a = fabs(u[0]) + fabs(u[1]) + fabs(u[2]);
+  const double a = fabs(u[0]) + fabs(u[1]) + fabs(u[2]);  // in [1, sqrt(3)]
+#ifdef __HIP_DEVICE_COMPILE__
+  const double ia = rcp_fast(a);
+  double px = u[0] * ia, py = u[1] * ia;
+#else
   double px = u[0] / a, py = u[1] / a;
+#endif
   if (u[2] < 0.0) {
     const double qx = (1.0 - fabs(py)) * (px >= 0.0 ? 1.0 : -1.0);
     const double qy = (1.0 - fabs(px)) * (py >= 0.0 ? 1.0 : -1.0);
