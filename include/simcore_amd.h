@@ -276,7 +276,7 @@ int sc_get_launch_shape(sc_handle h, sc_launch_shape *shape);
  * width; the sites per lane are kept), then the overlap of the two over
  * bond ranges, and the early integration of the long filaments -- over
  * `steps` steps each from a checkpoint of the current state; the four
- * fastest are timed once more, each scored by the faster of its two
+ * fastest are timed twice more, each scored by the fastest of its three
  * timings; keeps the best and restores the state, so the trajectory is the
  * one without tuning.  The cell size
  * is not tuned: in the reference it sets the rebuild threshold and the

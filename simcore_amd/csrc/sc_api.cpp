@@ -2165,8 +2165,9 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
     t.parts = 1;
     consider(t);
   }
-  // the four fastest shapes are timed once more and each scored by the
-  // faster of its two timings (a trial is only ever slowed by noise)
+  // the four fastest shapes are timed twice more, interleaved, and each
+  // scored by the fastest of its three timings (a trial is only ever
+  // slowed by noise)
   if (rc == SC_OK && seen.size() > 1) {
     auto same = [](const sc::Tune &a, const sc::Tune &b) {
       return a.pair_block == b.pair_block && a.pair_xcd == b.pair_xcd &&
@@ -2184,13 +2185,14 @@ int sc_autotune(sc_handle h, int32_t steps, sc_tune_report *report) {
       if (!dup) top.push_back(e);
       if (top.size() == 4) break;
     }
-    for (auto &e : top) {
-      if (rc) break;
-      double ms = 0;
-      rc = trial(e.second, &ms);
-      ++trials;
-      if (rc == SC_OK && ms < e.first) e.first = ms;
-    }
+    for (int round = 0; round < 2; ++round)
+      for (auto &e : top) {
+        if (rc) break;
+        double ms = 0;
+        rc = trial(e.second, &ms);
+        ++trials;
+        if (rc == SC_OK && ms < e.first) e.first = ms;
+      }
     if (rc == SC_OK) {
       const auto fastest = std::min_element(
           top.begin(), top.end(),

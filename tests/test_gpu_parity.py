@@ -380,8 +380,8 @@ def test_autotune_leaves_trajectory_unchanged(precision):
     # overlapped bond ranges; the mixed path's packed integrator is tried as
     # one launch (no side stream: no filament above 128 sites) and as one
     # launch per lane-group width (two here, 16- and 32-lane groups: one
-    # side stream); then the four fastest shapes are timed again
-    expect = (23 if precision == "fp64" else 8 + 3 * 1 + 3 * 2 + 3) + 4
+    # side stream); then the four fastest shapes are timed twice more
+    expect = (23 if precision == "fp64" else 8 + 3 * 1 + 3 * 2 + 3) + 8
     assert rep["n_trials"] == expect and rep["step_ms_best"] <= rep["step_ms_start"]
     assert b.counters()["i_step"] == 31
     a.step(40)
