@@ -711,8 +711,10 @@ __device__ __forceinline__ void pull_one(const Dev &d, const Geo &g, int32_t b,
     if (c == 13) {
       // own cell (members in bond order): self pairs, then the rows of the
       // half-shell blocks
+      // (a cell of one member holds b alone: no load needed)
       int32_t r = own.x;
-      while (r < own.y && d.sorted[r] != b) ++r;
+      if (own.y - own.x > 1)
+        while (r < own.y && d.sorted[r] != b) ++r;
       for (int32_t j = own.x; j < r; ++j) put(d.sorted[j], 1);
       for (int32_t j = r + 1; j < own.y; ++j) put(d.sorted[j], 0);
       for (uint32_t mm = m & 0x1fffu; mm; mm &= mm - 1) {
