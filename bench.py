@@ -13,8 +13,9 @@ step replayed from a hipGraph, launch shapes autotuned.  Synthetic initial
 state (reference-style random insertion), state resident in HBM before the
 timed region.
 
-Printed on one JSON line: the metric; the roofline of the pair kernel (the
-step's largest kernel) and of the integrator, from HIP-event timing on the
+Printed on one JSON line: the metric; the roofline of the step's dominant
+kernel by measured time (the integrator on both large systems), with the
+pair kernel's and the integrator's beside it, from HIP-event timing on the
 system's stream; the per-bond force error of the mixed path against the
 FP64 path from identical state; the plain FP64 GPU path on the same
 workload (eager and graph-replayed); and the CPU oracle (OpenMP, the
